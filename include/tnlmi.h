@@ -1,0 +1,204 @@
+/*
+ * tnlmi.h -- C ABI of the MI355X (gfx950) sparse matrix-vector product and
+ * Krylov engine.  Plain C: pointers, sizes and enums only.
+ *
+ * The reference (TNL, src/TNL/...) exposes this path as C++ templates selected
+ * by a Device tag.  Each entry point below names the reference interface it
+ * replaces.  A C++ template layer (tnl_amd/include/TNL/...) routes TNL's
+ * SparseMatrix / Segments / Solvers::Linear calls for the new device to these
+ * functions and turns a non-OK status back into the exception type the
+ * reference throws (tnlmi_status values are listed with that type).
+ *
+ * Memory: every array argument named d_* is device memory on the current HIP
+ * device, owned by the caller unless stated otherwise.  Segment objects own
+ * the device arrays they build.  Calls taking a stream are asynchronous with
+ * respect to the host unless stated otherwise; results are visible after the
+ * stream is synchronised (the reference's reduceSegments synchronises before
+ * returning, Algorithms/Segments/detail/ReducingOperations_CSR.h:327 -- the C++
+ * layer does so too).
+ */
+#ifndef TNLMI_H
+#define TNLMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* tnlmi_stream; /* a hipStream_t; NULL = the null stream */
+
+/* Status codes.  Each names the exception the reference throws in that case. */
+typedef enum {
+   TNLMI_OK = 0,
+   TNLMI_INVALID_ARGUMENT = 1, /* std::invalid_argument (Matrices/SparseMatrixBase.hpp:279-282) */
+   TNLMI_RUNTIME_ERROR = 2,    /* std::runtime_error (Segments/detail/ReducingOperations_CSR.h:108-109) */
+   TNLMI_OVERFLOW_ERROR = 3,   /* std::overflow_error (Segments/Ellpack.hpp:91-95) */
+   TNLMI_BACKEND_ERROR = 4,    /* Exceptions::BackendRuntimeError (Backend/Macros.h:10-39) */
+   TNLMI_NOT_IMPLEMENTED = 5,  /* Exceptions::NotImplementedError */
+   TNLMI_BAD_ALLOC = 6,        /* Exceptions::BackendBadAlloc */
+   TNLMI_LOGIC_ERROR = 7       /* std::logic_error (e.g. GMRES without a matrix, GMRES.hpp:75) */
+} tnlmi_status;
+
+/* Message of the last non-OK status returned on the calling thread. */
+const char* tnlmi_last_error( void );
+/* Library version string and the gfx target it was built for. */
+const char* tnlmi_version( void );
+
+typedef enum { TNLMI_F64 = 0, TNLMI_F32 = 1, TNLMI_BF16 = 2 } tnlmi_dtype;
+
+/* Segment formats (the classes of Algorithms/Segments/). */
+typedef enum {
+   TNLMI_CSR = 0,             /* Segments/CSR.h */
+   TNLMI_ADAPTIVE_CSR = 1,    /* Segments/AdaptiveCSR.h */
+   TNLMI_ELLPACK = 2,         /* Segments/Ellpack.h (Alignment 32) */
+   TNLMI_SLICED_ELLPACK = 3,  /* Segments/SlicedEllpack.h (param = SliceSize) */
+   TNLMI_BIELLPACK = 4,       /* Segments/BiEllpack.h (param = WarpSize, 32) */
+   TNLMI_CHUNKED_ELLPACK = 5, /* Segments/ChunkedEllpack.h (256 chunks/slice, chunk 16) */
+   TNLMI_SORTED_CSR = 6       /* Segments/SortedSegments.h over CSR */
+} tnlmi_format;
+
+/* Segments/ElementsOrganization.h:13-19 (same numeric values). */
+typedef enum { TNLMI_COLUMN_MAJOR = 0, TNLMI_ROW_MAJOR = 1 } tnlmi_organization;
+
+/* ------------------------------------------------------------ segments */
+
+typedef struct tnlmi_segments tnlmi_segments;
+
+/* Builds the format's index arrays from per-segment sizes (the format
+ * converter).  Replaces <Format>::setSegmentsSizes(sizes), e.g.
+ * Segments/CSR.hpp:86-102, Ellpack.hpp:72-97, SlicedEllpack.hpp:116-152,
+ * BiEllpack.hpp:104-211, ChunkedEllpack.hpp:156-216, AdaptiveCSR.hpp:102-126,
+ * SortedSegments.hpp:129-200.  param: SliceSize for SlicedEllpack (power of
+ * two, default 32), WarpSize for BiEllpack (32); ignored otherwise (0).
+ * d_sizes: n int32 on the device.  Synchronous (returns once built). */
+tnlmi_status tnlmi_segments_create( tnlmi_format format, tnlmi_organization organization, int32_t param,
+                                    const int32_t* d_sizes, int64_t n, tnlmi_stream stream,
+                                    tnlmi_segments** out );
+void tnlmi_segments_destroy( tnlmi_segments* segments );
+
+/* getSegmentCount() / getStorageSize() / getElementCount() */
+int64_t tnlmi_segments_count( const tnlmi_segments* segments );
+int64_t tnlmi_segments_storage_size( const tnlmi_segments* segments );
+int64_t tnlmi_segments_element_count( const tnlmi_segments* segments );
+tnlmi_format tnlmi_segments_format( const tnlmi_segments* segments );
+
+/* Index arrays of a segments object, for inspection (device pointers owned by
+ * the object).  which: see tnlmi_segments_array.  Returns
+ * TNLMI_INVALID_ARGUMENT if the format has no such array. */
+typedef enum {
+   TNLMI_ARR_OFFSETS = 0,             /* CSR / AdaptiveCSR / SortedCSR (embedded CSR) offsets, n+1 */
+   TNLMI_ARR_SLICE_OFFSETS = 1,       /* SlicedEllpack sliceOffsets, slices+1 */
+   TNLMI_ARR_SLICE_SEGMENT_SIZES = 2, /* SlicedEllpack sliceSegmentSizes, slices */
+   TNLMI_ARR_PERMUTATION = 3,         /* BiEllpack / Sorted segmentsPermutation, n */
+   TNLMI_ARR_GROUP_POINTERS = 4,      /* BiEllpack groupPointers, strips*(log2 W+1)+1 */
+   TNLMI_ARR_INVERSE_PERMUTATION = 5, /* Sorted inverseSegmentsPermutation, n */
+   TNLMI_ARR_CHUNK_SLICES = 6,        /* ChunkedEllpack slices {size,chunkSize,firstSegment,pointer}, 4*slices */
+   TNLMI_ARR_SEGMENT_TO_CHUNK = 7,    /* ChunkedEllpack segmentToChunkMapping, n */
+   TNLMI_ARR_SEGMENT_TO_SLICE = 8,    /* ChunkedEllpack segmentToSliceMapping, n */
+   TNLMI_ARR_SEGMENT_POINTERS = 9,    /* ChunkedEllpack segmentPointers, n+1 */
+   TNLMI_ARR_CHUNKS_TO_SEGMENTS = 10, /* ChunkedEllpack chunksToSegmentsMapping, slices*256 */
+   TNLMI_ARR_ADAPTIVE_BLOCKS = 11,    /* AdaptiveCSR descriptors, 6 int32 each (see below) */
+   TNLMI_ARR_SIZES = 12,              /* the segment sizes the object was built from, n */
+   TNLMI_ARR_TILE_ROWS = 13           /* CSR tile -> first row table of the tiled kernel */
+} tnlmi_segments_array;
+tnlmi_status tnlmi_segments_get_array( const tnlmi_segments* segments, tnlmi_segments_array which,
+                                       const int32_t** d_array, int64_t* length );
+/* Ellpack: segmentSize and alignedSize (EllpackBase.h). */
+tnlmi_status tnlmi_segments_ellpack_dims( const tnlmi_segments* segments, int32_t* segment_size,
+                                          int32_t* aligned_size );
+
+/* AdaptiveCSR descriptors (Segments/detail/CSRAdaptiveKernelBlockDescriptor.h)
+ * are exported as 6 int32 {type, firstSegment, blockSize, segmentsInBlock,
+ * warpIdx, warpsCount}, type LONG=0 STREAM=1 VECTOR=2, built for value size
+ * value_bytes (1..32) with the wave64 device semantics. */
+tnlmi_status tnlmi_segments_adaptive_blocks( const tnlmi_segments* segments, int32_t value_bytes,
+                                             const int32_t** d_blocks, int64_t* count );
+
+/* For every (segment r, local k < size[r]) write getGlobalIndex(r, k) into
+ * d_gidx[csr_offset(r) + k] (CSR-ordered, element-count entries).  Replaces
+ * <Format>Base::getGlobalIndex. */
+tnlmi_status tnlmi_segments_global_indexes( const tnlmi_segments* segments, int32_t* d_gidx, tnlmi_stream stream );
+
+/* -------------------------------------------------------------- matrix */
+
+/* A general sparse matrix: Matrices/SparseMatrix.h:41-56 with
+ * MatrixType = GeneralMatrix.  values / column_indexes have the segments'
+ * storage size; column index -1 marks a padding slot (MatrixBase.h:27). */
+typedef struct {
+   int64_t rows;
+   int64_t columns;
+   const tnlmi_segments* segments;
+   tnlmi_dtype value_type;
+   const void* values;
+   const int32_t* column_indexes;
+} tnlmi_matrix;
+
+/* Writes a CSR-ordered element list into the format storage: padding slots get
+ * column -1 and value 0 (SparseMatrix.hpp:316-322), element k of row r goes to
+ * getGlobalIndex(r, k) (RowView::setElement).  d_csr_offsets must be the
+ * exclusive scan of the segment sizes; column -1 inside a row is kept as
+ * padding.  Values are converted from csr_type to value_type (bf16: round to
+ * nearest even; f64 goes through f32 first).  Replaces SparseMatrix::setElements / forAllElements-based
+ * assembly. */
+tnlmi_status tnlmi_matrix_fill( const tnlmi_segments* segments, const int32_t* d_csr_offsets,
+                                const int32_t* d_csr_columns, const void* d_csr_values, tnlmi_dtype csr_type,
+                                void* d_values, tnlmi_dtype value_type, int32_t* d_column_indexes,
+                                tnlmi_stream stream );
+
+/* Segments/LaunchConfiguration.h: ThreadsToSegmentsMapping (same values) plus
+ * TNLMI_MAP_AUTO, the MI355X-tuned default (tiled nnz-balanced CSR). */
+typedef enum {
+   TNLMI_MAP_FIXED = 0,
+   TNLMI_MAP_WARP = 1,
+   TNLMI_MAP_BLOCK = 2,
+   TNLMI_MAP_BLOCK_MERGED = 3,
+   TNLMI_MAP_DYNAMIC_GROUPING = 4,
+   TNLMI_MAP_AUTO = 255
+} tnlmi_mapping;
+typedef struct {
+   int32_t mapping;             /* tnlmi_mapping */
+   int32_t threads_per_segment; /* power of two (LaunchConfiguration.h:36-42) */
+} tnlmi_launch;
+
+/* y = alpha*A*x + beta*y over rows [begin, end) (end == 0: all rows).
+ * Replaces SparseMatrixBase::vectorProduct (Matrices/SparseMatrixBase.hpp:
+ * 267-391) with its reduceSegments dispatch.  launch == NULL means the
+ * default (TNLMI_MAP_AUTO).  Supported (value, vector, compute) types:
+ * (f64,f64,f64), (f32,f32,f32), (f32,f64,f64), (bf16,f32,f32), (bf16,f64,f64).
+ * x has `columns` entries of x_type, y has `rows` entries of x_type. */
+tnlmi_status tnlmi_spmv( const tnlmi_matrix* A, tnlmi_dtype x_type, const void* d_x, void* d_y,
+                         tnlmi_dtype compute_type, double alpha, double beta, int64_t begin, int64_t end,
+                         const tnlmi_launch* launch, tnlmi_stream stream );
+
+/* ------------------------------------------------- input generators (host) */
+/* Seeded host-side generators of the workloads the engine is measured on.
+ * Random numbers come from a counter-based generator (splitmix64 of seed,
+ * stream and index), so every entry is reproducible independently of the
+ * thread count.  No reference counterpart (the reference tests build their
+ * matrices inline). */
+
+/* 2D 5-point Laplacian on an nx x ny grid, Dirichlet boundary: diagonal 4,
+ * neighbours -1, rows lexicographic (ix fastest), columns ascending.  Pass
+ * NULL arrays to get the nonzero count only.  Returns nnz. */
+int64_t tnlmi_gen_laplace2d( int64_t nx, int64_t ny, int32_t* offsets, int32_t* columns, double* values );
+/* 3D 7-point Laplacian on nx*ny*nz, diagonal 6, neighbours -1. */
+int64_t tnlmi_gen_poisson3d( int64_t nx, int64_t ny, int64_t nz, int32_t* offsets, int32_t* columns, double* values );
+/* Row lengths ~ Zipf(s) on [1, kmax], scaled to mean `mean` and clipped to
+ * [lo, hi]; the scale is bisected and the remaining difference is spread by
+ * +-1 in row order so that the sum is exactly round(mean * n). */
+void tnlmi_gen_zipf_sizes( int64_t n, double s, int32_t kmax, double mean, int32_t lo, int32_t hi, uint64_t seed,
+                           int32_t* sizes );
+/* Sorted column indexes drawn uniformly from [0, ncols) without repeats in a
+ * row, for CSR offsets `offsets` (n+1 entries). */
+void tnlmi_gen_random_columns( int64_t n, int64_t ncols, const int32_t* offsets, uint64_t seed, int32_t* columns );
+/* out[i] ~ U[lo, hi) or N(0, 1) for i < n, from (seed, stream). */
+void tnlmi_gen_uniform( int64_t n, double lo, double hi, uint64_t seed, uint64_t stream, double* out );
+void tnlmi_gen_normal( int64_t n, uint64_t seed, uint64_t stream, double* out );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TNLMI_H */

@@ -1,0 +1,276 @@
+"""Pins the CPU oracle (oracle/liboracle.so) before anything is checked against it.
+
+Pins, per oracle function:
+* SpMV: the reference's own known-answer tests (SparseMatrixVectorProductTest.hpp,
+  tests/golden/spmv_kats.json), a dense brute-force product on nonsymmetric
+  random matrices (catches transposed operands / wrong indices), the closed form
+  of the 2D Laplacian, and bit-exact agreement with TNL's Host path
+  (oracle/_ref) including alpha/beta/begin/end and padding slots.
+* Converters: bit-exact agreement with TNL's own setSegmentsSizes (oracle/_ref);
+  BiEllpack also against a loop-for-loop restatement of the cocktail sort.
+* Global indexes: the oracle's getGlobalIndex maps scatter a CSR matrix into
+  exactly the value/column arrays TNL's SparseMatrix builds for each format.
+* Solvers: TNL's CG / GMRES (all variants) / Jacobi / ILU(0) on the same inputs
+  (iteration counts and solutions), plus closed forms (ILU(0) of a tridiagonal
+  matrix is its exact LU factorisation; the 5x5 solver example solves to ones).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+RNG = np.random.default_rng(12345)
+
+
+def random_csr(n, ncols, maxlen, rng, pad=0.0, empty=0.1):
+    sizes = rng.integers(0, maxlen + 1, n).astype(np.int32)
+    sizes[rng.random(n) < empty] = 0
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum(sizes)
+    cols = np.full(off[-1], -1, np.int32)
+    vals = np.zeros(off[-1])
+    for r in range(n):
+        L = int(sizes[r])
+        used = L - int(rng.random() < pad and L > 0)
+        c = np.sort(rng.choice(ncols, size=min(used, ncols), replace=False))
+        cols[off[r]:off[r] + len(c)] = c
+        vals[off[r]:off[r] + len(c)] = rng.standard_normal(len(c))
+    return off, cols, vals, sizes
+
+
+# ------------------------------------------------------------------ SpMV
+@pytest.mark.parametrize("case", OL.load_kats(), ids=lambda c: c["name"])
+def test_spmv_kat(orc, case):
+    off, cols, vals, x, y, ncols = OL.kat_csr(case)
+    assert np.array_equal(orc.spmv(off, cols, vals, x), y)
+    assert np.array_equal(orc.spmv(off, cols, vals, x, dtype=np.float32), y.astype(np.float32))
+
+
+def test_spmv_dense_bruteforce(orc):
+    for n, m in [(1, 1), (7, 13), (40, 25), (100, 100)]:
+        off, cols, vals, _ = random_csr(n, m, min(m, 12), RNG, pad=0.3)
+        x = RNG.standard_normal(m)
+        A = OL.dense(off, cols, vals, m)
+        np.testing.assert_allclose(orc.spmv(off, cols, vals, x), A @ x, rtol=1e-13, atol=1e-13)
+        y0 = RNG.standard_normal(n)
+        np.testing.assert_allclose(orc.spmv(off, cols, vals, x, y=y0, alpha=-0.5, beta=2.0),
+                                   -0.5 * (A @ x) + 2.0 * y0, rtol=1e-13, atol=1e-13)
+
+
+def test_spmv_laplacian_closed_form(orc):
+    import tnl_amd
+    nx, ny = 37, 23
+    off, cols, vals = tnl_amd.laplace2d(nx, ny)
+    assert len(vals) == 5 * nx * ny - 2 * nx - 2 * ny
+    x = RNG.uniform(-1, 1, nx * ny)
+    X = x.reshape(ny, nx)
+    Y = 4 * X
+    Y[:, 1:] -= X[:, :-1]
+    Y[:, :-1] -= X[:, 1:]
+    Y[1:, :] -= X[:-1, :]
+    Y[:-1, :] -= X[1:, :]
+    np.testing.assert_allclose(orc.spmv(off, cols, vals, x), Y.ravel(), rtol=0, atol=1e-14)
+
+
+def test_spmv_bitexact_vs_reference(orc, ref):
+    for n, m in [(0, 3), (1, 5), (50, 60), (700, 500)]:
+        off, cols, vals, _ = random_csr(n, m, 40, RNG, pad=0.2)
+        x = RNG.standard_normal(m)
+        y0 = RNG.standard_normal(n)
+        for alpha, beta, b, e in [(1, 0, 0, 0), (2.5, 0, 0, 0), (1, -1.5, 0, 0), (0.5, 3.0, n // 3, n - n // 4)]:
+            if n == 0 and (b or e):
+                continue
+            yo = orc.spmv(off, cols, vals, x, y=y0, alpha=alpha, beta=beta, begin=b, end=e)
+            yr = ref.spmv("CSR", off, cols, vals, x, y=y0, alpha=alpha, beta=beta, begin=b, end=e)
+            assert np.array_equal(yo, yr)
+    off, cols, vals, _ = random_csr(300, 300, 30, RNG)
+    x = RNG.standard_normal(300)
+    assert np.array_equal(orc.spmv(off, cols, vals, x, dtype=np.float32), ref.spmv_f32(off, cols, vals, x))
+    assert np.array_equal(orc.spmv_omp(off, cols, vals, x, 4), orc.spmv(off, cols, vals, x))
+
+
+@pytest.mark.parametrize("fmt", ["CSR", "AdaptiveCSR", "Ellpack", "SlicedEllpack32", "SlicedEllpack64", "BiEllpack",
+                                 "ChunkedEllpack", "SortedCSR"])
+def test_reference_formats_agree_on_kats(ref, fmt):
+    # the reference build itself, every format, against the reference's test constants
+    for case in OL.load_kats()[:7]:
+        off, cols, vals, x, y, ncols = OL.kat_csr(case)
+        for cm in (True, False):
+            assert np.array_equal(ref.spmv(fmt, off, cols, vals, x, col_major=cm), y), (fmt, case["name"], cm)
+
+
+# ------------------------------------------------------------ converters
+SIZE_CASES = [np.array([], np.int32), np.array([0]), np.array([5]), np.array([1, 2, 1, 1]),
+              np.arange(31) % 7, np.arange(32) % 5, np.arange(33) % 9, np.full(64, 3),
+              RNG.integers(0, 40, 1000), np.concatenate([RNG.integers(0, 20, 900), [700, 0, 3000]]),
+              RNG.integers(0, 3, 5001), np.zeros(100, np.int64)]
+
+
+@pytest.mark.parametrize("k", range(len(SIZE_CASES)))
+def test_converters_vs_reference(orc, ref, k):
+    sizes = OL.i32(SIZE_CASES[k])
+    assert np.array_equal(orc.csr_offsets(sizes), ref.csr_offsets(sizes))
+    for cm in (True, False):
+        assert orc.ellpack_dims(sizes, cm) == ref.ellpack_dims(sizes, cm)
+    for S in (32, 64):
+        o1, s1 = orc.sliced(sizes, S)
+        o2, s2 = ref.sliced(sizes, S)
+        assert np.array_equal(o1, o2) and np.array_equal(s1, s2)
+    p1, g1 = orc.biellpack(sizes)
+    p2, g2 = ref.biellpack(sizes)
+    assert np.array_equal(p1, p2) and np.array_equal(g1, g2)
+    if len(sizes) <= 1000:
+        p3, g3 = orc.biellpack(sizes, brute=True)
+        assert np.array_equal(p1, p3) and np.array_equal(g1, g3)
+    c1, c2 = orc.chunked(sizes), ref.chunked(sizes)
+    assert c1["nslices"] == c2["nslices"] and c1["storage"] == c2["storage"]
+    for key in ["slices", "seg_to_chunk", "seg_to_slice", "seg_ptrs", "chunks_to_seg"]:
+        assert np.array_equal(c1[key], c2[key]), key
+    off = orc.csr_offsets(sizes)
+    for vs in (4, 8):
+        rb = ref.adaptive_blocks(sizes, vs)
+        ob = orc.adaptive_blocks(off, vs, device_warp=32)
+        if rb is None:  # LONG rows: pinned by test_adaptive_long_padding_wave64 instead
+            assert (ob[:, 0] == 0).any()
+            continue
+        assert np.array_equal(ob, rb)
+    ss, inv, perm = orc.sorted_segments(sizes)
+    eo, inv2, perm2 = ref.sorted_segments(sizes)
+    assert np.array_equal(inv, inv2) and np.array_equal(perm, perm2)
+    assert np.array_equal(orc.csr_offsets(ss), eo)
+
+
+def test_ellpack_overflow(orc):
+    # width 4096 x 4,194,304 rows overflows int32 (Ellpack.hpp:91-95)
+    sizes = np.full(2**22, 1, np.int32)
+    sizes[0] = 4096
+    assert orc.ellpack_dims(sizes)[0] == 3
+
+
+def test_adaptive_long_padding_wave64(orc):
+    # LONG descriptors of one row are padded to the end of a 256-thread block:
+    # 4 descriptors per block of 64-wide waves, 8 of 32-wide.
+    sizes = np.array([5] * 10 + [5000] + [5] * 10, np.int32)
+    off = orc.csr_offsets(sizes)
+    b64 = orc.adaptive_blocks(off, 8, device_warp=64)
+    b32 = orc.adaptive_blocks(off, 8, device_warp=32)
+    long64 = b64[b64[:, 0] == 0]
+    long32 = b32[b32[:, 0] == 0]
+    assert len(long64) == 3 and list(long64[:, 4]) == [0, 1, 2] and set(long64[:, 5]) == {3}
+    assert len(long32) == 7 and set(long32[:, 5]) == {7}
+
+
+@pytest.mark.parametrize("fmt", ["Ellpack", "SlicedEllpack32", "SlicedEllpack64", "BiEllpack", "ChunkedEllpack",
+                                 "SortedCSR", "CSR"])
+@pytest.mark.parametrize("cm", [True, False])
+def test_global_index_vs_reference_fill(orc, ref, fmt, cm):
+    for n, m, maxlen in [(1, 4, 3), (45, 60, 12), (333, 400, 50)]:
+        off, cols, vals, sizes = random_csr(n, m, maxlen, RNG)
+        kw = {}
+        name = fmt
+        if fmt.startswith("SlicedEllpack"):
+            kw["slice_size"] = int(fmt[-2:])
+            name = "SlicedEllpack"
+        g = orc.gidx(name, sizes, col_major=cm, **kw)
+        rc, rv = ref.fill(fmt, off, cols, vals, m, col_major=cm)
+        oc = np.full(len(rc), -1, np.int32)
+        ov = np.zeros(len(rc))
+        oc[g] = cols
+        ov[g] = vals
+        assert np.array_equal(oc, rc) and np.array_equal(ov, rv), (fmt, n)
+
+
+# ---------------------------------------------------------------- solvers
+def lap(nx, ny):
+    import tnl_amd
+    return tnl_amd.laplace2d(nx, ny)
+
+
+def nonsym(n, rng):
+    # diagonally dominant convection-diffusion-like tridiagonal + random couplings
+    off = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for i in range(n):
+        row = {i: 4.0}
+        if i > 0:
+            row[i - 1] = -1.3
+        if i < n - 1:
+            row[i + 1] = -0.7
+        j = int(rng.integers(0, n))
+        if j != i:
+            row[j] = row.get(j, 0.0) + 0.2 * rng.standard_normal()
+        for c in sorted(row):
+            cols.append(c)
+            vals.append(row[c])
+        off[i + 1] = len(cols)
+    return off, np.array(cols, np.int32), np.array(vals)
+
+
+def test_cg_vs_reference(orc, ref):
+    off, cols, vals = lap(20, 17)
+    b = RNG.standard_normal(len(off) - 1)
+    for pre in (0, 1):
+        xo, so = orc.cg(off, cols, vals, b, precond=pre, tol=1e-10)
+        xr, sr = ref.cg(off, cols, vals, b, precond=pre, tol=1e-10)
+        assert so.converged and sr.converged
+        assert abs(so.iterations - sr.iterations) <= 1
+        np.testing.assert_allclose(xo, xr, rtol=1e-8, atol=1e-10)
+        assert orc.true_residual(off, cols, vals, xo, b) < 1e-9
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_gmres_vs_reference(orc, ref, variant):
+    off, cols, vals = nonsym(300, np.random.default_rng(7))
+    b = np.random.default_rng(8).standard_normal(300)
+    for pre, rmin, rmax in [(0, 10, 10), (1, 30, 30), (2, 5, 20)]:
+        xo, so = orc.gmres(off, cols, vals, b, precond=pre, variant=variant, rmin=rmin, rmax=rmax, tol=1e-10)
+        xr, sr = ref.gmres(off, cols, vals, b, precond=pre, variant=variant, rmin=rmin, rmax=rmax, tol=1e-10)
+        assert so.converged == sr.converged == 1
+        assert abs(so.iterations - sr.iterations) <= 1, (so.iterations, sr.iterations)
+        np.testing.assert_allclose(xo, xr, rtol=1e-7, atol=1e-9)
+
+
+def test_jacobi_vs_reference(orc, ref):
+    off, cols, vals = nonsym(200, np.random.default_rng(3))
+    b = np.random.default_rng(4).standard_normal(200)
+    xo, so = orc.jacobi(off, cols, vals, b, tol=1e-9)
+    xr, sr = ref.jacobi(off, cols, vals, b, tol=1e-9)
+    assert so.iterations == sr.iterations and so.converged == sr.converged == 1
+    np.testing.assert_allclose(xo, xr, rtol=1e-12, atol=1e-13)
+
+
+def test_ilu0_tridiagonal_is_exact_lu(orc, ref):
+    # ILU(0) of a tridiagonal matrix is its LU factorisation (Thomas algorithm)
+    n = 50
+    rng = np.random.default_rng(5)
+    a, d, c = rng.uniform(-1, 0, n), rng.uniform(3, 4, n), rng.uniform(-1, 0, n)
+    off = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for i in range(n):
+        for j, v in [(i - 1, a[i]), (i, d[i]), (i + 1, c[i])]:
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(v)
+        off[i + 1] = len(cols)
+    cols, vals = np.array(cols, np.int32), np.array(vals)
+    b = rng.standard_normal(n)
+    L, U = orc.ilu0(off, cols, vals)
+    x = orc.ilu0_solve(L, U, b)
+    np.testing.assert_allclose(OL.dense(off, cols, vals, n) @ x, b, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(x, ref.ilu0_apply(off, cols, vals, b), rtol=1e-14, atol=1e-14)
+    # and on a 2D Laplacian (not exact) it equals the reference's ILU(0)
+    off, cols, vals = lap(9, 7)
+    b = rng.standard_normal(63)
+    L, U = orc.ilu0(off, cols, vals)
+    np.testing.assert_allclose(orc.ilu0_solve(L, U, b), ref.ilu0_apply(off, cols, vals, b), rtol=1e-13, atol=1e-14)
+
+
+def test_solver_example_5x5(orc):
+    # Documentation/Examples/Solvers/Linear/IterativeLinearSolverExample.cpp: tridiag(-1, 2.5, -1), b = A*1
+    off = np.array([0, 2, 5, 8, 11, 13], np.int32)
+    cols = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4], np.int32)
+    vals = np.array([2.5, -1, -1, 2.5, -1, -1, 2.5, -1, -1, 2.5, -1, -1, 2.5])
+    b = orc.spmv(off, cols, vals, np.ones(5))
+    for x, st in [orc.cg(off, cols, vals, b, tol=1e-12), orc.gmres(off, cols, vals, b, tol=1e-12)]:
+        assert st.converged
+        np.testing.assert_allclose(x, np.ones(5), rtol=1e-10)

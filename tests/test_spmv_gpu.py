@@ -1,0 +1,182 @@
+"""SpMV parity on the GPU, through the C ABI, against the pinned oracle.
+
+* every known-answer case of the reference's SparseMatrixVectorProductTest.hpp,
+  for every format, organisation and launch configuration: exact;
+* the 2D Laplacian 512x512 (x ~ U[-1,1], seed 0): FP64 relative error <= 1e-12;
+* random matrices with padding slots, alpha/beta and row ranges: <= 1e-12;
+* FP32 and BF16-stored values: against the oracle in FP32/FP64 with the
+  tolerance stated per case;
+* the full power-law matrix (4,194,304 rows, 67M nonzeros) FP64: <= 1e-12
+  against the oracle's OpenMP product;
+* error behaviour of the reference (size mismatch, invalid threads per segment).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+tnl = pytest.importorskip("tnl_amd")
+
+FORMATS = [(tnl.CSR, 0), (tnl.ADAPTIVE_CSR, 0), (tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32),
+           (tnl.SLICED_ELLPACK, 64), (tnl.BIELLPACK, 32), (tnl.CHUNKED_ELLPACK, 0), (tnl.SORTED_CSR, 0)]
+FMT_IDS = [f"{tnl.FORMAT_NAMES[f]}{p or ''}" for f, p in FORMATS]
+CSR_LAUNCHES = ([(tnl.AUTO, 1), (tnl.WARP, 1), (tnl.BLOCK, 1), (tnl.BLOCK_MERGED, 1)]
+                + [(tnl.FIXED, t) for t in (1, 2, 4, 8, 16, 32, 64, 128, 256)]
+                + [(tnl.DYNAMIC_GROUPING, t) for t in (1, 4, 32, 256)])
+
+
+def gpu(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def run(A, x, y0=None, dtype=torch.float64, **kw):
+    xd = gpu(x, dtype)
+    yd = gpu(y0, dtype) if y0 is not None else torch.zeros(A.rows, dtype=dtype, device="cuda")
+    A.vectorProduct(xd, yd, **kw)
+    torch.cuda.synchronize()
+    return yd.double().cpu().numpy()
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) if len(b) else 0.0
+
+
+@pytest.mark.parametrize("fmt,param", FORMATS, ids=FMT_IDS)
+@pytest.mark.parametrize("org", [tnl.COLUMN_MAJOR, tnl.ROW_MAJOR], ids=["cm", "rm"])
+def test_kats_all_formats(fmt, param, org):
+    for case in OL.load_kats():
+        off, cols, vals, x, y, ncols = OL.kat_csr(case)
+        A = tnl.SparseMatrix.from_csr(len(off) - 1, ncols, off, cols, vals, fmt=fmt, param=param, organization=org)
+        launches = CSR_LAUNCHES if fmt == tnl.CSR else [None]
+        for launch in launches:
+            got = run(A, x, np.full(len(y), -1.0), launch=launch)
+            assert np.array_equal(got, y), (case["name"], launch)
+        if case["rows"] <= 8:  # small integers: exact in FP32 and in BF16 storage
+            A32 = tnl.SparseMatrix.from_csr(len(off) - 1, ncols, off, cols, vals, fmt=fmt, param=param,
+                                            organization=org, value_dtype=torch.float32)
+            assert np.array_equal(run(A32, x, dtype=torch.float32), y), case["name"]
+            Ab = tnl.SparseMatrix.from_csr(len(off) - 1, ncols, off, cols, vals, fmt=fmt, param=param,
+                                           organization=org, value_dtype=torch.bfloat16)
+            assert np.array_equal(run(Ab, x, dtype=torch.float32), y), case["name"]
+
+
+@pytest.mark.parametrize("fmt,param", FORMATS, ids=FMT_IDS)
+def test_laplacian_512(orc, fmt, param):
+    off, cols, vals = tnl.laplace2d(512, 512)
+    n = len(off) - 1
+    x = tnl.uniform(n, -1.0, 1.0, seed=0)
+    y_ref = orc.spmv(off, cols, vals, x)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+    assert relerr(run(A, x), y_ref) <= 1e-12
+    if fmt == tnl.CSR:
+        for launch in CSR_LAUNCHES:
+            assert relerr(run(A, x, launch=launch), y_ref) <= 1e-12, launch
+
+
+def random_csr(n, m, maxlen, rng, pad=0.2, long_rows=()):
+    sizes = rng.integers(0, maxlen + 1, n).astype(np.int32)
+    for r, L in long_rows:
+        sizes[r] = L
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum(sizes)
+    cols = np.full(off[-1], -1, np.int32)
+    vals = np.zeros(off[-1])
+    for r in range(n):
+        used = int(sizes[r]) - int(rng.random() < pad and sizes[r] > 0)
+        c = np.sort(rng.choice(m, size=min(used, m), replace=False))
+        cols[off[r]:off[r] + len(c)] = c
+        vals[off[r]:off[r] + len(c)] = rng.standard_normal(len(c))
+    return off, cols, vals
+
+
+@pytest.mark.parametrize("fmt,param", FORMATS, ids=FMT_IDS)
+@pytest.mark.parametrize("org", [tnl.COLUMN_MAJOR, tnl.ROW_MAJOR], ids=["cm", "rm"])
+def test_random_alpha_beta_range(orc, fmt, param, org):
+    rng = np.random.default_rng(fmt * 10 + org)
+    for n, m, maxlen, longs in [(1, 1, 1, ()), (97, 130, 20, ()), (3001, 2500, 40, [(5, 1500), (2000, 2400)])]:
+        off, cols, vals = random_csr(n, m, maxlen, rng, long_rows=longs)
+        A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, fmt=fmt, param=param, organization=org)
+        x = rng.standard_normal(m)
+        y0 = rng.standard_normal(n)
+        for alpha, beta, b, e in [(1, 0, 0, 0), (-2.0, 0, 0, 0), (1, 0.5, 0, 0), (0.75, -1.25, n // 3, n - n // 5)]:
+            want = orc.spmv(off, cols, vals, x, y=y0, alpha=alpha, beta=beta, begin=b, end=e)
+            got = run(A, x, y0, alpha=alpha, beta=beta, begin=b, end=e)
+            assert relerr(got, want) <= 1e-12, (n, alpha, beta, b, e)
+            if b or e:  # rows outside [begin, end) are untouched
+                assert np.array_equal(got[:b], y0[:b]) and np.array_equal(got[e:], y0[e:])
+
+
+def bf16_round(v):
+    """float64 -> float32 (nearest even) -> bfloat16 (nearest even), as float64."""
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("fmt,param", FORMATS, ids=FMT_IDS)
+def test_fp32_and_bf16(orc, fmt, param):
+    rng = np.random.default_rng(99)
+    n = 5000
+    off, cols, vals = random_csr(n, n, 30, rng, long_rows=[(17, 3000)])
+    x = rng.standard_normal(n)
+    lens = np.diff(off).astype(np.float64)
+    absrow = orc.spmv(off, cols, np.abs(vals), np.abs(x))
+    # FP32 storage, vectors and accumulation, any summation order:
+    # |error| <= (len + 3) * 2^-24 * sum_j |a_ij x_j| (products, sums, rounding of x)
+    tol32 = (lens + 3) * 2.0**-24 * absrow * 1.01 + 1e-30
+    y = orc.spmv(off, cols, vals, x)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param, value_dtype=torch.float32)
+    got = run(A, x.astype(np.float32), dtype=torch.float32)
+    assert np.all(np.abs(got - y) <= tol32 + np.abs(y) * 2.0**-24)
+    # BF16 storage (round to nearest even), FP32 vectors and accumulation
+    vb = bf16_round(vals)
+    x32 = x.astype(np.float32).astype(np.float64)
+    yb = orc.spmv(off, cols, vb, x32)
+    Ab = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param, value_dtype=torch.bfloat16)
+    got = run(Ab, x.astype(np.float32), dtype=torch.float32)
+    assert np.all(np.abs(got - yb) <= tol32 + np.abs(yb) * 2.0**-24)
+    # BF16 storage with FP64 vectors and accumulation: FP64 bar
+    got = run(Ab, x, dtype=torch.float64)
+    assert relerr(got, orc.spmv(off, cols, vb, x)) <= 1e-12
+
+
+def test_powerlaw_full_size(orc):
+    # the benchmark matrix at full size: 4,194,304 rows, mean 16 nonzeros per row
+    off, cols, vals = tnl.zipf_csr(4_194_304, seed=0)
+    n = len(off) - 1
+    assert int(off[-1]) == 16 * n
+    sizes = np.diff(off)
+    assert sizes.min() >= 1 and sizes.max() <= 4096
+    x = tnl.uniform(n, -1.0, 1.0, seed=0)
+    want = orc.spmv_omp(off, cols, vals, x, 16)
+    for fmt, param in [(tnl.CSR, 0), (tnl.SLICED_ELLPACK, 64), (tnl.BIELLPACK, 32), (tnl.CHUNKED_ELLPACK, 0),
+                       (tnl.ADAPTIVE_CSR, 0), (tnl.SORTED_CSR, 0)]:
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+        assert relerr(run(A, x), want) <= 1e-12, tnl.FORMAT_NAMES[fmt]
+        del A
+        torch.cuda.empty_cache()
+    # Ellpack is infeasible here: width 4096 x 4,194,304 rows overflows int32 (Ellpack.hpp:91-95)
+    with pytest.raises(OverflowError):
+        tnl.Segments(tnl.ELLPACK, sizes)
+
+
+def test_errors():
+    off, cols, vals = tnl.laplace2d(8, 8)
+    A = tnl.SparseMatrix.from_csr(64, 64, off, cols, vals)
+    x = torch.zeros(64, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        A.vectorProduct(torch.zeros(63, dtype=torch.float64, device="cuda"), x)
+    with pytest.raises(ValueError):
+        A.vectorProduct(x, torch.zeros(65, dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        A.vectorProduct(x, x.clone(), launch=(tnl.FIXED, 3))
+    with pytest.raises(tnl.TnlmiError):
+        A.vectorProduct(x, x.clone(), launch=(tnl.FIXED, 512))
+    with pytest.raises(ValueError):
+        A.vectorProduct(x, x.clone(), begin=10, end=5)

@@ -1,0 +1,273 @@
+"""tnl_amd -- MI355X (gfx950) sparse matrix-vector product and Krylov solvers.
+
+A Python mirror of the reference's operator interface for this path, over the
+C ABI of ``libtnlmi.so`` (include/tnlmi.h):
+
+* ``Segments``      -- Algorithms::Segments::<Format>, built on the GPU from
+                       segment sizes (``setSegmentsSizes``).
+* ``SparseMatrix``  -- Matrices::SparseMatrix<Real, GPU, int, GeneralMatrix,
+                       Segments>: ``vectorProduct(x, y, alpha, beta, begin,
+                       end, launch)``.
+* generators        -- the seeded workloads the engine is measured on.
+
+Device memory and streams come from PyTorch (plumbing only); every operation
+runs in the library's own HIP kernels.  Errors map to the reference's
+exception types (see ``_lib``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (InvalidArgument, LogicError, NotImplementedError_, OverflowError_, TnlmiError,  # noqa: F401
+                   BackendError, Launch)
+
+# formats (tnlmi_format)
+CSR, ADAPTIVE_CSR, ELLPACK, SLICED_ELLPACK, BIELLPACK, CHUNKED_ELLPACK, SORTED_CSR = range(7)
+FORMAT_NAMES = {CSR: "CSR", ADAPTIVE_CSR: "AdaptiveCSR", ELLPACK: "Ellpack", SLICED_ELLPACK: "SlicedEllpack",
+                BIELLPACK: "BiEllpack", CHUNKED_ELLPACK: "ChunkedEllpack", SORTED_CSR: "SortedCSR"}
+# ElementsOrganization
+COLUMN_MAJOR, ROW_MAJOR = 0, 1
+# dtypes (tnlmi_dtype)
+F64, F32, BF16 = 0, 1, 2
+# ThreadsToSegmentsMapping (+ AUTO)
+FIXED, WARP, BLOCK, BLOCK_MERGED, DYNAMIC_GROUPING, AUTO = 0, 1, 2, 3, 4, 255
+
+# segments arrays (tnlmi_segments_array)
+ARR = dict(offsets=0, slice_offsets=1, slice_segment_sizes=2, permutation=3, group_pointers=4,
+           inverse_permutation=5, chunk_slices=6, segment_to_chunk=7, segment_to_slice=8, segment_pointers=9,
+           chunks_to_segments=10, adaptive_blocks=11, sizes=12, tile_rows=13)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dtype_code(t):
+    torch = _torch()
+    if t == torch.float64:
+        return F64
+    if t == torch.float32:
+        return F32
+    if t == torch.bfloat16:
+        return BF16
+    raise InvalidArgument(_lib.INVALID_ARGUMENT, f"unsupported dtype {t}")
+
+
+def torch_dtype(code):
+    torch = _torch()
+    return {F64: torch.float64, F32: torch.float32, BF16: torch.bfloat16}[code]
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def version():
+    return _lib.lib().tnlmi_version().decode()
+
+
+class Segments:
+    """Algorithms::Segments::<Format> on the GPU (setSegmentsSizes on create)."""
+
+    def __init__(self, fmt, sizes, organization=COLUMN_MAJOR, param=0, stream=None):
+        torch = _torch()
+        sizes = torch.as_tensor(sizes, dtype=torch.int32)
+        if not sizes.is_cuda:
+            sizes = sizes.cuda()
+        sizes = sizes.contiguous()
+        self._sizes = sizes
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tnlmi_segments_create(fmt, organization, param, ctypes.c_void_p(sizes.data_ptr()),
+                                                    sizes.numel(), _stream(stream), ctypes.byref(h)))
+        self.handle = h
+        self.format = fmt
+        self.organization = organization
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.lib().tnlmi_segments_destroy(h)
+            self.handle = None
+
+    def getSegmentCount(self):
+        return _lib.lib().tnlmi_segments_count(self.handle)
+
+    def getStorageSize(self):
+        return _lib.lib().tnlmi_segments_storage_size(self.handle)
+
+    def getElementCount(self):
+        return _lib.lib().tnlmi_segments_element_count(self.handle)
+
+    def array(self, name):
+        """Copy of one of the format's index arrays (numpy int32)."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().tnlmi_segments_get_array(self.handle, ARR[name], ctypes.byref(p), ctypes.byref(n)))
+        return _d2h(p.value, n.value, np.int32)
+
+    def ellpack_dims(self):
+        w = ctypes.c_int32()
+        a = ctypes.c_int32()
+        _lib.check(_lib.lib().tnlmi_segments_ellpack_dims(self.handle, ctypes.byref(w), ctypes.byref(a)))
+        return w.value, a.value
+
+    def adaptive_blocks(self, value_bytes):
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().tnlmi_segments_adaptive_blocks(self.handle, value_bytes, ctypes.byref(p),
+                                                             ctypes.byref(n)))
+        return _d2h(p.value, 6 * n.value, np.int32).reshape(-1, 6)
+
+    def global_indexes(self):
+        torch = _torch()
+        out = torch.empty(max(1, self.getElementCount()), dtype=torch.int32, device="cuda")
+        _lib.check(_lib.lib().tnlmi_segments_global_indexes(self.handle, ctypes.c_void_p(out.data_ptr()),
+                                                            _stream()))
+        return out[: self.getElementCount()]
+
+
+_hip = None
+
+
+def _d2h(ptr, count, dtype):
+    """Synchronous copy of `count` elements at device address `ptr` to numpy."""
+    global _hip
+    out = np.zeros(count, dtype=dtype)
+    if count == 0:
+        return out
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = ctypes.c_int
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipDeviceSynchronize.restype = ctypes.c_int
+    _hip.hipDeviceSynchronize()
+    rc = _hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(out.nbytes), 2)
+    if rc != 0:
+        raise BackendError(_lib.BACKEND_ERROR, f"hipMemcpy failed ({rc})")
+    return out
+
+
+class SparseMatrix:
+    """Matrices::SparseMatrix<Real, GPU, int, GeneralMatrix, Segments> over device tensors."""
+
+    def __init__(self, rows, columns, segments, values, column_indexes):
+        self.rows = int(rows)
+        self.columns = int(columns)
+        self.segments = segments
+        self.values = values
+        self.column_indexes = column_indexes
+        self._m = _lib.Matrix(self.rows, self.columns, segments.handle.value, dtype_code(values.dtype),
+                              values.data_ptr(), column_indexes.data_ptr())
+
+    @classmethod
+    def from_csr(cls, rows, columns, offsets, cols, vals, fmt=CSR, value_dtype=None, organization=COLUMN_MAJOR,
+                 param=0, capacities=None):
+        """Assemble from CSR arrays (any device): row r holds cols/vals[offsets[r]:offsets[r+1]].
+
+        capacities (optional) are the row capacities (setRowCapacities);
+        by default the CSR row lengths.
+        """
+        torch = _torch()
+        offsets = torch.as_tensor(offsets, dtype=torch.int32).cuda().contiguous()
+        cols = torch.as_tensor(cols, dtype=torch.int32).cuda().contiguous()
+        vals = torch.as_tensor(vals).cuda().contiguous()
+        if capacities is None:
+            capacities = offsets[1:] - offsets[:-1]
+        seg = Segments(fmt, capacities, organization, param)
+        vdt = value_dtype if value_dtype is not None else vals.dtype
+        storage = max(1, seg.getStorageSize())
+        values = torch.empty(storage, dtype=vdt, device="cuda")
+        colidx = torch.empty(storage, dtype=torch.int32, device="cuda")
+        if vals.dtype not in (torch.float64, torch.float32):
+            vals = vals.double()
+        _lib.check(_lib.lib().tnlmi_matrix_fill(seg.handle, ctypes.c_void_p(offsets.data_ptr()),
+                                                ctypes.c_void_p(cols.data_ptr()), ctypes.c_void_p(vals.data_ptr()),
+                                                dtype_code(vals.dtype), ctypes.c_void_p(values.data_ptr()),
+                                                dtype_code(vdt), ctypes.c_void_p(colidx.data_ptr()), _stream()))
+        return cls(rows, columns, seg, values, colidx)
+
+    def getRows(self):
+        return self.rows
+
+    def getColumns(self):
+        return self.columns
+
+    def vectorProduct(self, x, y, alpha=1.0, beta=0.0, begin=0, end=0, launch=None, compute_dtype=None,
+                      stream=None):
+        """y = alpha*A*x + beta*y over rows [begin, end) (SparseMatrixBase::vectorProduct)."""
+        if x.numel() != self.columns:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "vectorProduct: size of the input vector does not match "
+                                                         "the number of matrix columns")
+        if y.numel() != self.rows:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "vectorProduct: size of the output vector does not match "
+                                                         "the number of matrix rows")
+        if x.dtype != y.dtype:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "vectorProduct: x and y must have the same dtype")
+        xt = dtype_code(x.dtype)
+        ct = dtype_code(compute_dtype) if compute_dtype is not None else xt
+        L = None
+        if launch is not None:
+            mapping, tps = launch
+            L = ctypes.byref(Launch(mapping, tps))
+        _lib.check(_lib.lib().tnlmi_spmv(ctypes.byref(self._m), xt, ctypes.c_void_p(x.data_ptr()),
+                                         ctypes.c_void_p(y.data_ptr()), ct, float(alpha), float(beta), int(begin),
+                                         int(end), L, _stream(stream)))
+        return y
+
+
+# ------------------------------------------------------------------ generators
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def laplace2d(nx, ny):
+    """2D 5-point Laplacian (offsets, columns, values) as numpy arrays."""
+    L = _lib.lib()
+    nnz = L.tnlmi_gen_laplace2d(nx, ny, None, None, None)
+    off = np.zeros(nx * ny + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float64)
+    L.tnlmi_gen_laplace2d(nx, ny, _ptr(off), _ptr(col), _ptr(val))
+    return off, col, val
+
+
+def poisson3d(nx, ny, nz):
+    L = _lib.lib()
+    nnz = L.tnlmi_gen_poisson3d(nx, ny, nz, None, None, None)
+    off = np.zeros(nx * ny * nz + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float64)
+    L.tnlmi_gen_poisson3d(nx, ny, nz, _ptr(off), _ptr(col), _ptr(val))
+    return off, col, val
+
+
+def zipf_csr(n, s=2.0, kmax=4096, mean=16.0, seed=0):
+    """The power-law matrix: n x n, row lengths ~ Zipf(s) on [1, kmax] scaled to
+    mean `mean` (clipped to [1, kmax]), uniform distinct sorted columns, N(0,1)
+    values."""
+    L = _lib.lib()
+    sizes = np.zeros(n, np.int32)
+    L.tnlmi_gen_zipf_sizes(n, s, kmax, mean, 1, kmax, seed, _ptr(sizes))
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum(sizes, dtype=np.int64)
+    col = np.zeros(int(off[-1]), np.int32)
+    L.tnlmi_gen_random_columns(n, n, _ptr(off), seed, _ptr(col))
+    val = np.zeros(int(off[-1]), np.float64)
+    L.tnlmi_gen_normal(len(val), seed, 7, _ptr(val))
+    return off, col, val
+
+
+def uniform(n, lo=-1.0, hi=1.0, seed=0, stream=11):
+    out = np.zeros(n, np.float64)
+    _lib.lib().tnlmi_gen_uniform(n, lo, hi, seed, stream, _ptr(out))
+    return out
+
+
+def normal(n, seed=0, stream=13):
+    out = np.zeros(n, np.float64)
+    _lib.lib().tnlmi_gen_normal(n, seed, stream, _ptr(out))
+    return out

@@ -1,0 +1,154 @@
+// segments.hpp -- the device-resident segment formats (TNL's Algorithms::
+// Segments) and their global-index functions.
+#pragma once
+
+#include "common.hpp"
+
+// The opaque handle of the C ABI.  Every format also keeps the CSR offsets of
+// its sizes (used only by assembly and the CSR-ordered index export, never by
+// the format's own SpMV kernels).
+struct tnlmi_segments
+{
+   tnlmi_format format = TNLMI_CSR;
+   tnlmi_organization org = TNLMI_COLUMN_MAJOR;
+   int32_t param = 0;
+   int64_t n = 0;
+   int64_t storage = 0;
+   int64_t elements = 0;
+   int32_t max_size = 0;
+
+   tnlmi::DeviceArray< int32_t > sizes;
+   tnlmi::DeviceArray< int32_t > offsets;  // exclusive scan of sizes, n+1
+
+   // CSR tiled kernel plan: tile t covers elements [t*kTileNnz, (t+1)*kTileNnz)
+   // and owns rows [tile_rows[t], tile_rows[t+1]), the rows whose first
+   // element lies there; carry_row / carry_val hold the partial sums of rows
+   // continued from an earlier tile (scratch of the SpMV: concurrent products
+   // with one segments object on different streams are not allowed).
+   tnlmi::DeviceArray< int32_t > tile_rows;
+   tnlmi::DeviceArray< int32_t > carry_row;
+   tnlmi::DeviceArray< double > carry_val;
+   int64_t tiles = 0;
+   int32_t tile_nnz = 0;
+
+   // Ellpack (EllpackBase.h)
+   int32_t ell_width = 0, ell_aligned = 0;
+
+   // SlicedEllpack (SlicedEllpackBase.h)
+   tnlmi::DeviceArray< int32_t > slice_offsets, slice_seg_sizes;
+   int64_t slices = 0;
+
+   // BiEllpack (BiEllpackBase.h) and SortedSegments permutations
+   tnlmi::DeviceArray< int32_t > perm, group_ptrs, inv_perm;
+   int64_t strips = 0;
+   int32_t logw = 5;
+
+   // ChunkedEllpack (ChunkedEllpackBase.h)
+   tnlmi::DeviceArray< int32_t > ch_slices, seg_to_chunk, seg_to_slice, seg_ptrs, chunks_to_seg;
+   int64_t ch_nslices = 0;
+   int32_t chunks_in_slice = 256, desired_chunk_size = 16;
+
+   // AdaptiveCSR descriptors per value size log (1,2,4,8,16,32 bytes)
+   tnlmi::DeviceArray< int32_t > blocks[ 6 ];
+   int64_t nblocks[ 6 ] = { 0, 0, 0, 0, 0, 0 };
+
+   // SortedSegments: CSR offsets of the sorted sizes (the embedded CSR)
+   tnlmi::DeviceArray< int32_t > sorted_offsets;
+};
+
+namespace tnlmi {
+
+// elements per wavefront tile of the tiled CSR kernel
+constexpr int32_t kTileNnz = 512;
+
+// Plain-value view of a segments object for kernels.
+struct SegView
+{
+   int32_t format, col_major;
+   int64_t n;
+   const int32_t* sizes;
+   const int32_t* offsets;
+   // ellpack
+   int32_t ell_width, ell_aligned;
+   // sliced
+   int32_t slice_size;
+   const int32_t* slice_offsets;
+   const int32_t* slice_seg_sizes;
+   // biellpack / sorted
+   const int32_t* perm;
+   const int32_t* group_ptrs;
+   const int32_t* inv_perm;
+   int32_t warp, logw;
+   // chunked
+   const int32_t* ch_slices;
+   const int32_t* seg_to_chunk;
+   const int32_t* seg_to_slice;
+   int32_t chunks_in_slice;
+   // sorted
+   const int32_t* sorted_offsets;
+};
+
+SegView make_view( const tnlmi_segments& s );
+
+// getGlobalIndex( r, k ) of every format (reference file per branch).
+__device__ inline int64_t global_index( const SegView& v, int64_t r, int32_t k )
+{
+   switch( v.format ) {
+      case TNLMI_CSR:
+      case TNLMI_ADAPTIVE_CSR:
+         return v.offsets[ r ] + k;  // CSRBase.hpp:117-131
+      case TNLMI_ELLPACK:            // EllpackBase.hpp:108-117
+         return v.col_major ? r + (int64_t) v.ell_aligned * k : r * v.ell_width + k;
+      case TNLMI_SLICED_ELLPACK: {  // SlicedEllpackBase.hpp:119-146
+         const int64_t s = r / v.slice_size, in = r % v.slice_size;
+         return v.col_major ? v.slice_offsets[ s ] + in + (int64_t) v.slice_size * k
+                            : v.slice_offsets[ s ] + in * v.slice_seg_sizes[ s ] + k;
+      }
+      case TNLMI_BIELLPACK: {  // detail/BiEllpack.h getGlobalIndexDirect
+         const int64_t strip = r / v.warp;
+         const int64_t g0 = strip * ( v.logw + 1 );
+         const int32_t rsp = v.perm[ r ] - (int32_t) ( strip * v.warp );
+         int groups = 1;
+         {
+            int32_t bis = 1;
+            for( int i = 0; i < v.logw + 1; i++ ) {
+               if( rsp < bis ) {
+                  groups = v.logw + 1 - i;
+                  break;
+               }
+               bis *= 2;
+            }
+         }
+         int32_t local = k, global = v.group_ptrs[ g0 ], height = v.warp;
+         for( int g = 0; g < groups; g++ ) {
+            const int32_t gsize = v.group_ptrs[ g0 + g + 1 ] - v.group_ptrs[ g0 + g ];
+            if( gsize ) {
+               const int32_t gw = gsize / height;
+               if( local >= gw ) {
+                  local -= gw;
+                  global += gsize;
+               }
+               else
+                  return v.col_major ? global + rsp + local * height : global + rsp * gw + local;
+            }
+            height /= 2;
+         }
+         return -1;
+      }
+      case TNLMI_CHUNKED_ELLPACK: {  // detail/ChunkedEllpack.h getGlobalIndexDirect
+         const int32_t si = v.seg_to_slice[ r ];
+         const int32_t first = r != v.ch_slices[ 4 * si + 2 ] ? v.seg_to_chunk[ r - 1 ] : 0;
+         const int32_t off = v.ch_slices[ 4 * si + 3 ], cs = v.ch_slices[ 4 * si + 1 ];
+         return v.col_major ? off + ( k % cs ) * v.chunks_in_slice + first + k / cs : off + first * cs + k;
+      }
+      case TNLMI_SORTED_CSR:  // SortedSegmentsBase.hpp:85-96
+         return v.sorted_offsets[ v.perm[ r ] ] + k;
+   }
+   return -1;
+}
+
+// Exclusive scan of int32 values into int32 (n+1 outputs, out[n] = total);
+// returns the total as int64 (checked by the caller for int32 overflow).
+int64_t exclusive_scan_i32( const int32_t* d_in, int32_t* d_out, int64_t n, hipStream_t s );
+
+}  // namespace tnlmi

@@ -1,0 +1,12 @@
+// spmv.hpp -- internal entry of the SpMV dispatch (used by the solvers).
+#pragma once
+
+#include "common.hpp"
+
+namespace tnlmi {
+
+// y = alpha*A*x + beta*y over rows [begin, end) on stream s (asynchronous).
+void spmv_dispatch( const tnlmi_matrix& A, tnlmi_dtype xt, const void* x, void* y, tnlmi_dtype ct, double alpha,
+                    double beta, int64_t begin, int64_t end, const tnlmi_launch& cfg, hipStream_t s );
+
+}  // namespace tnlmi
