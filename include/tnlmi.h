@@ -172,6 +172,93 @@ tnlmi_status tnlmi_spmv( const tnlmi_matrix* A, tnlmi_dtype x_type, const void* 
                          tnlmi_dtype compute_type, double alpha, double beta, int64_t begin, int64_t end,
                          const tnlmi_launch* launch, tnlmi_stream stream );
 
+/* ------------------------------------------------ XCD column-split plan */
+/* An execution plan of y = alpha*A*x + beta*y for a CSR matrix whose x
+ * gathers are random (power-law graphs): the columns are cut into `parts`
+ * ranges (8: one per XCD; 16: two per XCD) and a reordered copy of the CSR
+ * arrays is kept per range, so that every XCD gathers from one L2-sized slice
+ * of x; per-range partial row sums are added in range order.  Built once
+ * (synchronous); the plan owns its copy (the matrix may be freed), so value
+ * changes of the matrix need a new plan.  Results equal vectorProduct up to
+ * the FP summation order.  No reference counterpart (TNL's vectorProduct has
+ * no analysis phase); it serves SparseMatrix::vectorProduct of CSR matrices.
+ * Concurrent executions of one plan on different streams are not allowed
+ * (the plan holds the partial sums). */
+typedef struct tnlmi_spmv_plan tnlmi_spmv_plan;
+tnlmi_status tnlmi_spmv_plan_create( const tnlmi_matrix* A, int32_t parts, tnlmi_stream stream,
+                                     tnlmi_spmv_plan** out );
+void tnlmi_spmv_plan_destroy( tnlmi_spmv_plan* plan );
+tnlmi_status tnlmi_spmv_plan_execute( const tnlmi_spmv_plan* plan, tnlmi_dtype x_type, const void* d_x, void* d_y,
+                                      tnlmi_dtype compute_type, double alpha, double beta, int64_t begin, int64_t end,
+                                      tnlmi_stream stream );
+
+/* ------------------------------------------------------------- BLAS-1 */
+/* (a, b) and lpNorm(a, 2) of FP64 device vectors into *d_result (device
+ * memory, asynchronous; deterministic two-pass reduction).  Replace the
+ * host-returning (a, b) / lpNorm of Containers/Expressions/
+ * ExpressionTemplates.h:372-384, :442-460 (the C++ layer reads the scalar). */
+tnlmi_status tnlmi_dot( int64_t n, const double* d_a, const double* d_b, double* d_result, tnlmi_stream stream );
+tnlmi_status tnlmi_nrm2( int64_t n, const double* d_a, double* d_result, tnlmi_stream stream );
+/* y = alpha*x + beta*y (Containers/detail/VectorAssignment.h expressions) */
+tnlmi_status tnlmi_axpby( int64_t n, double alpha, const double* d_x, double beta, double* d_y,
+                          tnlmi_stream stream );
+
+/* ------------------------------------------------------- linear solvers */
+/* IterativeSolver settings (in) and results (out); defaults of
+ * Solvers/IterativeSolver.h:226-235: max 1e9, min 0, convergence 1e-6,
+ * divergence DBL_MAX. */
+typedef struct {
+   int64_t max_iterations;
+   int64_t min_iterations;
+   double convergence_residue;
+   double divergence_residue;
+   int64_t iterations; /* out: getIterations() */
+   double residue;     /* out: getResidue() (the solver's own residue) */
+   int32_t converged;  /* out: the bool LinearSolver::solve returns */
+} tnlmi_solver_state;
+
+typedef enum { TNLMI_PRECOND_NONE = 0, TNLMI_PRECOND_DIAGONAL = 1, TNLMI_PRECOND_ILU0 = 2 } tnlmi_precond_kind;
+typedef struct tnlmi_preconditioner tnlmi_preconditioner;
+
+/* Preconditioner::update(matrix) -- Preconditioners/Diagonal.hpp:12-31
+ * (d_i = A(i,i)) and ILU0.hpp:12-96 (IKJ incomplete LU on A's pattern,
+ * factorised on the host as in the reference; the triangular solves run on the
+ * GPU by dependency levels -- the reference's GPU ILU0 throws
+ * NotImplementedError, ILU0.h:122-141).  Synchronous. */
+tnlmi_status tnlmi_preconditioner_create( tnlmi_precond_kind kind, const tnlmi_matrix* A, tnlmi_stream stream,
+                                          tnlmi_preconditioner** out );
+void tnlmi_preconditioner_destroy( tnlmi_preconditioner* P );
+/* Preconditioner::solve(b, x): x = M^{-1} b (asynchronous). */
+tnlmi_status tnlmi_preconditioner_solve( const tnlmi_preconditioner* P, const double* d_b, double* d_x,
+                                         tnlmi_stream stream );
+
+/* LinearSolver<Matrix>::solve(b, x) -> bool (synchronous; FP64 vectors; any
+ * matrix format and value type, FP64 accumulation).  plan: optional
+ * column-split plan of A used for its products (NULL: A's own kernels).
+ * CG: Solvers/Linear/CG.hpp:10-108.  GMRES: GMRES.hpp:71-806 (restarted,
+ * Baker-Jessup-Kolev adaptive restart between restarting_min and _max,
+ * restarting_max <= 31).  Jacobi: Jacobi.hpp:65-125. */
+tnlmi_status tnlmi_cg_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                             const double* d_b, double* d_x, tnlmi_solver_state* state, tnlmi_stream stream );
+typedef enum {
+   TNLMI_GMRES_CGS = 0,
+   TNLMI_GMRES_CGSR = 1,
+   TNLMI_GMRES_MGS = 2,
+   TNLMI_GMRES_MGSR = 3,
+   TNLMI_GMRES_CWY = 4
+} tnlmi_gmres_variant;
+tnlmi_status tnlmi_gmres_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                                tnlmi_gmres_variant variant, int32_t restarting_min, int32_t restarting_max,
+                                int32_t step_min, int32_t step_max, const double* d_b, double* d_x,
+                                tnlmi_solver_state* state, tnlmi_stream stream );
+tnlmi_status tnlmi_jacobi_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, double omega,
+                                 int64_t residue_period, const double* d_b, double* d_x, tnlmi_solver_state* state,
+                                 tnlmi_stream stream );
+/* ||A x - b||_2 / ||b||_2 -- Solvers/Linear/Utils/LinearResidueGetter.hpp:11-23
+ * (synchronous, result on the host). */
+tnlmi_status tnlmi_true_residual( const tnlmi_matrix* A, const double* d_x, const double* d_b, double* residual,
+                                  tnlmi_stream stream );
+
 /* ------------------------------------------------- input generators (host) */
 /* Seeded host-side generators of the workloads the engine is measured on.
  * Random numbers come from a counter-based generator (splitmix64 of seed,

@@ -1,0 +1,190 @@
+"""Krylov solvers on the GPU against the pinned oracle (which agrees with TNL's
+own CG / GMRES / Jacobi / ILU0 on the host, tests/test_oracle.py).
+
+FP64 throughout.  The GPU reductions sum in a different order than the host
+loops, so iteration counts may differ by one near the stopping threshold; the
+solutions agree to the tolerance stated per test and reach the same true
+residual."""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+tnl = pytest.importorskip("tnl_amd")
+from tnl_amd import solvers as S  # noqa: E402
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def nonsym(n, seed):
+    rng = np.random.default_rng(seed)
+    off = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for i in range(n):
+        row = {i: 4.0}
+        if i > 0:
+            row[i - 1] = -1.3
+        if i < n - 1:
+            row[i + 1] = -0.7
+        j = int(rng.integers(0, n))
+        if j != i:
+            row[j] = row.get(j, 0.0) + 0.2 * rng.standard_normal()
+        for c in sorted(row):
+            cols.append(c)
+            vals.append(row[c])
+        off[i + 1] = len(cols)
+    return off, np.array(cols, np.int32), np.array(vals)
+
+
+FORMATS = [(tnl.CSR, 0), (tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 64), (tnl.BIELLPACK, 32), (tnl.CHUNKED_ELLPACK, 0),
+           (tnl.ADAPTIVE_CSR, 0), (tnl.SORTED_CSR, 0)]
+
+
+@pytest.mark.parametrize("fmt,param", FORMATS, ids=[tnl.FORMAT_NAMES[f] for f, _ in FORMATS])
+@pytest.mark.parametrize("pre", [0, 1], ids=["none", "diagonal"])
+def test_cg(orc, fmt, param, pre):
+    off, cols, vals = tnl.laplace2d(40, 31)
+    n = len(off) - 1
+    b = np.random.default_rng(1).standard_normal(n)
+    xo, so = orc.cg(off, cols, vals, b, precond=pre, tol=1e-10)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+    cg = S.CG()
+    cg.setMatrix(A)
+    if pre:
+        cg.setPreconditioner(S.Diagonal(A))
+    cg.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert cg.solve(dev(b), x)
+    assert abs(cg.getIterations() - so.iterations) <= 1
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-9)
+    assert S.true_residual(A, x, dev(b)) < 1e-9
+    assert abs(S.true_residual(A, x, dev(b)) - orc.true_residual(off, cols, vals, x.cpu().numpy(), b)) < 1e-12
+
+
+def test_cg_split_plan_and_stopping(orc):
+    off, cols, vals = tnl.laplace2d(30, 30)
+    n = len(off) - 1
+    b = np.random.default_rng(2).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    P = tnl.SpmvPlan(A, 8)
+    cg = S.CG()
+    cg.setMatrix(A, plan=P)
+    cg.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert cg.solve(dev(b), x)
+    xo, so = orc.cg(off, cols, vals, b, tol=1e-10)
+    assert abs(cg.getIterations() - so.iterations) <= 1
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-9)
+    # IterativeSolver stopping: max iterations, divergence, NaN (IterativeSolver.hpp:98-143)
+    cg = S.CG()
+    cg.setMatrix(A)
+    cg.setConvergenceResidue(1e-14)
+    cg.setMaxIterations(5)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert not cg.solve(dev(b), x)
+    xo, so = orc.cg(off, cols, vals, b, tol=1e-14, maxit=5)
+    assert cg.getIterations() == so.iterations and not so.converged
+    cg.setMaxIterations(10**9)
+    cg.setDivergenceResidue(1e-3)
+    assert not cg.solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    bn = b.copy()
+    bn[3] = np.nan
+    assert not cg.solve(dev(bn), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    # zero right-hand side: x = 0, converged (CG.hpp:31-35)
+    x = dev(np.ones(n))
+    assert cg.solve(dev(np.zeros(n)), x) and float(x.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("variant", ["CGS", "CGSR", "MGS", "MGSR"])
+@pytest.mark.parametrize("pre", [0, 1, 2], ids=["none", "diagonal", "ilu0"])
+def test_gmres(orc, variant, pre):
+    off, cols, vals = nonsym(500, 7)
+    n = 500
+    b = np.random.default_rng(8).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    for rmin, rmax in [(10, 10), (5, 20)]:
+        xo, so = orc.gmres(off, cols, vals, b, precond=pre, variant=S.GMRES_VARIANTS[variant], rmin=rmin, rmax=rmax,
+                           tol=1e-10)
+        g = S.GMRES()
+        g.setMatrix(A)
+        assert g.setup(variant=variant, restarting_min=rmin, restarting_max=rmax)
+        if pre == 1:
+            g.setPreconditioner(S.Diagonal(A))
+        elif pre == 2:
+            g.setPreconditioner(S.ILU0(A))
+        g.setConvergenceResidue(1e-10)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        assert g.solve(dev(b), x) and so.converged
+        assert abs(g.getIterations() - so.iterations) <= 1, (g.getIterations(), so.iterations)
+        np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-9)
+
+
+def test_gmres_errors():
+    g = S.GMRES()
+    with pytest.raises(tnl.LogicError):
+        g.solve(dev(np.ones(3)), dev(np.zeros(3)))
+    off, cols, vals = nonsym(50, 1)
+    A = tnl.SparseMatrix.from_csr(50, 50, off, cols, vals)
+    g.setMatrix(A)
+    assert g.setup(variant="CWY")
+    with pytest.raises(NotImplementedError):
+        g.solve(dev(np.ones(50)), dev(np.zeros(50)))
+    assert not g.setup(variant="XYZ")
+    g.setup(variant="MGS", restarting_min=20, restarting_max=10)
+    assert not g.solve(dev(np.ones(50)), dev(np.zeros(50)))  # wrong restarting parameters -> false
+
+
+def test_jacobi(orc):
+    off, cols, vals = nonsym(400, 3)
+    b = np.random.default_rng(4).standard_normal(400)
+    xo, so = orc.jacobi(off, cols, vals, b, tol=1e-9)
+    A = tnl.SparseMatrix.from_csr(400, 400, off, cols, vals)
+    j = S.Jacobi()
+    j.setMatrix(A)
+    j.setConvergenceResidue(1e-9)
+    x = torch.zeros(400, dtype=torch.float64, device="cuda")
+    assert j.solve(dev(b), x)
+    assert j.getIterations() == so.iterations
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-10, atol=1e-12)
+
+
+def test_preconditioners(orc, ):
+    off, cols, vals = tnl.laplace2d(23, 19)
+    n = len(off) - 1
+    b = np.random.default_rng(9).standard_normal(n)
+    for fmt, param in FORMATS:
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+        L, U = orc.ilu0(off, cols, vals)
+        got = S.ILU0(A).solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+        np.testing.assert_allclose(got, orc.ilu0_solve(L, U, b), rtol=1e-12, atol=1e-13)
+        got = S.Diagonal(A).solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+        np.testing.assert_array_equal(got, b / 4.0)
+
+
+def test_blas1(orc):
+    rng = np.random.default_rng(5)
+    for n in [1, 1000, 3_000_001]:
+        a, c = rng.standard_normal(n), rng.standard_normal(n)
+        assert abs(S.dot(dev(a), dev(c)) - float(np.dot(a, c))) <= 1e-12 * (np.abs(a) @ np.abs(c))
+        assert abs(S.l2Norm(dev(a)) - float(np.linalg.norm(a))) <= 1e-13 * np.linalg.norm(a)
+
+
+def test_solver_example_5x5():
+    # Documentation/Examples/Solvers/Linear/IterativeLinearSolverExample.cpp
+    off = np.array([0, 2, 5, 8, 11, 13], np.int32)
+    cols = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4], np.int32)
+    vals = np.array([2.5, -1, -1, 2.5, -1, -1, 2.5, -1, -1, 2.5, -1, -1, 2.5])
+    A = tnl.SparseMatrix.from_csr(5, 5, off, cols, vals)
+    b = torch.zeros(5, dtype=torch.float64, device="cuda")
+    A.vectorProduct(dev(np.ones(5)), b)
+    for solver in (S.CG(), S.GMRES()):
+        solver.setMatrix(A)
+        solver.setConvergenceResidue(1e-12)
+        x = torch.zeros(5, dtype=torch.float64, device="cuda")
+        assert solver.solve(b, x)
+        np.testing.assert_allclose(x.cpu().numpy(), np.ones(5), rtol=1e-10)

@@ -180,3 +180,39 @@ def test_errors():
         A.vectorProduct(x, x.clone(), launch=(tnl.FIXED, 512))
     with pytest.raises(ValueError):
         A.vectorProduct(x, x.clone(), begin=10, end=5)
+
+
+@pytest.mark.parametrize("parts", [8, 16])
+def test_column_split_plan(orc, parts):
+    rng = np.random.default_rng(parts)
+    for n, m, maxlen, longs in [(1, 1, 1, ()), (300, 700, 30, ()), (20000, 30000, 40, [(3, 5000), (19999, 2500)])]:
+        off, cols, vals = random_csr(n, m, maxlen, rng, long_rows=longs)
+        A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals)
+        P = tnl.SpmvPlan(A, parts)
+        x = rng.standard_normal(m)
+        y0 = rng.standard_normal(n)
+        for alpha, beta, b, e in [(1, 0, 0, 0), (-2.0, 0.5, 0, 0), (0.75, -1.25, n // 3, n - n // 5)]:
+            want = orc.spmv(off, cols, vals, x, y=y0, alpha=alpha, beta=beta, begin=b, end=e)
+            xd, yd = gpu(x), gpu(y0)
+            P.vectorProduct(xd, yd, alpha=alpha, beta=beta, begin=b, end=e)
+            torch.cuda.synchronize()
+            got = yd.cpu().numpy()
+            assert relerr(got, want) <= 1e-12, (n, alpha, beta, b, e)
+            if b or e:
+                assert np.array_equal(got[:b], y0[:b]) and np.array_equal(got[e:], y0[e:])
+        del A  # the plan owns a copy of the matrix data
+        xd, yd = gpu(x), torch.zeros(n, dtype=torch.float64, device="cuda")
+        P.vectorProduct(xd, yd)
+        torch.cuda.synchronize()
+        assert relerr(yd.cpu().numpy(), orc.spmv(off, cols, vals, x)) <= 1e-12
+    # full-size power-law matrix
+    off, cols, vals = tnl.zipf_csr(4_194_304, seed=0)
+    n = len(off) - 1
+    x = tnl.uniform(n, -1.0, 1.0, seed=0)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    P = tnl.SpmvPlan(A, parts)
+    del A
+    xd, yd = gpu(x), torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.vectorProduct(xd, yd)
+    torch.cuda.synchronize()
+    assert relerr(yd.cpu().numpy(), orc.spmv_omp(off, cols, vals, x, 16)) <= 1e-12

@@ -89,9 +89,12 @@ class Segments:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
-            _lib.lib().tnlmi_segments_destroy(h)
-            self.handle = None
+        try:
+            if h is not None and h.value and _lib._lib is not None:
+                _lib.lib().tnlmi_segments_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown
+            pass
+        self.handle = None
 
     def getSegmentCount(self):
         return _lib.lib().tnlmi_segments_count(self.handle)
@@ -216,6 +219,35 @@ class SparseMatrix:
         _lib.check(_lib.lib().tnlmi_spmv(ctypes.byref(self._m), xt, ctypes.c_void_p(x.data_ptr()),
                                          ctypes.c_void_p(y.data_ptr()), ct, float(alpha), float(beta), int(begin),
                                          int(end), L, _stream(stream)))
+        return y
+
+
+class SpmvPlan:
+    """XCD column-split execution plan of a CSR SparseMatrix (tnlmi_spmv_plan)."""
+
+    def __init__(self, A, parts=8, stream=None):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tnlmi_spmv_plan_create(ctypes.byref(A._m), parts, _stream(stream), ctypes.byref(h)))
+        self.handle = h
+        self.rows, self.columns = A.rows, A.columns
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        try:
+            if h is not None and h.value and _lib._lib is not None:
+                _lib.lib().tnlmi_spmv_plan_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown
+            pass
+        self.handle = None
+
+    def vectorProduct(self, x, y, alpha=1.0, beta=0.0, begin=0, end=0, compute_dtype=None, stream=None):
+        if x.numel() != self.columns or y.numel() != self.rows or x.dtype != y.dtype:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "vectorProduct: vector sizes/dtypes do not match the matrix")
+        xt = dtype_code(x.dtype)
+        ct = dtype_code(compute_dtype) if compute_dtype is not None else xt
+        _lib.check(_lib.lib().tnlmi_spmv_plan_execute(self.handle, xt, ctypes.c_void_p(x.data_ptr()),
+                                                      ctypes.c_void_p(y.data_ptr()), ct, float(alpha), float(beta),
+                                                      int(begin), int(end), _stream(stream)))
         return y
 
 

@@ -100,12 +100,9 @@ class SolverState(ctypes.Structure):
         ("min_iterations", ctypes.c_int64),
         ("convergence_residue", ctypes.c_double),
         ("divergence_residue", ctypes.c_double),
-        ("residue_check_period", ctypes.c_int64),
         ("iterations", ctypes.c_int64),
         ("residue", ctypes.c_double),
-        ("true_residue", ctypes.c_double),
         ("converged", ctypes.c_int32),
-        ("spmv_count", ctypes.c_int64),
     ]
 
 
@@ -129,6 +126,24 @@ _PROTOS = {
     "tnlmi_matrix_fill": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_vp, c_vp]),
     "tnlmi_spmv": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int, c_vp, c_vp, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double, c_i64, c_i64, ctypes.POINTER(Launch), c_vp]),
+    "tnlmi_spmv_plan_create": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)]),
+    "tnlmi_spmv_plan_destroy": (None, [c_vp]),
+    "tnlmi_spmv_plan_execute": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, c_i64, c_i64, c_vp]),
+    "tnlmi_dot": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "tnlmi_nrm2": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp]),
+    "tnlmi_axpby": (ctypes.c_int, [c_i64, ctypes.c_double, c_vp, ctypes.c_double, c_vp, c_vp]),
+    "tnlmi_preconditioner_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Matrix), c_vp, ctypes.POINTER(c_vp)]),
+    "tnlmi_preconditioner_destroy": (None, [c_vp]),
+    "tnlmi_preconditioner_solve": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "tnlmi_cg_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(SolverState),
+                                      c_vp]),
+    "tnlmi_gmres_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_vp, c_vp,
+                                         ctypes.POINTER(SolverState), c_vp]),
+    "tnlmi_jacobi_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, ctypes.c_double, c_i64, c_vp, c_vp,
+                                          ctypes.POINTER(SolverState), c_vp]),
+    "tnlmi_true_residual": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.POINTER(ctypes.c_double), c_vp]),
     "tnlmi_gen_laplace2d": (c_i64, [c_i64, c_i64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_poisson3d": (c_i64, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_zipf_sizes": (None, [c_i64, ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,

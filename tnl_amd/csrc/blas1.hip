@@ -1,0 +1,265 @@
+// blas1.hip -- deterministic device reductions and vector kernels (see blas1.hpp)
+// plus their C ABI (TNL's Containers/Expressions BLAS-1 on the new device).
+#include <algorithm>
+#include <cmath>
+
+#include "blas1.hpp"
+
+namespace tnlmi {
+namespace {
+
+inline int red_grid( int64_t n )
+{
+   return (int) std::max< int64_t >( 1, std::min< int64_t >( kRedBlocks, ceil_div( n, 256 * 4 ) ) );
+}
+
+// first pass: one partial per workgroup (grid-stride, 4 independent sums per thread)
+template< typename F >
+__global__ void __launch_bounds__( 256 ) red_pass1( int64_t n, F f, double* part )
+{
+   __shared__ double sm[ 4 ];
+   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+   const int64_t stride = (int64_t) gridDim.x * 256;
+   int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
+   for( ; i + 3 * stride < n; i += 4 * stride ) {
+      a0 += f( i );
+      a1 += f( i + stride );
+      a2 += f( i + 2 * stride );
+      a3 += f( i + 3 * stride );
+   }
+   for( ; i < n; i += stride )
+      a0 += f( i );
+   const double v = block_sum< 256 >( ( a0 + a1 ) + ( a2 + a3 ), sm );
+   if( threadIdx.x == 0 )
+      part[ blockIdx.x ] = v;
+}
+
+// second pass: fixed-order sum of the partials; mode 0: sum, 1: sqrt(sum)
+__global__ void __launch_bounds__( 256 ) red_pass2( const double* part, int np, double* out, int mode )
+{
+   __shared__ double sm[ 4 ];
+   double a = 0;
+   for( int i = threadIdx.x; i < np; i += 256 )
+      a += part[ i ];
+   a = block_sum< 256 >( a, sm );
+   if( threadIdx.x == 0 )
+      out[ 0 ] = mode == 1 ? sqrt( a ) : a;
+}
+
+struct DotF
+{
+   const double *a, *b;
+   __device__ double operator()( int64_t i ) const { return a[ i ] * b[ i ]; }
+};
+struct SqF
+{
+   const double* a;
+   __device__ double operator()( int64_t i ) const { return a[ i ] * a[ i ]; }
+};
+
+constexpr int kMaxCols = 32;
+
+__global__ void __launch_bounds__( 256 ) multi_dot_pass1( int64_t n, int cols, const double* __restrict__ V,
+                                                          int64_t ld, const double* __restrict__ w,
+                                                          double* __restrict__ part )
+{
+   __shared__ double sm[ 4 ];
+   double acc[ kMaxCols ];
+#pragma unroll
+   for( int k = 0; k < kMaxCols; k++ )
+      acc[ k ] = 0;
+   const int64_t stride = (int64_t) gridDim.x * 256;
+   for( int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += stride ) {
+      const double wi = w[ i ];
+#pragma unroll
+      for( int k = 0; k < kMaxCols; k++ )
+         if( k < cols )
+            acc[ k ] += V[ i + k * ld ] * wi;
+   }
+   for( int k = 0; k < cols; k++ ) {
+      const double v = block_sum< 256 >( acc[ k ], sm );
+      if( threadIdx.x == 0 )
+         part[ (int64_t) k * kRedBlocks + blockIdx.x ] = v;
+   }
+}
+
+__global__ void __launch_bounds__( 256 ) multi_dot_pass2( const double* part, int np, int cols, double* out )
+{
+   __shared__ double sm[ 4 ];
+   for( int k = 0; k < cols; k++ ) {
+      double a = 0;
+      for( int i = threadIdx.x; i < np; i += 256 )
+         a += part[ (int64_t) k * kRedBlocks + i ];
+      a = block_sum< 256 >( a, sm );
+      if( threadIdx.x == 0 )
+         out[ k ] = a;
+   }
+}
+
+__global__ void axpby_k( int64_t n, double alpha, const double* x, double beta, double* y )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      y[ i ] = beta == 0.0 ? alpha * x[ i ] : alpha * x[ i ] + beta * y[ i ];
+}
+
+__global__ void scale_ratio_k( int64_t n, const double* x, const double* num, const double* den, double* y )
+{
+   const double d = den[ 0 ];
+   const double sc = d != 0.0 ? num[ 0 ] / d : 0.0;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      y[ i ] = sc * x[ i ];
+}
+
+template< int SIGN >
+__global__ void gemv_k( int64_t n, int cols, const double* __restrict__ V, int64_t ld, const double* __restrict__ h,
+                        double* __restrict__ y )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x ) {
+      double acc = 0;
+      for( int k = 0; k < cols; k++ )
+         acc += V[ i + k * ld ] * h[ k ];
+      y[ i ] = SIGN > 0 ? y[ i ] + acc : y[ i ] - acc;
+   }
+}
+
+__global__ void divide_k( int64_t n, const double* x, const double* d, double* y )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      y[ i ] = x[ i ] / d[ i ];
+}
+
+__global__ void rsub_k( int64_t n, const double* x, double* y )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      y[ i ] = x[ i ] - y[ i ];
+}
+
+__global__ void copy_k( int64_t n, const double* x, double* y )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      y[ i ] = x[ i ];
+}
+
+__global__ void fill_k( int64_t n, double v, double* y )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      y[ i ] = v;
+}
+
+inline dim3 vgrid( int64_t n )
+{
+   return dim3( (unsigned) std::max< int64_t >( 1, std::min< int64_t >( 8192, ceil_div( n, 256 ) ) ) );
+}
+
+}  // namespace
+
+void dot( int64_t n, const double* a, const double* b, double* out, RedWork& w, hipStream_t s, int slot )
+{
+   w.ensure( slot + 1 );
+   const int g = red_grid( n );
+   hipLaunchKernelGGL( red_pass1< DotF >, dim3( g ), dim3( 256 ), 0, s, n, DotF{ a, b }, w.slot( slot ) );
+   hipLaunchKernelGGL( red_pass2, dim3( 1 ), dim3( 256 ), 0, s, w.slot( slot ), g, out, 0 );
+}
+
+void nrm2( int64_t n, const double* a, double* out, RedWork& w, hipStream_t s, int slot )
+{
+   w.ensure( slot + 1 );
+   const int g = red_grid( n );
+   hipLaunchKernelGGL( red_pass1< SqF >, dim3( g ), dim3( 256 ), 0, s, n, SqF{ a }, w.slot( slot ) );
+   hipLaunchKernelGGL( red_pass2, dim3( 1 ), dim3( 256 ), 0, s, w.slot( slot ), g, out, 1 );
+}
+
+void multi_dot( int64_t n, int cols, const double* V, int64_t ld, const double* w, double* out, RedWork& rw,
+                hipStream_t s )
+{
+   if( cols > kMaxCols )
+      fail( TNLMI_NOT_IMPLEMENTED, "multi_dot: at most 32 columns" );
+   rw.ensure( kMaxCols );
+   const int g = red_grid( n );
+   hipLaunchKernelGGL( multi_dot_pass1, dim3( g ), dim3( 256 ), 0, s, n, cols, V, ld, w, rw.slot( 0 ) );
+   hipLaunchKernelGGL( multi_dot_pass2, dim3( 1 ), dim3( 256 ), 0, s, rw.slot( 0 ), g, cols, out );
+}
+
+void axpby( int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( axpby_k, vgrid( n ), dim3( 256 ), 0, s, n, alpha, x, beta, y );
+}
+
+void scale_by_ratio( int64_t n, const double* x, const double* num, const double* den, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( scale_ratio_k, vgrid( n ), dim3( 256 ), 0, s, n, x, num, den, y );
+}
+
+void gemv_sub( int64_t n, int cols, const double* V, int64_t ld, const double* h, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( gemv_k< -1 >, vgrid( n ), dim3( 256 ), 0, s, n, cols, V, ld, h, y );
+}
+
+void gemv_add( int64_t n, int cols, const double* V, int64_t ld, const double* c, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( gemv_k< 1 >, vgrid( n ), dim3( 256 ), 0, s, n, cols, V, ld, c, y );
+}
+
+void divide( int64_t n, const double* x, const double* d, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( divide_k, vgrid( n ), dim3( 256 ), 0, s, n, x, d, y );
+}
+
+void rsub( int64_t n, const double* x, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( rsub_k, vgrid( n ), dim3( 256 ), 0, s, n, x, y );
+}
+
+void copy( int64_t n, const double* x, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( copy_k, vgrid( n ), dim3( 256 ), 0, s, n, x, y );
+}
+
+void fill( int64_t n, double v, double* y, hipStream_t s )
+{
+   hipLaunchKernelGGL( fill_k, vgrid( n ), dim3( 256 ), 0, s, n, v, y );
+}
+
+}  // namespace tnlmi
+
+using namespace tnlmi;
+
+extern "C" {
+
+// (a, b) and lpNorm(a, 2) of FP64 device vectors, result to device memory
+// (asynchronous) -- Containers/Expressions/ExpressionTemplates.h:372-384, :442-460.
+tnlmi_status tnlmi_dot( int64_t n, const double* d_a, const double* d_b, double* d_result, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( n < 0 || ( n && ( ! d_a || ! d_b ) ) || ! d_result )
+         fail( TNLMI_INVALID_ARGUMENT, "dot: invalid argument" );
+      static thread_local RedWork w;
+      dot( n, d_a, d_b, d_result, w, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+tnlmi_status tnlmi_nrm2( int64_t n, const double* d_a, double* d_result, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( n < 0 || ( n && ! d_a ) || ! d_result )
+         fail( TNLMI_INVALID_ARGUMENT, "nrm2: invalid argument" );
+      static thread_local RedWork w;
+      nrm2( n, d_a, d_result, w, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+// y = alpha*x + beta*y (VectorAssignment of x += a*p, p = z + b*p, ...)
+tnlmi_status tnlmi_axpby( int64_t n, double alpha, const double* d_x, double beta, double* d_y, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( n < 0 || ( n && ( ! d_x || ! d_y ) ) )
+         fail( TNLMI_INVALID_ARGUMENT, "axpby: invalid argument" );
+      if( n )
+         axpby( n, alpha, d_x, beta, d_y, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+}  // extern "C"

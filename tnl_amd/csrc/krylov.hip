@@ -1,0 +1,954 @@
+// krylov.hip -- Krylov solvers and preconditioners on the GPU, FP64.
+//
+// Solvers/Linear/CG.hpp, GMRES.hpp, Jacobi.hpp, Preconditioners/Diagonal.hpp,
+// ILU0.hpp, Utils/LinearResidueGetter.hpp and the stopping rules of
+// Solvers/IterativeSolver.hpp:98-143.  The iteration structure and the
+// stopping semantics follow the reference line by line; the arithmetic runs
+// in this library's kernels with every Krylov scalar (dot products, norms,
+// alpha/beta, the Hessenberg matrix, Givens rotations and the least-squares
+// back-substitution of GMRES) resident on the GPU.  The host reads one
+// scalar (the solver's residue) per iteration to evaluate the stopping rule.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "blas1.hpp"
+#include "krylov.hpp"
+#include "segments.hpp"
+#include "spmv.hpp"
+
+using namespace tnlmi;
+
+// ------------------------------------------------------- preconditioners
+struct tnlmi_preconditioner
+{
+   tnlmi_precond_kind kind = TNLMI_PRECOND_NONE;
+   int64_t n = 0;
+   DeviceArray< double > diag;
+   // ILU(0): L (strictly lower, unit diagonal implied) and U with reversed rows
+   // (U row N-1-i holds row i, diagonal first), CSR on the device; rows grouped
+   // by dependency level for the triangular solves.
+   DeviceArray< int32_t > lp, lc, up, uc, lrows, urows;
+   DeviceArray< double > lv, uv;
+   std::vector< int64_t > llev, ulev;  // level boundaries into lrows / urows
+};
+
+namespace tnlmi {
+namespace {
+
+// d_i = A(i, i): linear search of row i (Diagonal.hpp:27-30 via getElement)
+template< typename V >
+__global__ void diag_kernel( SegView v, const int32_t* __restrict__ cols, const V* __restrict__ vals, double* d )
+{
+   const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( r >= v.n )
+      return;
+   double dv = 0.0;
+   for( int32_t k = 0; k < v.sizes[ r ]; k++ ) {
+      const int64_t g = global_index( v, r, k );
+      if( cols[ g ] == r ) {
+         dv = (double) load_val< V, double >( vals, g );
+         break;
+      }
+   }
+   d[ r ] = dv;
+}
+
+// CSR-ordered copy of any format (padding dropped later on the host)
+template< typename V >
+__global__ void to_csr_kernel( SegView v, const int32_t* __restrict__ cols, const V* __restrict__ vals,
+                               int32_t* __restrict__ oc, double* __restrict__ ov )
+{
+   const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( r >= v.n )
+      return;
+   const int32_t b = v.offsets[ r ];
+   for( int32_t k = 0; k < v.sizes[ r ]; k++ ) {
+      const int64_t g = global_index( v, r, k );
+      oc[ b + k ] = cols[ g ];
+      ov[ b + k ] = (double) load_val< V, double >( vals, g );
+   }
+}
+
+template< typename F >
+void with_value_type( tnlmi_dtype t, F&& f )
+{
+   if( t == TNLMI_F64 )
+      f( (const double*) nullptr );
+   else if( t == TNLMI_F32 )
+      f( (const float*) nullptr );
+   else
+      f( (const bf16*) nullptr );
+}
+
+// one dependency level of the lower solve: x_i = b_i - sum_j L_ij x_j
+__global__ void lower_level( const int32_t* __restrict__ rows, int64_t nrows, const int32_t* __restrict__ lp,
+                             const int32_t* __restrict__ lc, const double* __restrict__ lv,
+                             const double* __restrict__ b, double* __restrict__ x )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( t >= nrows )
+      return;
+   const int32_t i = rows[ t ];
+   double xi = b[ i ];
+   for( int32_t g = lp[ i ]; g < lp[ i + 1 ]; g++ )
+      xi -= lv[ g ] * x[ lc[ g ] ];
+   x[ i ] = xi;
+}
+
+// one level of the upper solve on reversed rows: x_i = (x_i - sum_{j>i} U_ij x_j) / U_ii
+__global__ void upper_level( const int32_t* __restrict__ rows, int64_t nrows, int64_t N,
+                             const int32_t* __restrict__ up, const int32_t* __restrict__ uc,
+                             const double* __restrict__ uv, double* __restrict__ x )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( t >= nrows )
+      return;
+   const int32_t i = rows[ t ];
+   const int64_t k = N - 1 - i;
+   double xi = x[ i ];
+   const double uii = up[ k ] < up[ k + 1 ] ? uv[ up[ k ] ] : 0.0;  // a row without entries has no pivot
+   for( int32_t g = up[ k ] + 1; g < up[ k + 1 ]; g++ )
+      xi -= uv[ g ] * x[ uc[ g ] ];
+   x[ i ] = xi / uii;
+}
+
+// rows grouped by level; level(i) = 1 + max level of the rows it depends on
+void levels( int64_t N, const std::vector< int32_t >& ptr, const std::vector< int32_t >& col, bool upper,
+             std::vector< int32_t >& order, std::vector< int64_t >& bounds )
+{
+   std::vector< int32_t > lev( N, 0 );
+   int32_t maxl = 0;
+   if( ! upper )
+      for( int64_t i = 0; i < N; i++ ) {
+         int32_t l = 0;
+         for( int32_t g = ptr[ i ]; g < ptr[ i + 1 ]; g++ )
+            l = std::max( l, lev[ col[ g ] ] + 1 );
+         lev[ i ] = l;
+         maxl = std::max( maxl, l );
+      }
+   else
+      for( int64_t i = N - 1; i >= 0; i-- ) {
+         const int64_t k = N - 1 - i;
+         int32_t l = 0;
+         for( int32_t g = ptr[ k ] + 1; g < ptr[ k + 1 ]; g++ )
+            l = std::max( l, lev[ col[ g ] ] + 1 );
+         lev[ i ] = l;
+         maxl = std::max( maxl, l );
+      }
+   std::vector< int64_t > cnt( maxl + 2, 0 );
+   for( int64_t i = 0; i < N; i++ )
+      cnt[ lev[ i ] + 1 ]++;
+   for( int32_t l = 0; l <= maxl; l++ )
+      cnt[ l + 1 ] += cnt[ l ];
+   bounds.assign( cnt.begin(), cnt.end() );
+   order.assign( N, 0 );
+   std::vector< int64_t > pos( cnt.begin(), cnt.end() - 1 );
+   for( int64_t i = 0; i < N; i++ )
+      order[ pos[ lev[ i ] ]++ ] = (int32_t) i;
+}
+
+template< typename T >
+void upload_vec( DeviceArray< T >& d, const std::vector< T >& h, hipStream_t s )
+{
+   d.allocate( std::max< int64_t >( 1, (int64_t) h.size() ) );
+   if( ! h.empty() )
+      TNLMI_HIP( hipMemcpyAsync( d.ptr, h.data(), sizeof( T ) * h.size(), hipMemcpyHostToDevice, s ) );
+}
+
+// ILU0.hpp:12-96 (IKJ on A's pattern) on the host, as the reference does;
+// the triangular solves run on the GPU by levels.
+void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
+{
+   const tnlmi_segments& seg = *A.segments;
+   const int64_t N = A.rows;
+   const int64_t nel = seg.elements;
+   DeviceArray< int32_t > dc;
+   DeviceArray< double > dv;
+   dc.allocate( std::max< int64_t >( 1, nel ) );
+   dv.allocate( std::max< int64_t >( 1, nel ) );
+   with_value_type( A.value_type, [ & ]( auto tag ) {
+      using V = std::remove_const_t< std::remove_pointer_t< decltype( tag ) > >;
+      if( N )
+         hipLaunchKernelGGL( to_csr_kernel< V >, dim3( (unsigned) ceil_div( N, 256 ) ), dim3( 256 ), 0, s,
+                             make_view( seg ), A.column_indexes, (const V*) A.values, dc.ptr, dv.ptr );
+   } );
+   std::vector< int32_t > off( N + 1 ), col( nel );
+   std::vector< double > val( nel );
+   TNLMI_HIP( hipMemcpyAsync( off.data(), seg.offsets.ptr, 4 * ( N + 1 ), hipMemcpyDeviceToHost, s ) );
+   if( nel ) {
+      TNLMI_HIP( hipMemcpyAsync( col.data(), dc.ptr, 4 * nel, hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipMemcpyAsync( val.data(), dv.ptr, 8 * nel, hipMemcpyDeviceToHost, s ) );
+   }
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+   // compress each row (drop padding), keep the column order of the row
+   std::vector< int32_t > rp( N + 1, 0 ), rc;
+   std::vector< double > rv;
+   rc.reserve( nel );
+   rv.reserve( nel );
+   for( int64_t i = 0; i < N; i++ ) {
+      for( int32_t g = off[ i ]; g < off[ i + 1 ]; g++ )
+         if( col[ g ] >= 0 ) {
+            rc.push_back( col[ g ] );
+            rv.push_back( val[ g ] );
+         }
+      rp[ i + 1 ] = (int32_t) rc.size();
+   }
+   std::vector< int32_t > Lr( N ), Ur( N );
+   for( int64_t i = 0; i < N; i++ ) {
+      int32_t le = 0, ue = 0;
+      for( int32_t g = rp[ i ]; g < rp[ i + 1 ]; g++ ) {
+         if( rc[ g ] < i )
+            le++;
+         else if( rc[ g ] < N )
+            ue++;
+         else
+            break;
+      }
+      Lr[ i ] = le;
+      Ur[ N - 1 - i ] = ue;
+   }
+   std::vector< int32_t > lp( N + 1, 0 ), up( N + 1, 0 );
+   for( int64_t i = 0; i < N; i++ ) {
+      lp[ i + 1 ] = lp[ i ] + Lr[ i ];
+      up[ i + 1 ] = up[ i ] + Ur[ i ];
+   }
+   std::vector< int32_t > lc( lp[ N ] ), uc( up[ N ] );
+   std::vector< double > lv( lp[ N ] ), uv( up[ N ] );
+   auto uget = [ & ]( int64_t urow, int32_t c ) -> double {
+      for( int32_t g = up[ urow ]; g < up[ urow + 1 ]; g++ )
+         if( uc[ g ] == c )
+            return uv[ g ];
+      return 0.0;
+   };
+   for( int64_t i = 0; i < N; i++ ) {
+      const int32_t le = Lr[ i ], ue = Ur[ N - 1 - i ];
+      const int64_t ub = up[ N - 1 - i ], lb = lp[ i ], rb = rp[ i ];
+      for( int32_t j = 0; j < ue; j++ ) {
+         uc[ ub + j ] = rc[ rb + le + j ];
+         uv[ ub + j ] = rv[ rb + le + j ];
+      }
+      for( int32_t j = 0; j < le; j++ ) {
+         lc[ lb + j ] = rc[ rb + j ];
+         lv[ lb + j ] = rv[ rb + j ];
+      }
+      for( int32_t ck = 0; ck < le; ck++ ) {
+         const int32_t k = lc[ lb + ck ];
+         const double lik = lv[ lb + ck ] / uget( N - 1 - k, k );
+         lv[ lb + ck ] = lik;
+         for( int32_t cj = ck + 1; cj < le; cj++ )
+            lv[ lb + cj ] = lv[ lb + cj ] - lik * uget( N - 1 - k, lc[ lb + cj ] );
+         for( int32_t cj = 0; cj < ue; cj++ )
+            uv[ ub + cj ] = uv[ ub + cj ] - lik * uget( N - 1 - k, uc[ ub + cj ] );
+      }
+   }
+   std::vector< int32_t > lo, uo;
+   levels( N, lp, lc, false, lo, P.llev );
+   levels( N, up, uc, true, uo, P.ulev );
+   upload_vec( P.lp, lp, s );
+   upload_vec( P.lc, lc, s );
+   upload_vec( P.lv, lv, s );
+   upload_vec( P.up, up, s );
+   upload_vec( P.uc, uc, s );
+   upload_vec( P.uv, uv, s );
+   upload_vec( P.lrows, lo, s );
+   upload_vec( P.urows, uo, s );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+}
+
+}  // namespace
+
+void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, double* x, hipStream_t s )
+{
+   if( ! P || P->kind == TNLMI_PRECOND_NONE ) {
+      copy( n, b, x, s );
+      return;
+   }
+   if( P->kind == TNLMI_PRECOND_DIAGONAL ) {
+      divide( n, b, P->diag.ptr, x, s );  // Diagonal.hpp:38-43: x = b / diagonal
+      return;
+   }
+   // ILU0.hpp:99-113: triangularSolveLower then triangularSolveUpper<reversed rows>
+   for( size_t l = 0; l + 1 < P->llev.size(); l++ ) {
+      const int64_t cnt = P->llev[ l + 1 ] - P->llev[ l ];
+      if( cnt )
+         hipLaunchKernelGGL( lower_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
+                             P->lrows.ptr + P->llev[ l ], cnt, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
+   }
+   for( size_t l = 0; l + 1 < P->ulev.size(); l++ ) {
+      const int64_t cnt = P->ulev[ l + 1 ] - P->ulev[ l ];
+      if( cnt )
+         hipLaunchKernelGGL( upper_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
+                             P->urows.ptr + P->ulev[ l ], cnt, n, P->up.ptr, P->uc.ptr, P->uv.ptr, x );
+   }
+}
+
+// y = A x (FP64 vectors and accumulation) with the matrix or its plan
+void Operator::apply( const double* x, double* y, hipStream_t s ) const
+{
+   if( plan )
+      plan_execute( *plan, TNLMI_F64, x, y, TNLMI_F64, 1.0, 0.0, 0, rows, s );
+   else {
+      const tnlmi_launch cfg{ TNLMI_MAP_AUTO, 1 };
+      spmv_dispatch( *A, TNLMI_F64, x, y, TNLMI_F64, 1.0, 0.0, 0, rows, cfg, s );
+   }
+}
+
+// ---------------------------------------------------------- solver state
+namespace {
+
+// IterativeSolver.hpp:98-143
+struct Stop
+{
+   tnlmi_solver_state* st;
+   int64_t it = 0;
+   double res = 0;
+   bool check() const
+   {
+      return ! ( std::isnan( res ) || it > st->max_iterations
+                 || ( res > st->divergence_residue && it >= st->min_iterations )
+                 || ( res < st->convergence_residue && it >= st->min_iterations ) );
+   }
+   bool next()
+   {
+      const bool r = check();
+      it++;
+      return r;
+   }
+   bool converged() const
+   {
+      if( std::isnan( res ) )
+         return false;
+      if( res > st->divergence_residue && it > st->min_iterations )
+         return false;
+      if( it >= st->max_iterations )
+         return false;
+      if( res > st->convergence_residue )
+         return false;
+      return true;
+   }
+};
+
+// pinned host mirror of a few device scalars
+struct HostScalars
+{
+   double* h = nullptr;
+   HostScalars() { TNLMI_HIP( hipHostMalloc( (void**) &h, 64 * sizeof( double ), hipHostMallocDefault ) ); }
+   ~HostScalars()
+   {
+      if( h )
+         (void) hipHostFree( h );
+   }
+   // synchronous read of k device scalars starting at d
+   const double* read( const double* d, int k, hipStream_t s )
+   {
+      TNLMI_HIP( hipMemcpyAsync( h, d, k * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipStreamSynchronize( s ) );
+      return h;
+   }
+};
+
+// ---- CG kernels (CG.hpp:60-105), scalars in sc[]:
+// 0 s1, 1 s2, 2 normb, 3 residue, 4 alpha, 5 beta, 6 s1_new
+__global__ void cg_alpha( double* sc )
+{
+   // alpha = s1 / s2 (s2 == 0 is caught by the host before the update)
+   sc[ 4 ] = sc[ 1 ] != 0.0 ? sc[ 0 ] / sc[ 1 ] : 0.0;
+}
+
+__global__ void cg_update( int64_t n, const double* __restrict__ p, const double* __restrict__ Ap,
+                           double* __restrict__ x, double* __restrict__ r, const double* __restrict__ sc )
+{
+   const double a = sc[ 4 ];
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x ) {
+      x[ i ] += a * p[ i ];
+      r[ i ] -= a * Ap[ i ];
+   }
+}
+
+__global__ void cg_beta( double* sc )
+{
+   // s2 = s1; s1 = (r, z); beta = s1 / s2; residue = sqrt(s1) / normb
+   const double s2 = sc[ 0 ], s1 = sc[ 6 ];
+   sc[ 1 ] = s2;
+   sc[ 0 ] = s1;
+   sc[ 5 ] = s2 == 0.0 ? 0.0 : s1 / s2;
+   sc[ 3 ] = sqrt( s1 ) / sc[ 2 ];
+}
+
+__global__ void cg_dir( int64_t n, const double* __restrict__ z, double* __restrict__ p, const double* __restrict__ sc )
+{
+   const double b = sc[ 5 ];
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      p[ i ] = z[ i ] + b * p[ i ];
+}
+
+__global__ void cg_init( double* sc )
+{
+   sc[ 3 ] = sqrt( sc[ 0 ] ) / sc[ 2 ];
+}
+
+inline dim3 vgrid( int64_t n )
+{
+   return dim3( (unsigned) std::max< int64_t >( 1, std::min< int64_t >( 8192, ceil_div( n, 256 ) ) ) );
+}
+
+void cg_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x, tnlmi_solver_state* st,
+               hipStream_t s )
+{
+   const int64_t n = A.rows;
+   DeviceArray< double > r, p, Ap, z, sc;
+   r.allocate( n );
+   p.allocate( n );
+   Ap.allocate( n );
+   z.allocate( n );
+   sc.allocate( 8 );
+   RedWork w;
+   HostScalars hs;
+   Stop S{ st };
+   const bool pre = M && M->kind != TNLMI_PRECOND_NONE;
+   // normb = ||M^{-1} b|| or ||b||  (CG.hpp:24-29)
+   if( pre ) {
+      precond_apply( M, n, b, r.ptr, s );
+      nrm2( n, r.ptr, sc.ptr + 2, w, s );
+   }
+   else
+      nrm2( n, b, sc.ptr + 2, w, s );
+   if( hs.read( sc.ptr + 2, 1, s )[ 0 ] == 0.0 ) {
+      fill( n, 0.0, x, s );
+      st->iterations = 0;
+      st->residue = 0.0;
+      st->converged = 1;
+      return;
+   }
+   // r = b - A x; z = M^{-1} r; p = z; s1 = (r, z)
+   A.apply( x, r.ptr, s );
+   rsub( n, b, r.ptr, s );
+   const double* zz = r.ptr;
+   if( pre ) {
+      precond_apply( M, n, r.ptr, z.ptr, s );
+      zz = z.ptr;
+   }
+   copy( n, zz, p.ptr, s );
+   dot( n, r.ptr, zz, sc.ptr + 0, w, s );
+   hipLaunchKernelGGL( cg_init, dim3( 1 ), dim3( 1 ), 0, s, sc.ptr );
+   S.res = hs.read( sc.ptr + 3, 1, s )[ 0 ];
+   while( S.next() ) {
+      A.apply( p.ptr, Ap.ptr, s );
+      dot( n, Ap.ptr, p.ptr, sc.ptr + 1, w, s );
+      // s2 == 0 => p = 0 => r = 0: the solution is found (CG.hpp:66-70)
+      if( hs.read( sc.ptr + 1, 1, s )[ 0 ] == 0.0 ) {
+         S.res = 0.0;
+         break;
+      }
+      hipLaunchKernelGGL( cg_alpha, dim3( 1 ), dim3( 1 ), 0, s, sc.ptr );
+      hipLaunchKernelGGL( cg_update, vgrid( n ), dim3( 256 ), 0, s, n, p.ptr, Ap.ptr, x, r.ptr, sc.ptr );
+      if( pre )
+         precond_apply( M, n, r.ptr, z.ptr, s );
+      dot( n, r.ptr, zz, sc.ptr + 6, w, s );
+      hipLaunchKernelGGL( cg_beta, dim3( 1 ), dim3( 1 ), 0, s, sc.ptr );
+      hipLaunchKernelGGL( cg_dir, vgrid( n ), dim3( 256 ), 0, s, n, zz, p.ptr, sc.ptr );
+      S.res = hs.read( sc.ptr + 3, 1, s )[ 0 ];
+   }
+   st->iterations = S.it;
+   st->residue = S.res;
+   st->converged = S.converged();
+}
+
+// ---- Jacobi (Jacobi.hpp:65-125)
+__global__ void jacobi_step( int64_t n, const double* __restrict__ Ain, const double* __restrict__ in,
+                             const double* __restrict__ b, const double* __restrict__ d, double omega,
+                             double* __restrict__ out )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      out[ i ] = in[ i ] + omega / d[ i ] * ( b[ i ] - Ain[ i ] );
+}
+
+__global__ void diff_k( int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o )
+{
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      o[ i ] = a[ i ] - b[ i ];
+}
+
+__global__ void ratio_k( const double* a, const double* b, double* o )
+{
+   o[ 0 ] = a[ 0 ] / b[ 0 ];
+}
+
+// ||A x - b|| / bnorm into d_out[0] (LinearResidueGetter.hpp:11-23)
+void residue_dev( const Operator& A, const double* x, const double* b, const double* d_bnorm, double* tmp,
+                  double* d_out, RedWork& w, hipStream_t s )
+{
+   const int64_t n = A.rows;
+   A.apply( x, tmp, s );
+   hipLaunchKernelGGL( diff_k, vgrid( n ), dim3( 256 ), 0, s, n, tmp, b, tmp );
+   nrm2( n, tmp, d_out, w, s, 1 );
+   hipLaunchKernelGGL( ratio_k, dim3( 1 ), dim3( 1 ), 0, s, d_out, d_bnorm, d_out );
+}
+
+void jacobi_solve( const Operator& A, const tnlmi_segments& seg, const tnlmi_matrix& M, const double* b, double* x,
+                   double omega, int64_t period, tnlmi_solver_state* st, hipStream_t s )
+{
+   const int64_t n = A.rows;
+   DeviceArray< double > d, aux, tmp, sc;
+   d.allocate( n );
+   aux.allocate( n );
+   tmp.allocate( n );
+   sc.allocate( 4 );
+   RedWork w;
+   HostScalars hs;
+   with_value_type( M.value_type, [ & ]( auto tag ) {
+      using V = std::remove_const_t< std::remove_pointer_t< decltype( tag ) > >;
+      hipLaunchKernelGGL( diag_kernel< V >, dim3( (unsigned) ceil_div( n, 256 ) ), dim3( 256 ), 0, s,
+                          make_view( seg ), M.column_indexes, (const V*) M.values, d.ptr );
+   } );
+   Stop S{ st };
+   S.res = st->convergence_residue + 1.0;
+   nrm2( n, b, sc.ptr + 0, w, s );
+   copy( n, x, aux.ptr, s );
+   auto step = [ & ]( const double* in, double* out ) {
+      A.apply( in, tmp.ptr, s );
+      hipLaunchKernelGGL( jacobi_step, vgrid( n ), dim3( 256 ), 0, s, n, tmp.ptr, in, b, d.ptr, omega, out );
+   };
+   auto residue = [ & ]() {
+      residue_dev( A, x, b, sc.ptr + 0, tmp.ptr, sc.ptr + 1, w, s );
+      return hs.read( sc.ptr + 1, 1, s )[ 0 ];
+   };
+   while( S.next() ) {
+      step( x, aux.ptr );
+      if( S.it % period == 0 )
+         S.res = residue();
+      S.it++;
+      step( aux.ptr, x );
+      if( S.it % period == 0 )
+         S.res = residue();
+   }
+   S.res = residue();
+   st->iterations = S.it;
+   st->residue = S.res;
+   st->converged = S.converged();
+}
+
+// ---- GMRES (GMRES.hpp:71-431, 645-770) with CGS/CGSR/MGS/MGSR orthogonalisation.
+// Device layout of the small FP64 state g[]:
+//   H: (m+1)*m at 0, s: m+1, cs: m+1, sn: m+1, y: m+1, tmp: 32, scalars: 8
+struct GLayout
+{
+   int m;
+   int64_t H, s, cs, sn, y, tmp, sc, total;
+   explicit GLayout( int m_ ) : m( m_ )
+   {
+      H = 0;
+      s = H + ( m + 1 ) * m;
+      cs = s + m + 1;
+      sn = cs + m + 1;
+      y = sn + m + 1;
+      tmp = y + m + 1;
+      sc = tmp + 32;
+      total = sc + 8;  // sc: 0 normb, 1 beta, 2 normw, 3 residue
+   }
+};
+
+__global__ void gm_start( double* g, GLayout L, int m )
+{
+   // H = 0, s = 0, s[0] = beta
+   for( int64_t i = threadIdx.x; i < ( m + 1 ) * m; i += blockDim.x )
+      g[ L.H + i ] = 0.0;
+   for( int i = threadIdx.x; i <= m; i += blockDim.x )
+      g[ L.s + i ] = 0.0;
+   __syncthreads();
+   if( threadIdx.x == 0 )
+      g[ L.s ] = g[ L.sc + 1 ];
+}
+
+// H[k, i] += h[k] for k <= i (one reorthogonalisation pass of CGS)
+__global__ void gm_hadd( double* g, GLayout L, int i, int m, const double* h )
+{
+   const int k = threadIdx.x;
+   if( k <= i )
+      g[ L.H + k + i * ( m + 1 ) ] += h[ k ];
+}
+
+// MGS step: h = (w, v_k) already in tmp[0]: H[k, i] += h
+__global__ void gm_hadd1( double* g, GLayout L, int k, int i, int m )
+{
+   g[ L.H + k + i * ( m + 1 ) ] += g[ L.tmp ];
+}
+
+// w -= h * v_k with h = tmp[0]
+__global__ void gm_axpy( int64_t n, const double* __restrict__ vk, double* __restrict__ w, const double* __restrict__ g,
+                         GLayout L )
+{
+   const double h = g[ L.tmp ];
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      w[ i ] = w[ i ] - h * vk[ i ];
+}
+
+__device__ void gen_rot( double dx, double dy, double& c, double& sn )
+{
+   // generatePlaneRotation, GMRES.hpp:729-746
+   if( dy == 0.0 ) {
+      c = 1.0;
+      sn = 0.0;
+   }
+   else if( fabs( dy ) > fabs( dx ) ) {
+      const double t = dx / dy;
+      sn = 1.0 / sqrt( 1.0 + t * t );
+      c = t * sn;
+   }
+   else {
+      const double t = dy / dx;
+      c = 1.0 / sqrt( 1.0 + t * t );
+      sn = t * c;
+   }
+}
+__device__ void app_rot( double& dx, double& dy, double c, double sn )
+{
+   const double t = c * dx + sn * dy;
+   dy = c * dy - sn * dx;
+   dx = t;
+}
+
+// H[i+1, i] = ||w|| (already in sc[2]); Givens rotations on column i;
+// residue = |s[i+1]| / normb   (GMRES.hpp:327-341, 758-770)
+__global__ void gm_givens( double* g, GLayout L, int i, int m )
+{
+   double* H = g + L.H;
+   double* s = g + L.s;
+   double* cs = g + L.cs;
+   double* sn = g + L.sn;
+   H[ i + 1 + i * ( m + 1 ) ] = g[ L.sc + 2 ];
+   for( int k = 0; k < i; k++ )
+      app_rot( H[ k + i * ( m + 1 ) ], H[ k + 1 + i * ( m + 1 ) ], cs[ k ], sn[ k ] );
+   if( H[ i + 1 + i * ( m + 1 ) ] != 0.0 ) {
+      gen_rot( H[ i + i * ( m + 1 ) ], H[ i + 1 + i * ( m + 1 ) ], cs[ i ], sn[ i ] );
+      app_rot( H[ i + i * ( m + 1 ) ], H[ i + 1 + i * ( m + 1 ) ], cs[ i ], sn[ i ] );
+      app_rot( s[ i ], s[ i + 1 ], cs[ i ], sn[ i ] );
+   }
+   g[ L.sc + 3 ] = fabs( s[ i + 1 ] ) / g[ L.sc + 0 ];
+}
+
+// y = H^{-1} s on the leading (k+1) x (k+1) block (update, GMRES.hpp:645-662);
+// flag sc[4] = 1 when a zero pivot is met (the reference throws)
+__global__ void gm_backsub( double* g, GLayout L, int k, int m )
+{
+   const double* H = g + L.H;
+   double* y = g + L.y;
+   for( int i = 0; i <= m; i++ )
+      y[ i ] = g[ L.s + i ];
+   g[ L.sc + 4 ] = 0.0;
+   for( int i = k; i >= 0; i-- ) {
+      if( H[ i + i * ( m + 1 ) ] == 0.0 ) {
+         g[ L.sc + 4 ] = 1.0;
+         return;
+      }
+      y[ i ] /= H[ i + i * ( m + 1 ) ];
+      for( int j = i - 1; j >= 0; j-- )
+         y[ j ] -= H[ j + i * ( m + 1 ) ] * y[ i ];
+   }
+}
+
+__global__ void gm_set_scalar( double* dst, const double* src )
+{
+   dst[ 0 ] = src[ 0 ];
+}
+
+__global__ void gm_residue_beta( double* g, GLayout L )
+{
+   g[ L.sc + 3 ] = g[ L.sc + 1 ] / g[ L.sc + 0 ];
+}
+
+// v = w / ||w||: scale by 1 / sc[2] (the reference multiplies by (1.0 / normw))
+__global__ void gm_normalize( int64_t n, const double* __restrict__ w, double* __restrict__ v,
+                              const double* __restrict__ g, int64_t idx )
+{
+   const double inv = 1.0 / g[ idx ];
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      v[ i ] = inv * w[ i ];
+}
+
+void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x, int variant, int rmin,
+                  int rmax, int dmin, int dmax, tnlmi_solver_state* st, hipStream_t s )
+{
+   if( variant == 4 )
+      fail( TNLMI_NOT_IMPLEMENTED, "GMRES: the CWY variant is not implemented on this device" );
+   if( rmin <= 0 || rmax <= 0 || rmin > rmax || dmin < 0 || dmax < 0 || dmin > dmax ) {
+      st->converged = 0;
+      st->iterations = 0;
+      return;  // the reference prints and returns false (GMRES.hpp:76-86)
+   }
+   if( rmax > 31 )
+      fail( TNLMI_NOT_IMPLEMENTED, "GMRES: restarting-max above 31 is not supported by the GPU orthogonalisation" );
+   const int64_t n = A.rows;
+   const int64_t ld = ( n + 31 ) / 32 * 32;  // columns of V aligned to 256 bytes (GMRES.hpp:775-778)
+   const bool pre = M && M->kind != TNLMI_PRECOND_NONE;
+   GLayout L( rmax );
+   DeviceArray< double > V, r, wv, tmp, g;
+   V.allocate( ld * ( rmax + 1 ) );
+   r.allocate( n );
+   wv.allocate( n );
+   tmp.allocate( n );
+   g.allocate( L.total );
+   RedWork rw;
+   HostScalars hs;
+   Stop S{ st };
+   double* G = g.ptr;
+   auto matvec_prec = [ & ]( const double* v, double* out ) {  // preconditioned_matvec, :451-465
+      if( pre ) {
+         A.apply( v, tmp.ptr, s );
+         precond_apply( M, n, tmp.ptr, out, s );
+      }
+      else
+         A.apply( v, out, s );
+   };
+   auto compute_residue = [ & ]() {  // compute_residue, :433-449
+      if( pre ) {
+         A.apply( x, tmp.ptr, s );
+         rsub( n, b, tmp.ptr, s );
+         precond_apply( M, n, tmp.ptr, r.ptr, s );
+      }
+      else {
+         A.apply( x, r.ptr, s );
+         rsub( n, b, r.ptr, s );
+      }
+   };
+   // normb
+   if( pre ) {
+      precond_apply( M, n, b, tmp.ptr, s );
+      nrm2( n, tmp.ptr, G + L.sc + 0, rw, s );
+   }
+   else
+      nrm2( n, b, G + L.sc + 0, rw, s );
+   if( hs.read( G + L.sc, 1, s )[ 0 ] == 0.0 ) {
+      fill( n, 0.0, x, s );
+      st->iterations = 0;
+      st->residue = 0;
+      st->converged = 1;
+      return;
+   }
+   compute_residue();
+   nrm2( n, r.ptr, G + L.sc + 1, rw, s );
+   hipLaunchKernelGGL( gm_residue_beta, dim3( 1 ), dim3( 1 ), 0, s, G, L );
+   S.it = 0;
+   const double* h0 = hs.read( G + L.sc, 4, s );
+   double beta = h0[ 1 ];
+   S.res = h0[ 3 ];
+   double beta_ratio = 1;
+   int cycles = 0;
+   int m = rmax;
+   const bool classical = variant <= 1;
+   const int passes = ( variant == 1 || variant == 3 ) ? 2 : 1;
+   auto update = [ & ]( int k ) {
+      hipLaunchKernelGGL( gm_backsub, dim3( 1 ), dim3( 1 ), 0, s, G, L, k, m );
+      if( hs.read( G + L.sc + 4, 1, s )[ 0 ] != 0.0 )
+         fail( TNLMI_RUNTIME_ERROR, "GMRES: zero on the diagonal of the Hessenberg matrix" );
+      gemv_add( n, k + 1, V.ptr, ld, G + L.y, x, s );
+   };
+   while( S.check() ) {
+      if( rmax > rmin && cycles > 0 ) {  // Baker-Jessup-Kolev adaptive restart, :121-146
+         if( beta_ratio > 0.99 )
+            m = rmax;
+         else if( beta_ratio >= 0.175 ) {
+            const int step = dmin + ( dmax - dmin ) / static_cast< float >( rmax - rmin ) * ( m - rmin );
+            if( m - step >= rmin )
+               m -= step;
+            else
+               m = rmax;
+         }
+      }
+      // v_0 = r / beta; H = 0; s = 0; s[0] = beta   (:190-202, :268-280)
+      hipLaunchKernelGGL( gm_normalize, dim3( (unsigned) ceil_div( n, 256 ) ), dim3( 256 ), 0, s, n, r.ptr, V.ptr, G,
+                          L.sc + 1 );
+      hipLaunchKernelGGL( gm_start, dim3( 1 ), dim3( 256 ), 0, s, G, L, m );
+      int o = m;
+      for( int i = 0; i < m && S.next(); i++ ) {
+         const double* vi = V.ptr + i * ld;
+         matvec_prec( vi, wv.ptr );
+         for( int pass = 0; pass < passes; pass++ ) {
+            if( classical ) {
+               multi_dot( n, i + 1, V.ptr, ld, wv.ptr, G + L.tmp, rw, s );
+               hipLaunchKernelGGL( gm_hadd, dim3( 1 ), dim3( 64 ), 0, s, G, L, i, m, G + L.tmp );
+               gemv_sub( n, i + 1, V.ptr, ld, G + L.tmp, wv.ptr, s );
+            }
+            else
+               for( int k = 0; k <= i; k++ ) {
+                  dot( n, wv.ptr, V.ptr + k * ld, G + L.tmp, rw, s );
+                  hipLaunchKernelGGL( gm_hadd1, dim3( 1 ), dim3( 1 ), 0, s, G, L, k, i, m );
+                  hipLaunchKernelGGL( gm_axpy, dim3( (unsigned) std::min< int64_t >( 8192, ceil_div( n, 256 ) ) ),
+                                      dim3( 256 ), 0, s, n, V.ptr + k * ld, wv.ptr, G, L );
+               }
+         }
+         nrm2( n, wv.ptr, G + L.sc + 2, rw, s );
+         hipLaunchKernelGGL( gm_normalize, dim3( (unsigned) ceil_div( n, 256 ) ), dim3( 256 ), 0, s, n, wv.ptr,
+                             V.ptr + ( i + 1 ) * ld, G, L.sc + 2 );
+         hipLaunchKernelGGL( gm_givens, dim3( 1 ), dim3( 1 ), 0, s, G, L, i, m );
+         S.res = hs.read( G + L.sc + 3, 1, s )[ 0 ];
+         if( ! S.check() ) {
+            o = i;
+            break;
+         }
+      }
+      if( o < m ) {
+         update( o );
+         st->iterations = S.it;
+         st->residue = S.res;
+         st->converged = S.converged();
+         return;
+      }
+      update( m - 1 );
+      compute_residue();
+      nrm2( n, r.ptr, G + L.sc + 1, rw, s );
+      hipLaunchKernelGGL( gm_residue_beta, dim3( 1 ), dim3( 1 ), 0, s, G, L );
+      const double* h = hs.read( G + L.sc, 4, s );
+      const double beta_old = beta;
+      beta = h[ 1 ];
+      S.res = h[ 3 ];
+      ++cycles;
+      beta_ratio = beta / beta_old;
+   }
+   st->iterations = S.it;
+   st->residue = S.res;
+   st->converged = S.converged();
+}
+
+}  // namespace
+}  // namespace tnlmi
+
+// --------------------------------------------------------------- C ABI
+namespace {
+
+Operator make_op( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan )
+{
+   if( ! A || ! A->segments )
+      fail( TNLMI_INVALID_ARGUMENT, "solver: matrix is NULL" );
+   if( A->rows != A->columns )
+      fail( TNLMI_INVALID_ARGUMENT, "solver: the matrix must be square" );
+   return Operator{ A, plan, A->rows };
+}
+
+void check_state( tnlmi_solver_state* st )
+{
+   if( ! st )
+      fail( TNLMI_INVALID_ARGUMENT, "solver: state is NULL" );
+}
+
+}  // namespace
+
+extern "C" {
+
+tnlmi_status tnlmi_preconditioner_create( tnlmi_precond_kind kind, const tnlmi_matrix* A, tnlmi_stream stream,
+                                          tnlmi_preconditioner** out )
+{
+   return guarded( [ & ] {
+      if( ! out || ! A || ! A->segments )
+         fail( TNLMI_INVALID_ARGUMENT, "preconditioner: NULL argument" );
+      *out = nullptr;
+      if( A->rows == 0 )
+         fail( TNLMI_INVALID_ARGUMENT, "preconditioner update: the matrix is empty" );
+      if( A->rows != A->columns )
+         fail( TNLMI_INVALID_ARGUMENT, "preconditioner update: matrix must be square" );
+      auto* P = new tnlmi_preconditioner;
+      P->kind = kind;
+      P->n = A->rows;
+      hipStream_t s = as_stream( stream );
+      try {
+         if( kind == TNLMI_PRECOND_DIAGONAL ) {
+            P->diag.allocate( A->rows );
+            with_value_type( A->value_type, [ & ]( auto tag ) {
+               using V = std::remove_const_t< std::remove_pointer_t< decltype( tag ) > >;
+               hipLaunchKernelGGL( diag_kernel< V >, dim3( (unsigned) ceil_div( A->rows, 256 ) ), dim3( 256 ), 0, s,
+                                   make_view( *A->segments ), A->column_indexes, (const V*) A->values, P->diag.ptr );
+            } );
+            TNLMI_HIP( hipGetLastError() );
+            TNLMI_HIP( hipStreamSynchronize( s ) );
+         }
+         else if( kind == TNLMI_PRECOND_ILU0 )
+            ilu0_build( *P, *A, s );
+         else if( kind != TNLMI_PRECOND_NONE )
+            fail( TNLMI_INVALID_ARGUMENT, "preconditioner: unknown kind" );
+      }
+      catch( ... ) {
+         delete P;
+         throw;
+      }
+      *out = P;
+   } );
+}
+
+void tnlmi_preconditioner_destroy( tnlmi_preconditioner* P )
+{
+   delete P;
+}
+
+tnlmi_status tnlmi_preconditioner_solve( const tnlmi_preconditioner* P, const double* d_b, double* d_x,
+                                         tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( ! P || ! d_b || ! d_x )
+         fail( TNLMI_INVALID_ARGUMENT, "preconditioner solve: NULL argument" );
+      precond_apply( P, P->n, d_b, d_x, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+tnlmi_status tnlmi_cg_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                             const double* d_b, double* d_x, tnlmi_solver_state* st, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      check_state( st );
+      const Operator op = make_op( A, plan );
+      cg_solve( op, M, d_b, d_x, st, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+tnlmi_status tnlmi_gmres_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                                tnlmi_gmres_variant variant, int32_t restarting_min, int32_t restarting_max,
+                                int32_t step_min, int32_t step_max, const double* d_b, double* d_x,
+                                tnlmi_solver_state* st, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      check_state( st );
+      const Operator op = make_op( A, plan );
+      gmres_solve( op, M, d_b, d_x, variant, restarting_min, restarting_max, step_min, step_max, st,
+                   as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+tnlmi_status tnlmi_jacobi_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, double omega,
+                                 int64_t residue_period, const double* d_b, double* d_x, tnlmi_solver_state* st,
+                                 tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      check_state( st );
+      const Operator op = make_op( A, plan );
+      if( residue_period <= 0 )
+         fail( TNLMI_INVALID_ARGUMENT, "Jacobi: residue period must be positive" );
+      jacobi_solve( op, *A->segments, *A, d_b, d_x, omega, residue_period, st, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+tnlmi_status tnlmi_true_residual( const tnlmi_matrix* A, const double* d_x, const double* d_b, double* residual,
+                                  tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( ! residual )
+         fail( TNLMI_INVALID_ARGUMENT, "true_residual: NULL output" );
+      const Operator op = make_op( A, nullptr );
+      hipStream_t s = as_stream( stream );
+      DeviceArray< double > tmp, sc;
+      tmp.allocate( std::max< int64_t >( 1, op.rows ) );
+      sc.allocate( 4 );
+      RedWork w;
+      nrm2( op.rows, d_b, sc.ptr + 0, w, s );
+      residue_dev( op, d_x, d_b, sc.ptr + 0, tmp.ptr, sc.ptr + 1, w, s );
+      double h[ 2 ];
+      TNLMI_HIP( hipMemcpyAsync( h, sc.ptr, 2 * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipStreamSynchronize( s ) );
+      *residual = h[ 1 ];
+   } );
+}
+
+}  // extern "C"

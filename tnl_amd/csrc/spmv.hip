@@ -13,17 +13,10 @@
 
 #include "segments.hpp"
 #include "spmv.hpp"
+#include "spmv_tile.hpp"
 
 namespace tnlmi {
 namespace {
-
-// LDS ordering between lanes of one wavefront (no block barrier needed).
-__device__ __forceinline__ void wave_sync()
-{
-   __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-   __builtin_amdgcn_wave_barrier();
-   __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-}
 
 template< typename V, typename X, typename C >
 __device__ __forceinline__ C fetch( const int32_t* __restrict__ cols, const V* __restrict__ vals,
@@ -40,16 +33,6 @@ __device__ __forceinline__ C fetch_nt( const int32_t* __restrict__ cols, const V
    const int32_t c = nt_load( cols + g );
    const C v = nt_load_val< V, C >( vals, g );
    return c >= 0 ? v * (C) x[ c ] : C( 0 );
-}
-
-// SparseMatrixBase.hpp:346-389
-template< typename X, typename C >
-__device__ __forceinline__ void keep( X* __restrict__ y, int64_t r, C v, C alpha, C beta )
-{
-   if( beta == C( 0 ) )
-      y[ r ] = alpha == C( 1 ) ? X( v ) : X( alpha * v );
-   else
-      y[ r ] = alpha == C( 1 ) ? X( beta * (C) y[ r ] + v ) : X( beta * (C) y[ r ] + alpha * v );
 }
 
 template< typename V, typename X, typename C >
@@ -194,8 +177,6 @@ __global__ void __launch_bounds__( 256 ) csr_dynamic( const int32_t* __restrict_
 // row-length distribution.  Workgroups are renumbered so that each XCD walks
 // one contiguous range of tiles (x entries shared by neighbouring rows stay in
 // one L2).
-constexpr int kUnroll = kTileNnz / 64;
-
 template< typename V, typename X, typename C, bool MAPPED >
 __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ off,
                                                     const int32_t* __restrict__ tile_rows, int64_t tiles, int64_t nnz,
@@ -207,88 +188,21 @@ __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ 
    const int64_t t = xcd_contiguous( blockIdx.x, gridDim.x ) * 4 + wave;
    if( t >= tiles )
       return;
-   const int64_t e0 = t * kTileNnz;
-   const int64_t e1 = min( e0 + kTileNnz, nnz );
-   const int32_t nel = (int32_t) max( (int64_t) 0, e1 - e0 );
-   int64_t rs = tile_rows[ t ], re = tile_rows[ t + 1 ];
-   // is a row with rows index r (MAPPED: position in the sorted order) written?
+   // is the row at position r (MAPPED: position in the sorted order) written?
    auto in_range = [ & ]( int64_t r ) -> bool {
       const int64_t o = MAPPED ? (int64_t) row_map[ r ] : r;
       return o >= a.begin && o < a.end;
    };
-   auto store = [ & ]( int64_t r, C v ) {
-      if( in_range( r ) )
-         keep( a.y, MAPPED ? (int64_t) row_map[ r ] : r, v, a.alpha, a.beta );
-   };
-   C* s = lds[ wave ];
-   {
-      const int32_t* __restrict__ cp = a.cols + e0;
-      const V* __restrict__ vp = a.vals + e0;
-      int32_t c[ kUnroll ];
-      C v[ kUnroll ];
-#pragma unroll
-      for( int u = 0; u < kUnroll; u++ ) {
-         const int32_t i = lane + 64 * u;
-         c[ u ] = i < nel ? nt_load( cp + i ) : -1;
-      }
-#pragma unroll
-      for( int u = 0; u < kUnroll; u++ ) {
-         const int32_t i = lane + 64 * u;
-         v[ u ] = i < nel ? nt_load_val< V, C >( vp, i ) : C( 0 );
-      }
-#pragma unroll
-      for( int u = 0; u < kUnroll; u++ ) {
-         const int32_t i = lane + 64 * u;
-         if( i < nel )
-            s[ i ] = c[ u ] >= 0 ? v[ u ] * (C) a.x[ c[ u ] ] : C( 0 );
-      }
-   }
-   wave_sync();
-   // carry: the part of a row that started in an earlier tile
-   {
-      int64_t rc = -1;
-      int32_t cend = 0;
-      if( e0 < nnz && rs > 0 && ( rs == a.n_rows || off[ rs ] > e0 ) ) {
-         rc = rs - 1;
-         cend = (int32_t) ( min( (int64_t) off[ rs ], e1 ) - e0 );
-      }
-      C acc = 0;
-      if( rc >= 0 )
-         for( int32_t i = lane; i < cend; i += 64 )
-            acc += s[ i ];
-      acc = group_sum< 64 >( acc );
-      if( lane == 0 ) {
+   csr_tile< V, X, C >(
+      off, a.n_rows, a.cols, a.vals, a.x, t, tile_rows[ t ], tile_rows[ t + 1 ], nnz, lds[ wave ], lane,
+      [ & ]( int64_t r, C v ) {
+         if( in_range( r ) )
+            keep( a.y, MAPPED ? (int64_t) row_map[ r ] : r, v, a.alpha, a.beta );
+      },
+      [ & ]( int64_t rc, C v ) {
          carry_row[ t ] = rc >= 0 && in_range( rc ) ? (int32_t) rc : -1;
-         carry_val[ t ] = acc;
-      }
-   }
-   const int32_t nrows = (int32_t) ( re - rs );
-   if( nrows >= 64 ) {
-      for( int64_t r = rs + lane; r < re; r += 64 ) {
-         const int32_t b = (int32_t) ( off[ r ] - e0 ), e = (int32_t) ( min( (int64_t) off[ r + 1 ], e1 ) - e0 );
-         C acc = 0;
-         for( int32_t i = b; i < e; i++ )
-            acc = acc + s[ i ];
-         store( r, acc );
-      }
-   }
-   else if( nrows > 0 ) {
-      // lanes per row: largest power of two with nrows * G <= 64
-      int G = 64;
-      while( G > 1 && nrows * G > 64 )
-         G >>= 1;
-      const int grp = lane / G, gl = lane % G;
-      C acc = 0;
-      const int64_t r = rs + grp;
-      if( grp < nrows ) {
-         const int32_t b = (int32_t) ( off[ r ] - e0 ), e = (int32_t) ( min( (int64_t) off[ r + 1 ], e1 ) - e0 );
-         for( int32_t i = b + gl; i < e; i += G )
-            acc += s[ i ];
-      }
-      acc = group_sum_dyn( acc, G );
-      if( gl == 0 && grp < nrows )
-         store( r, acc );
-   }
+         carry_val[ t ] = v;
+      } );
 }
 
 // Adds the carried partial sums of rows that span several tiles, in tile

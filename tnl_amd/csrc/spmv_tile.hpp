@@ -1,0 +1,117 @@
+// spmv_tile.hpp -- the wavefront tile of the element-split CSR kernels
+// (shared by the plain CSR path and the XCD column-split plan).
+#pragma once
+
+#include "common.hpp"
+#include "segments.hpp"
+
+namespace tnlmi {
+
+// LDS ordering between lanes of one wavefront (no workgroup barrier needed).
+__device__ __forceinline__ void wave_sync()
+{
+   __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+}
+
+// Result store of one row (Matrices/SparseMatrixBase.hpp:346-389).
+template< typename X, typename C >
+__device__ __forceinline__ void keep( X* __restrict__ y, int64_t r, C v, C alpha, C beta )
+{
+   if( beta == C( 0 ) )
+      y[ r ] = alpha == C( 1 ) ? X( v ) : X( alpha * v );
+   else
+      y[ r ] = alpha == C( 1 ) ? X( beta * (C) y[ r ] + v ) : X( beta * (C) y[ r ] + alpha * v );
+}
+
+constexpr int kTileUnroll = kTileNnz / 64;
+
+// Tile t of a CSR element stream (offsets `off`, n_rows rows, nnz elements):
+// the elements [t*kTileNnz, (t+1)*kTileNnz) and the rows [rs, re) whose first
+// element lies there.
+// Phase 1 streams the tile's columns/values once with non-temporal, fully
+// coalesced loads (kTileUnroll independent loads per lane in flight), gathers
+// x and stages the products in LDS (s, kTileNnz entries of this wavefront).
+// Phase 2: carry(rc, v) receives the partial sum of the row rc that started
+// in an earlier tile (rc = -1 if none) -- called by lane 0 only; store(r, v)
+// the sum of each owned row's part in this tile -- 64/pow2(rows) lanes per
+// row, or one lane per row summing left to right from 0 (the reference's host
+// order) when the tile holds 64 rows or more.
+template< typename V, typename X, typename C, typename Store, typename Carry >
+__device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64_t n_rows,
+                                          const int32_t* __restrict__ cols, const V* __restrict__ vals,
+                                          const X* __restrict__ x, int64_t t, int64_t rs, int64_t re, int64_t nnz,
+                                          C* __restrict__ s, int lane, Store&& store, Carry&& carry )
+{
+   const int64_t e0 = t * kTileNnz;
+   const int64_t e1 = min( e0 + kTileNnz, nnz );
+   const int32_t nel = (int32_t) max( (int64_t) 0, e1 - e0 );
+   {
+      const int32_t* __restrict__ cp = cols + e0;
+      const V* __restrict__ vp = vals + e0;
+      int32_t c[ kTileUnroll ];
+      C v[ kTileUnroll ];
+#pragma unroll
+      for( int u = 0; u < kTileUnroll; u++ ) {
+         const int32_t i = lane + 64 * u;
+         c[ u ] = i < nel ? nt_load( cp + i ) : -1;
+      }
+#pragma unroll
+      for( int u = 0; u < kTileUnroll; u++ ) {
+         const int32_t i = lane + 64 * u;
+         v[ u ] = i < nel ? nt_load_val< V, C >( vp, i ) : C( 0 );
+      }
+#pragma unroll
+      for( int u = 0; u < kTileUnroll; u++ ) {
+         const int32_t i = lane + 64 * u;
+         if( i < nel )
+            s[ i ] = c[ u ] >= 0 ? v[ u ] * (C) x[ c[ u ] ] : C( 0 );
+      }
+   }
+   wave_sync();
+   {
+      // the row containing element e0 started earlier iff no owned row starts at e0
+      int64_t rc = -1;
+      int32_t cend = 0;
+      if( e0 < nnz && rs > 0 && ( rs == n_rows || off[ rs ] > e0 ) ) {
+         rc = rs - 1;
+         cend = (int32_t) ( min( (int64_t) off[ rs ], e1 ) - e0 );
+      }
+      C acc = 0;
+      if( rc >= 0 )
+         for( int32_t i = lane; i < cend; i += 64 )
+            acc += s[ i ];
+      acc = group_sum< 64 >( acc );
+      if( lane == 0 )
+         carry( rc, acc );
+   }
+   const int32_t nrows = (int32_t) ( re - rs );
+   if( nrows >= 64 ) {
+      for( int64_t r = rs + lane; r < re; r += 64 ) {
+         const int32_t b = (int32_t) ( off[ r ] - e0 ), e = (int32_t) ( min( (int64_t) off[ r + 1 ], e1 ) - e0 );
+         C acc = 0;
+         for( int32_t i = b; i < e; i++ )
+            acc = acc + s[ i ];
+         store( r, acc );
+      }
+   }
+   else if( nrows > 0 ) {
+      int G = 64;  // lanes per row: largest power of two with nrows * G <= 64
+      while( G > 1 && nrows * G > 64 )
+         G >>= 1;
+      const int grp = lane / G, gl = lane % G;
+      C acc = 0;
+      const int64_t r = rs + grp;
+      if( grp < nrows ) {
+         const int32_t b = (int32_t) ( off[ r ] - e0 ), e = (int32_t) ( min( (int64_t) off[ r + 1 ], e1 ) - e0 );
+         for( int32_t i = b + gl; i < e; i += G )
+            acc += s[ i ];
+      }
+      acc = group_sum_dyn( acc, G );
+      if( gl == 0 && grp < nrows )
+         store( r, acc );
+   }
+}
+
+}  // namespace tnlmi

@@ -1,0 +1,196 @@
+"""Solvers::Linear mirror: CG, GMRES, Jacobi and the Diagonal / ILU0
+preconditioners of the reference (Solvers/Linear/*.h), running on the GPU
+through the C ABI.  Method names and defaults follow the reference
+(IterativeSolver.h:226-235, GMRES.hpp:18-69, Jacobi.h:121-123)."""
+import ctypes
+import sys
+
+from . import _lib
+from . import _stream, InvalidArgument
+
+NONE, DIAGONAL, ILU0_KIND = 0, 1, 2
+GMRES_VARIANTS = {"CGS": 0, "CGSR": 1, "MGS": 2, "MGSR": 3, "CWY": 4}
+
+
+def _f64(t):
+    import torch
+    if t.dtype != torch.float64 or not t.is_cuda:
+        raise InvalidArgument(_lib.INVALID_ARGUMENT, "solvers work on FP64 device vectors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Preconditioner:
+    kind = NONE
+
+    def __init__(self, matrix=None):
+        self.handle = None
+        if matrix is not None:
+            self.update(matrix)
+
+    def update(self, matrix):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tnlmi_preconditioner_create(self.kind, ctypes.byref(matrix._m), _stream(),
+                                                          ctypes.byref(h)))
+        self.handle = h
+        self.n = matrix.rows
+
+    def solve(self, b, x):
+        _lib.check(_lib.lib().tnlmi_preconditioner_solve(self.handle, _f64(b), _f64(x), _stream()))
+        return x
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        try:
+            if h is not None and h.value and _lib._lib is not None:
+                _lib.lib().tnlmi_preconditioner_destroy(h)
+        except (AttributeError, TypeError):
+            pass
+        self.handle = None
+
+
+class Diagonal(Preconditioner):
+    """Preconditioners::Diagonal (Jacobi)."""
+    kind = DIAGONAL
+
+
+class ILU0(Preconditioner):
+    """Preconditioners::ILU0."""
+    kind = ILU0_KIND
+
+
+class IterativeSolver:
+    def __init__(self):
+        self.max_iterations = 1_000_000_000
+        self.min_iterations = 0
+        self.convergence_residue = 1e-6
+        self.divergence_residue = sys.float_info.max
+        self.iterations = 0
+        self.residue = 0.0
+        self.matrix = None
+        self.plan = None
+        self.preconditioner = None
+
+    def setMatrix(self, matrix, plan=None):
+        self.matrix = matrix
+        self.plan = plan
+
+    def setPreconditioner(self, p):
+        self.preconditioner = p
+
+    def setMaxIterations(self, v):
+        self.max_iterations = int(v)
+
+    def setMinIterations(self, v):
+        self.min_iterations = int(v)
+
+    def setConvergenceResidue(self, v):
+        self.convergence_residue = float(v)
+
+    def setDivergenceResidue(self, v):
+        self.divergence_residue = float(v)
+
+    def getIterations(self):
+        return self.iterations
+
+    def getResidue(self):
+        return self.residue
+
+    def _state(self):
+        return _lib.SolverState(self.max_iterations, self.min_iterations, self.convergence_residue,
+                                self.divergence_residue, 0, 0.0, 0)
+
+    def _finish(self, st):
+        self.iterations = st.iterations
+        self.residue = st.residue
+        return bool(st.converged)
+
+    def _handles(self):
+        if self.matrix is None:
+            raise _lib.LogicError(_lib.LOGIC_ERROR, "No matrix was set. Call setMatrix() before solve().")
+        plan = self.plan.handle if self.plan is not None else None
+        pre = self.preconditioner.handle if self.preconditioner is not None else None
+        return ctypes.byref(self.matrix._m), plan, pre
+
+
+class CG(IterativeSolver):
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_cg_solve(m, plan, pre, _f64(b), _f64(x), ctypes.byref(st), _stream()))
+        return self._finish(st)
+
+
+class GMRES(IterativeSolver):
+    def __init__(self):
+        super().__init__()
+        self.variant = "MGSR"  # gmres-variant default of configSetup (GMRES.hpp:23)
+        self.restarting_min = 10
+        self.restarting_max = 10
+        self.restarting_step_min = 3
+        self.restarting_step_max = 3
+
+    def setup(self, variant=None, restarting_min=None, restarting_max=None, step_min=None, step_max=None):
+        if variant is not None:
+            if variant not in GMRES_VARIANTS:
+                return False
+            self.variant = variant
+        if restarting_min is not None:
+            self.restarting_min = restarting_min
+        if restarting_max is not None:
+            self.restarting_max = restarting_max
+        if step_min is not None:
+            self.restarting_step_min = step_min
+        if step_max is not None:
+            self.restarting_step_max = step_max
+        return True
+
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_gmres_solve(m, plan, pre, GMRES_VARIANTS[self.variant], self.restarting_min,
+                                                self.restarting_max, self.restarting_step_min,
+                                                self.restarting_step_max, _f64(b), _f64(x), ctypes.byref(st),
+                                                _stream()))
+        return self._finish(st)
+
+
+class Jacobi(IterativeSolver):
+    def __init__(self):
+        super().__init__()
+        self.omega = 1.0
+        self.residue_period = 4
+
+    def setOmega(self, w):
+        self.omega = float(w)
+
+    def setResiduePeriod(self, p):
+        self.residue_period = int(p)
+
+    def solve(self, b, x):
+        m, plan, _ = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_jacobi_solve(m, plan, self.omega, self.residue_period, _f64(b), _f64(x),
+                                                 ctypes.byref(st), _stream()))
+        return self._finish(st)
+
+
+def true_residual(matrix, x, b):
+    """LinearResidueGetter::getResidue: ||A x - b|| / ||b||."""
+    r = ctypes.c_double()
+    _lib.check(_lib.lib().tnlmi_true_residual(ctypes.byref(matrix._m), _f64(x), _f64(b), ctypes.byref(r), _stream()))
+    return r.value
+
+
+def dot(a, b):
+    """(a, b) of FP64 device vectors (host scalar, like the reference's operator)."""
+    import torch
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().tnlmi_dot(a.numel(), _f64(a), _f64(b), ctypes.c_void_p(out.data_ptr()), _stream()))
+    return float(out.item())
+
+
+def l2Norm(a):
+    import torch
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().tnlmi_nrm2(a.numel(), _f64(a), ctypes.c_void_p(out.data_ptr()), _stream()))
+    return float(out.item())
