@@ -259,6 +259,46 @@ tnlmi_status tnlmi_jacobi_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* p
 tnlmi_status tnlmi_true_residual( const tnlmi_matrix* A, const double* d_x, const double* d_b, double* residual,
                                   tnlmi_stream stream );
 
+/* Mixed-precision CG: FP64 iterative refinement (r = b - A x, stop on the
+ * true residual ||r||/||b|| <= state->convergence_residue, x += d) around a
+ * Jacobi-preconditioned inner CG on A_lo -- the same matrix with FP32 or BF16
+ * values -- with FP32 vectors and SpMV accumulation, run to the relative
+ * tolerance inner_tolerance (at most inner_max_iterations per refinement
+ * step).  state->iterations counts refinement steps; *inner_iterations (may be
+ * NULL) the inner CG iterations.  Optional plans of A / A_lo.  Synchronous.
+ * No reference counterpart (TNL has no BF16 type and no mixed-precision
+ * solver); the inner iteration restates CG.hpp:10-108 in FP32. */
+tnlmi_status tnlmi_ir_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const tnlmi_spmv_plan* plan,
+                                const tnlmi_spmv_plan* plan_lo, double inner_tolerance,
+                                int64_t inner_max_iterations, const double* d_b, double* d_x,
+                                tnlmi_solver_state* state, int64_t* inner_iterations, tnlmi_stream stream );
+
+/* ------------------------------------------------------------ batched CG */
+/* Many independent SPD systems, the diagonal blocks of one block-diagonal
+ * matrix A (FP64 values, any segment format; system k owns rows
+ * sys_rows[k] .. sys_rows[k+1], a host array of nsys+1 entries), solved by
+ * one persistent kernel: each workgroup of block_size threads (128/256/512/
+ * 1024; 0 = 512) takes the next system from an atomic counter and runs
+ * Jacobi-preconditioned CG (CG.hpp:10-108 per system, TNL's stopping rule on
+ * the preconditioned residue, default min/divergence settings) to
+ * `tolerance`, at most max_iterations.  A_lo (optional): an FP32 copy of A on
+ * the same segments for the mixed variant -- FP64 refinement on the true
+ * residual ||b-Ax||/||b|| around an FP32 Jacobi-CG run to inner_tolerance.
+ * d_iterations / d_residues (device, nsys entries) receive per-system
+ * iterations (mixed: inner iterations) and final residues.  use_graph: the
+ * launch (counter reset + solve) is captured in a hipGraph once per argument
+ * set and replayed.  Asynchronous.  Calls with one object must not overlap.
+ * No reference counterpart (TNL solves one system per solver call). */
+typedef struct tnlmi_batched_cg tnlmi_batched_cg;
+tnlmi_status tnlmi_batched_cg_create( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const int64_t* sys_rows,
+                                      int64_t nsys, int32_t block_size, tnlmi_stream stream,
+                                      tnlmi_batched_cg** out );
+void tnlmi_batched_cg_destroy( tnlmi_batched_cg* batch );
+tnlmi_status tnlmi_batched_cg_solve( tnlmi_batched_cg* batch, int32_t mixed, const double* d_b, double* d_x,
+                                     double tolerance, int64_t max_iterations, double inner_tolerance,
+                                     int64_t* d_iterations, double* d_residues, int32_t use_graph,
+                                     tnlmi_stream stream );
+
 /* ------------------------------------------------- input generators (host) */
 /* Seeded host-side generators of the workloads the engine is measured on.
  * Random numbers come from a counter-based generator (splitmix64 of seed,
@@ -283,6 +323,14 @@ void tnlmi_gen_random_columns( int64_t n, int64_t ncols, const int32_t* offsets,
 /* out[i] ~ U[lo, hi) or N(0, 1) for i < n, from (seed, stream). */
 void tnlmi_gen_uniform( int64_t n, double lo, double hi, uint64_t seed, uint64_t stream, double* out );
 void tnlmi_gen_normal( int64_t n, uint64_t seed, uint64_t stream, double* out );
+/* Batch of 2D 5-point variable-coefficient diffusion systems as one
+ * block-diagonal CSR matrix: system k on a side s_k grid, s_k =
+ * round(exp(U[ln smin, ln smax])), face coefficients ~ LogNormal(0, 1) on all
+ * cell faces (Dirichlet boundary faces add to the diagonal), A_ii = sum of the
+ * cell's 4 face coefficients, A_ij = -(shared face coefficient).  Pass NULL
+ * offsets to get sides[nsys], *rows and *nnz only. */
+void tnlmi_gen_batched_diffusion( int64_t nsys, int32_t smin, int32_t smax, uint64_t seed, int32_t* sides,
+                                  int64_t* rows, int64_t* nnz, int32_t* offsets, int32_t* columns, double* values );
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,208 @@
+// The reference's own SpMV known-answer tests and solver example, written
+// against TNL's class templates with Devices::MI355X in place of
+// Devices::Host (src/UnitTests/Matrices/SparseMatrixVectorProductTest.hpp,
+// Documentation/Examples/Solvers/Linear/IterativeLinearSolverExample.cpp).
+// Prints "ALL OK" and exits 0 when every check passes.
+#include <TNL/MI355X.h>
+
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+#include <memory>
+
+using namespace TNL;
+using Device = Devices::MI355X;
+using Vector = Containers::Vector< double, Device, int >;
+namespace Seg = Algorithms::Segments;
+
+static int failures = 0;
+#define EXPECT_EQ( a, b )                                                                                            \
+   do {                                                                                                              \
+      if( !( ( a ) == ( b ) ) ) {                                                                                    \
+         std::cerr << __FILE__ << ":" << __LINE__ << ": " #a " == " #b " failed (" << ( a ) << " vs " << ( b ) << ")\n"; \
+         failures++;                                                                                                 \
+      }                                                                                                              \
+   } while( 0 )
+
+template< template< class, class, class > class Segments >
+void vectorProductKATs( const char* name )
+{
+   using Matrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Segments >;
+   std::vector< Seg::LaunchConfiguration > configs{ Seg::LaunchConfiguration() };
+   if( std::is_same_v< Segments< Device, int, void >, Seg::CSR< Device, int, void > > )
+      for( int t : { 1, 2, 4, 8, 16, 32, 64, 128, 256 } )
+         configs.emplace_back( Seg::ThreadsToSegmentsMapping::Fixed, t );
+   for( const auto& cfg : configs ) {
+      // zeroMatrix: capacities [1, 2, 1, 1], no entries
+      {
+         Matrix m( 4, 4 );
+         m.setRowCapacities( std::vector< int >{ 1, 2, 1, 1 } );
+         Vector in( 4, 1 ), out( 4, -1 );
+         m.vectorProduct( in, out, cfg );
+         for( int i = 0; i < 4; i++ )
+            EXPECT_EQ( out.getElement( i ), 0.0 );
+      }
+      // smallMatrix1
+      {
+         Matrix m( 4, 4, { { 0, 0, 1 }, { 1, 1, 2 }, { 1, 3, 3 }, { 2, 1, 4 }, { 3, 2, 5 } } );
+         Vector in( 4, 2 ), out( 4, 0 );
+         m.vectorProduct( in, out, cfg );
+         EXPECT_EQ( out.getElement( 0 ), 2.0 );
+         EXPECT_EQ( out.getElement( 1 ), 10.0 );
+         EXPECT_EQ( out.getElement( 2 ), 8.0 );
+         EXPECT_EQ( out.getElement( 3 ), 10.0 );
+      }
+      // smallMatrix3
+      {
+         Matrix m( 4, 4,
+                   { { 0, 0, 1 }, { 0, 1, 2 }, { 0, 2, 3 }, { 1, 1, 4 }, { 1, 2, 5 }, { 1, 3, 6 },
+                     { 2, 0, 7 }, { 2, 1, 8 }, { 2, 2, 9 }, { 3, 1, 10 }, { 3, 2, 11 }, { 3, 3, 12 } } );
+         Vector in( 4, 2 ), out( 4, 0 );
+         m.vectorProduct( in, out, cfg );
+         EXPECT_EQ( out.getElement( 0 ), 12.0 );
+         EXPECT_EQ( out.getElement( 1 ), 30.0 );
+         EXPECT_EQ( out.getElement( 2 ), 48.0 );
+         EXPECT_EQ( out.getElement( 3 ), 66.0 );
+      }
+      // mediumSizeMatrix1
+      {
+         Matrix m( 8, 8,
+                   { { 0, 0, 1 },  { 0, 1, 2 },  { 0, 2, 3 },  { 0, 5, 4 },  { 1, 1, 5 },  { 1, 2, 6 },  { 1, 3, 7 },
+                     { 1, 4, 8 },  { 2, 0, 9 },  { 2, 1, 10 }, { 2, 2, 11 }, { 2, 3, 12 }, { 2, 4, 13 }, { 3, 1, 14 },
+                     { 3, 2, 15 }, { 3, 3, 16 }, { 3, 4, 17 }, { 4, 2, 18 }, { 4, 3, 19 }, { 4, 4, 20 }, { 4, 5, 21 },
+                     { 5, 3, 22 }, { 5, 4, 23 }, { 5, 5, 24 }, { 5, 6, 25 }, { 6, 0, 26 }, { 6, 1, 27 }, { 6, 2, 28 },
+                     { 6, 3, 29 }, { 6, 4, 30 }, { 7, 0, 31 }, { 7, 1, 32 }, { 7, 2, 33 }, { 7, 3, 34 }, { 7, 4, 35 } } );
+         Vector in( 8, 2 ), out( 8, 0 );
+         m.vectorProduct( in, out, cfg );
+         const double y[ 8 ] = { 20, 52, 110, 124, 156, 188, 280, 330 };
+         for( int i = 0; i < 8; i++ )
+            EXPECT_EQ( out.getElement( i ), y[ i ] );
+      }
+      // largeMatrix: lower triangular, a_ij = j + 1
+      {
+         const int n = 1051;
+         Matrix m( n, n );
+         std::vector< int > off( n + 1, 0 ), cols;
+         std::vector< double > vals;
+         for( int i = 0; i < n; i++ ) {
+            for( int j = 0; j <= i; j++ ) {
+               cols.push_back( j );
+               vals.push_back( j + 1 );
+            }
+            off[ i + 1 ] = (int) cols.size();
+         }
+         m.setFromCSR( off, cols, vals );
+         Vector in( n, 1.0 ), out( n, 0.0 );
+         m.vectorProduct( in, out, cfg );
+         const auto h = out.toHost();
+         for( int i = 0; i < n; i++ )
+            EXPECT_EQ( h[ i ], double( ( i + 1 ) * ( i + 2 ) / 2 ) );
+      }
+   }
+   // alpha / beta / row range (SparseMatrixBase.hpp:346-389)
+   {
+      Matrix m( 4, 4, { { 0, 0, 1 }, { 1, 1, 2 }, { 1, 3, 3 }, { 2, 1, 4 }, { 3, 2, 5 } } );
+      Vector in( 4, 2 ), out( 4, 1 );
+      m.vectorProduct( in, out, 2.0, 3.0, 1, 3 );
+      EXPECT_EQ( out.getElement( 0 ), 1.0 );
+      EXPECT_EQ( out.getElement( 1 ), 2.0 * 10 + 3.0 );
+      EXPECT_EQ( out.getElement( 2 ), 2.0 * 8 + 3.0 );
+      EXPECT_EQ( out.getElement( 3 ), 1.0 );
+   }
+   std::cout << name << " KATs done\n";
+}
+
+template< typename Solver, typename Matrix >
+void solveExample( const std::shared_ptr< Matrix >& A, const char* name, bool diag = false )
+{
+   Vector x( 5, 1.0 ), b( 5 );
+   A->vectorProduct( x, b );
+   x = 0.0;
+   Solver s;
+   s.setMatrix( A );
+   if( diag ) {
+      auto p = std::make_shared< Solvers::Linear::Preconditioners::Diagonal< Matrix > >();
+      p->update( A );
+      s.setPreconditioner( p );
+   }
+   s.setConvergenceResidue( 1e-12 );
+   const bool ok = s.solve( b, x );
+   EXPECT_EQ( ok, true );
+   const auto h = x.toHost();
+   for( double v : h )
+      EXPECT_EQ( std::fabs( v - 1.0 ) < 1e-10, true );
+   EXPECT_EQ( Solvers::Linear::LinearResidueGetter::getResidue( *A, x, b ) < 1e-12, true );
+   std::cout << name << ": " << s.getIterations() << " iterations, residue " << s.getResidue() << '\n';
+}
+
+int main()
+{
+   vectorProductKATs< Seg::CSR >( "CSR" );
+   vectorProductKATs< Seg::AdaptiveCSR >( "AdaptiveCSR" );
+   vectorProductKATs< Seg::Ellpack >( "Ellpack" );
+   vectorProductKATs< Seg::SlicedEllpack >( "SlicedEllpack" );
+   vectorProductKATs< Seg::SlicedEllpack64 >( "SlicedEllpack<64>" );
+   vectorProductKATs< Seg::BiEllpack >( "BiEllpack" );
+   vectorProductKATs< Seg::ChunkedEllpack >( "ChunkedEllpack" );
+   vectorProductKATs< Seg::SortedCSR >( "SortedCSR" );
+
+   // IterativeLinearSolverExample.cpp: tridiag(-1, 2.5, -1), b = A * 1
+   using Matrix = Matrices::SparseMatrix< double, Device, int >;
+   auto A = std::make_shared< Matrix >( 5, 5,
+                                        std::initializer_list< std::tuple< int, int, double > >{
+                                           { 0, 0, 2.5 },  { 0, 1, -1 },  { 1, 0, -1 },  { 1, 1, 2.5 }, { 1, 2, -1 },
+                                           { 2, 1, -1 },   { 2, 2, 2.5 }, { 2, 3, -1 },  { 3, 2, -1 },  { 3, 3, 2.5 },
+                                           { 3, 4, -1 },   { 4, 3, -1 },  { 4, 4, 2.5 } } );
+   solveExample< Solvers::Linear::CG< Matrix > >( A, "CG" );
+   solveExample< Solvers::Linear::CG< Matrix > >( A, "CG + Diagonal", true );
+   {
+      Solvers::Linear::GMRES< Matrix > g;
+      Config::ParameterContainer p;
+      p.addParameter< std::string >( "gmres-variant", "MGSR" );
+      EXPECT_EQ( g.setup( p ), true );
+      Vector x( 5, 1.0 ), b( 5 );
+      A->vectorProduct( x, b );
+      x = 0.0;
+      g.setMatrix( A );
+      g.setConvergenceResidue( 1e-12 );
+      EXPECT_EQ( g.solve( b, x ), true );
+      std::cout << "GMRES(MGSR): " << g.getIterations() << " iterations\n";
+   }
+   solveExample< Solvers::Linear::Jacobi< Matrix > >( A, "Jacobi" );
+
+   // error behaviour of the reference
+   {
+      Vector in( 3, 1.0 ), out( 5 );
+      bool thrown = false;
+      try {
+         A->vectorProduct( in, out );
+      }
+      catch( const std::invalid_argument& ) {
+         thrown = true;
+      }
+      EXPECT_EQ( thrown, true );
+      thrown = false;
+      try {
+         Seg::LaunchConfiguration cfg( Seg::ThreadsToSegmentsMapping::Fixed, 3 );
+      }
+      catch( const std::invalid_argument& ) {
+         thrown = true;
+      }
+      EXPECT_EQ( thrown, true );
+      thrown = false;
+      try {
+         Solvers::Linear::CG< Matrix > cg;
+         cg.solve( out, out );
+      }
+      catch( const std::logic_error& ) {
+         thrown = true;
+      }
+      EXPECT_EQ( thrown, true );
+   }
+   if( failures ) {
+      std::cout << failures << " FAILURES\n";
+      return 1;
+   }
+   std::cout << "ALL OK\n";
+   return 0;
+}

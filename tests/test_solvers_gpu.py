@@ -188,3 +188,44 @@ def test_solver_example_5x5():
         x = torch.zeros(5, dtype=torch.float64, device="cuda")
         assert solver.solve(b, x)
         np.testing.assert_allclose(x.cpu().numpy(), np.ones(5), rtol=1e-10)
+
+
+@pytest.mark.parametrize("vt", ["f32", "bf16"])
+def test_mixed_precision_cg(orc, vt):
+    # 3D Poisson 24^3: values 6 and -1 are exact in BF16, so both inner
+    # matrices equal A; the FP64 refinement must reach the same true residual
+    # as plain FP64 CG
+    off, cols, vals = tnl.poisson3d(24, 24, 24)
+    n = len(off) - 1
+    b = np.random.default_rng(11).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    Alo = tnl.SparseMatrix.from_csr(n, n, off, cols, vals,
+                                    value_dtype=torch.float32 if vt == "f32" else torch.bfloat16)
+    s = S.MixedCG()
+    s.setMatrix(A)
+    s.setInnerMatrix(Alo)
+    s.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    r = orc.true_residual(off, cols, vals, x.cpu().numpy(), b)
+    assert r <= 1e-10 and abs(r - s.getResidue()) < 1e-12
+    xo, _ = orc.cg(off, cols, vals, b, precond=1, tol=1e-12)
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-8, atol=1e-9)
+    assert 1 <= s.getIterations() <= 8 and s.getInnerIterations() > 0
+
+
+def test_mixed_precision_bf16_rounded_values(orc):
+    # values that BF16 rounds: the refinement still converges to the FP64 solution
+    off, cols, vals = tnl.laplace2d(30, 30)
+    vals = vals * 1.0137  # 4.0548 / -1.0137 are not BF16 numbers
+    n = len(off) - 1
+    b = np.random.default_rng(12).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    Ab = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, value_dtype=torch.bfloat16)
+    s = S.MixedCG()
+    s.setMatrix(A)
+    s.setInnerMatrix(Ab)
+    s.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-10

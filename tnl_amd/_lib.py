@@ -144,6 +144,16 @@ _PROTOS = {
     "tnlmi_jacobi_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, ctypes.c_double, c_i64, c_vp, c_vp,
                                           ctypes.POINTER(SolverState), c_vp]),
     "tnlmi_true_residual": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.POINTER(ctypes.c_double), c_vp]),
+    "tnlmi_ir_cg_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_double,
+                                         c_i64, c_vp, c_vp, ctypes.POINTER(SolverState), ctypes.POINTER(c_i64),
+                                         c_vp]),
+    "tnlmi_batched_cg_create": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_i64,
+                                               ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)]),
+    "tnlmi_batched_cg_destroy": (None, [c_vp]),
+    "tnlmi_batched_cg_solve": (ctypes.c_int, [c_vp, ctypes.c_int32, c_vp, c_vp, ctypes.c_double, c_i64,
+                                              ctypes.c_double, c_vp, c_vp, ctypes.c_int32, c_vp]),
+    "tnlmi_gen_batched_diffusion": (None, [c_i64, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_vp, c_vp, c_vp,
+                                           c_vp, c_vp, c_vp]),
     "tnlmi_gen_laplace2d": (c_i64, [c_i64, c_i64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_poisson3d": (c_i64, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_zipf_sizes": (None, [c_i64, ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,

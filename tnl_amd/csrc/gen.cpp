@@ -203,6 +203,71 @@ void tnlmi_gen_random_columns( int64_t n, int64_t ncols, const int32_t* off, uin
    }
 }
 
+// Batch of 2D 5-point variable-coefficient diffusion systems (block-diagonal
+// CSR).  System k: side s_k = round(exp(U[ln smin, ln smax])), cells
+// (ix, iy), face coefficients c ~ LogNormal(0, 1) on every cell face
+// (boundary faces included: Dirichlet), row = sum of the cell's 4 face
+// coefficients on the diagonal, -c of each interior face off the diagonal.
+// Pass NULL arrays to get the sizes only: sides[nsys] and *rows, *nnz.
+void tnlmi_gen_batched_diffusion( int64_t nsys, int32_t smin, int32_t smax, uint64_t seed, int32_t* sides,
+                                  int64_t* rows, int64_t* nnz, int32_t* offsets, int32_t* columns, double* values )
+{
+   std::vector< int64_t > row0( nsys + 1, 0 ), nz0( nsys + 1, 0 );
+   for( int64_t k = 0; k < nsys; k++ ) {
+      const double u = unit( rnd( seed, 101, k ) );
+      const double ls = std::log( (double) smin ) + u * ( std::log( (double) smax ) - std::log( (double) smin ) );
+      const int32_t s = (int32_t) std::min< double >( smax, std::max< double >( smin, std::floor( std::exp( ls ) + 0.5 ) ) );
+      sides[ k ] = s;
+      row0[ k + 1 ] = row0[ k ] + (int64_t) s * s;
+      nz0[ k + 1 ] = nz0[ k ] + 5 * (int64_t) s * s - 4 * (int64_t) s;
+   }
+   *rows = row0[ nsys ];
+   *nnz = nz0[ nsys ];
+   if( ! offsets )
+      return;
+#pragma omp parallel for schedule( dynamic, 4 )
+   for( int64_t k = 0; k < nsys; k++ ) {
+      const int64_t s = sides[ k ];
+      // horizontal faces h[iy][ix] between (ix-1, iy) and (ix, iy), ix = 0..s; vertical v[iy][ix], iy = 0..s
+      auto hface = [ & ]( int64_t ix, int64_t iy ) {
+         const double z = std::sqrt( -2.0 * std::log( 1.0 - unit( rnd( seed, 1000 + 2 * (uint64_t) k, 2 * ( iy * ( s + 1 ) + ix ) ) ) ) )
+                          * std::cos( 6.283185307179586 * unit( rnd( seed, 1000 + 2 * (uint64_t) k, 2 * ( iy * ( s + 1 ) + ix ) + 1 ) ) );
+         return std::exp( z );
+      };
+      auto vface = [ & ]( int64_t ix, int64_t iy ) {
+         const double z = std::sqrt( -2.0 * std::log( 1.0 - unit( rnd( seed, 1001 + 2 * (uint64_t) k, 2 * ( iy * s + ix ) ) ) ) )
+                          * std::cos( 6.283185307179586 * unit( rnd( seed, 1001 + 2 * (uint64_t) k, 2 * ( iy * s + ix ) + 1 ) ) );
+         return std::exp( z );
+      };
+      int64_t q = nz0[ k ];
+      for( int64_t iy = 0; iy < s; iy++ )
+         for( int64_t ix = 0; ix < s; ix++ ) {
+            const int64_t i = row0[ k ] + iy * s + ix;
+            offsets[ i ] = (int32_t) q;
+            const double w = hface( ix, iy ), e = hface( ix + 1, iy ), so = vface( ix, iy ), no = vface( ix, iy + 1 );
+            if( iy > 0 ) {
+               columns[ q ] = (int32_t) ( i - s );
+               values[ q++ ] = -so;
+            }
+            if( ix > 0 ) {
+               columns[ q ] = (int32_t) ( i - 1 );
+               values[ q++ ] = -w;
+            }
+            columns[ q ] = (int32_t) i;
+            values[ q++ ] = w + e + so + no;
+            if( ix < s - 1 ) {
+               columns[ q ] = (int32_t) ( i + 1 );
+               values[ q++ ] = -e;
+            }
+            if( iy < s - 1 ) {
+               columns[ q ] = (int32_t) ( i + s );
+               values[ q++ ] = -no;
+            }
+         }
+   }
+   offsets[ row0[ nsys ] ] = (int32_t) nz0[ nsys ];
+}
+
 void tnlmi_gen_uniform( int64_t n, double lo, double hi, uint64_t seed, uint64_t stream, double* out )
 {
 #pragma omp parallel for schedule( static )

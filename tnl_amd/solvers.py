@@ -174,6 +174,44 @@ class Jacobi(IterativeSolver):
         return self._finish(st)
 
 
+class MixedCG(IterativeSolver):
+    """FP64 iterative refinement around an FP32-arithmetic Jacobi-CG on a copy
+    of the matrix with FP32 or BF16 values (tnlmi_ir_cg_solve).  The stopping
+    rule is the FP64 true residual; getIterations() counts refinement steps and
+    getInnerIterations() the inner CG iterations."""
+
+    def __init__(self):
+        super().__init__()
+        self.matrix_lo = None
+        self.plan_lo = None
+        self.inner_tolerance = 1e-4
+        self.inner_max_iterations = 10_000
+        self.inner_iterations = 0
+
+    def setInnerMatrix(self, matrix_lo, plan=None):
+        self.matrix_lo = matrix_lo
+        self.plan_lo = plan
+
+    def setInnerTolerance(self, tol):
+        self.inner_tolerance = float(tol)
+
+    def getInnerIterations(self):
+        return self.inner_iterations
+
+    def solve(self, b, x):
+        m, plan, _ = self._handles()
+        if self.matrix_lo is None:
+            raise _lib.LogicError(_lib.LOGIC_ERROR, "No inner matrix was set. Call setInnerMatrix() first.")
+        st = self._state()
+        inner = ctypes.c_int64()
+        plan_lo = self.plan_lo.handle if self.plan_lo is not None else None
+        _lib.check(_lib.lib().tnlmi_ir_cg_solve(m, ctypes.byref(self.matrix_lo._m), plan, plan_lo,
+                                                self.inner_tolerance, self.inner_max_iterations, _f64(b), _f64(x),
+                                                ctypes.byref(st), ctypes.byref(inner), _stream()))
+        self.inner_iterations = inner.value
+        return self._finish(st)
+
+
 def true_residual(matrix, x, b):
     """LinearResidueGetter::getResidue: ||A x - b|| / ||b||."""
     r = ctypes.c_double()
