@@ -100,15 +100,31 @@ __global__ void batch_inv_diag( SegView v, const int32_t* __restrict__ cols, con
    dinv[ r ] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
-template< int B >
-__device__ __forceinline__ int64_t next_system( int* counter, int64_t* slot )
+// The next system for the whole workgroup.  Only whole wavefronts branch
+// here (wave 0 fetches, the others wait at the barrier): a lane-divergent
+// "thread 0 fetches" block next to the loop's back edge lets the compiler
+// split the loop so that the other lanes of thread 0's wavefront loop back
+// without it and re-read a stale slot forever.  Lane 0 adds one, the other lanes add zero
+// (the wave's adds become one atomic); the value is lane 0's.
+__device__ __forceinline__ int64_t next_system( int* counter, int* slot )
 {
-   if( threadIdx.x == 0 )
-      *slot = atomicAdd( counter, 1 );
+   if( threadIdx.x < 64 ) {
+      const int got = atomicAdd( counter, threadIdx.x == 0 ? 1 : 0 );
+      *slot = __builtin_amdgcn_readfirstlane( got );
+   }
    __syncthreads();
-   const int64_t s = *slot;
+   const int s = __builtin_amdgcn_readfirstlane( *slot );
    __syncthreads();
    return s;
+}
+
+// Per-system results, written by wavefront 0 (every lane the same value).
+__device__ __forceinline__ void put_result( int64_t* iters, double* res, int64_t sys, int64_t it, double r )
+{
+   if( threadIdx.x < 64 ) {
+      iters[ sys ] = it;
+      res[ sys ] = r;
+   }
 }
 
 // FP64 Jacobi-CG per system, TNL's stopping rule with default min/divergence
@@ -116,11 +132,8 @@ template< int B >
 __global__ void __launch_bounds__( B ) batched_cg64( BatchArgs a )
 {
    __shared__ double red[ B / 64 ];
-   __shared__ int64_t slot;
-   while( true ) {
-      const int64_t sys = next_system< B >( a.counter, &slot );
-      if( sys >= a.nsys )
-         break;
+   __shared__ int slot;
+   for( int64_t sys = next_system( a.counter, &slot ); sys < a.nsys; sys = next_system( a.counter, &slot ) ) {
       const int64_t r0 = a.sys_rows[ sys ], r1 = a.sys_rows[ sys + 1 ];
       // r = b - A x, p = z = D^{-1} r, s1 = (r, z), normb = ||D^{-1} b||
       double acc1 = 0, acc2 = 0;
@@ -173,10 +186,7 @@ __global__ void __launch_bounds__( B ) batched_cg64( BatchArgs a )
             res = sqrt( s1 ) / normb;
          }
       }
-      if( threadIdx.x == 0 ) {
-         a.iters[ sys ] = it;
-         a.res[ sys ] = res;
-      }
+      put_result( a.iters, a.res, sys, it, res );
    }
 }
 
@@ -186,11 +196,8 @@ template< int B >
 __global__ void __launch_bounds__( B ) batched_cg_mixed( BatchArgs a )
 {
    __shared__ double red[ B / 64 ];
-   __shared__ int64_t slot;
-   while( true ) {
-      const int64_t sys = next_system< B >( a.counter, &slot );
-      if( sys >= a.nsys )
-         break;
+   __shared__ int slot;
+   for( int64_t sys = next_system( a.counter, &slot ); sys < a.nsys; sys = next_system( a.counter, &slot ) ) {
       const int64_t r0 = a.sys_rows[ sys ], r1 = a.sys_rows[ sys + 1 ];
       double acc = 0;
       for( int64_t i = r0 + threadIdx.x; i < r1; i += B )
@@ -254,10 +261,7 @@ __global__ void __launch_bounds__( B ) batched_cg_mixed( BatchArgs a )
             a.x[ i ] += rn * (double) a.d32[ i ];
          __syncthreads();
       }
-      if( threadIdx.x == 0 ) {
-         a.iters[ sys ] = inner_total;
-         a.res[ sys ] = res;
-      }
+      put_result( a.iters, a.res, sys, inner_total, res );
    }
 }
 

@@ -198,8 +198,33 @@ __device__ __forceinline__ T group_sum_dyn( T v, int g )
    return v;
 }
 
-// Block-wide sum for blockDim.x == B (multiple of 64); result valid in all
-// threads.  smem must hold B/64 values.
+// The value of the wavefront's first active lane, as a value the compiler
+// knows to be uniform: loops and branches on it stay scalar (no per-lane exit
+// masks around barriers).  For values that are equal in every lane.
+__device__ __forceinline__ double wave_uniform( double v )
+{
+   const uint64_t b = (uint64_t) __double_as_longlong( v );
+   const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane( (int) (uint32_t) b );
+   const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane( (int) (uint32_t) ( b >> 32 ) );
+   return __longlong_as_double( (long long) ( ( (uint64_t) hi << 32 ) | lo ) );
+}
+__device__ __forceinline__ float wave_uniform( float v )
+{
+   return __int_as_float( __builtin_amdgcn_readfirstlane( __float_as_int( v ) ) );
+}
+__device__ __forceinline__ int32_t wave_uniform( int32_t v )
+{
+   return __builtin_amdgcn_readfirstlane( v );
+}
+__device__ __forceinline__ int64_t wave_uniform( int64_t v )
+{
+   const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane( (int) (uint32_t) v );
+   const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane( (int) (uint32_t) ( (uint64_t) v >> 32 ) );
+   return (int64_t) ( ( (uint64_t) hi << 32 ) | lo );
+}
+
+// Block-wide sum for blockDim.x == B (multiple of 64); result valid (and
+// uniform) in all threads; all threads must call it.  smem must hold B/64 values.
 template< int B, typename T >
 __device__ __forceinline__ T block_sum( T v, T* smem )
 {
@@ -213,7 +238,7 @@ __device__ __forceinline__ T block_sum( T v, T* smem )
 #pragma unroll
    for( int i = 0; i < B / 64; i++ )
       r += smem[ i ];
-   return r;
+   return wave_uniform( r );
 }
 
 // ------------------------------------------------------- XCD-aware mapping
