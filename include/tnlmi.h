@@ -273,6 +273,23 @@ tnlmi_status tnlmi_ir_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_lo,
                                 int64_t inner_max_iterations, const double* d_b, double* d_x,
                                 tnlmi_solver_state* state, int64_t* inner_iterations, tnlmi_stream stream );
 
+/* Mixed-precision CG with reliable updates: Jacobi-preconditioned CG run in
+ * FP32 (FP32 vectors and SpMV accumulation, FP64 dot products) on A_lo -- the
+ * same matrix with FP32 or BF16 values -- with x accumulated in FP64.  When
+ * the FP32 residual has fallen by `delta` (0 < delta < 1, e.g. 0.1) since the
+ * last update, or its estimate reaches the tolerance, x += correction and
+ * r = b - A x are recomputed in FP64 (the FP32 residual is replaced, the
+ * search direction kept).  Stops when the FP64 true residual ||b-Ax||/||b||
+ * is below state->convergence_residue (min/max iterations and divergence as
+ * in IterativeSolver.hpp:98-121, checked at the updates); state->iterations
+ * counts the FP32 iterations, *updates (may be NULL) the FP64 residual
+ * evaluations.  Optional plan of A.  Synchronous.  No reference counterpart
+ * (TNL has no BF16 type and no mixed-precision solver); the iteration
+ * restates CG.hpp:10-108 in FP32. */
+tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const tnlmi_spmv_plan* plan,
+                                   double delta, const double* d_b, double* d_x, tnlmi_solver_state* state,
+                                   int64_t* updates, tnlmi_stream stream );
+
 /* ------------------------------------------------------------ batched CG */
 /* Many independent SPD systems, the diagonal blocks of one block-diagonal
  * matrix A (FP64 values, any segment format; system k owns rows

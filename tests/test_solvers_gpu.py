@@ -229,3 +229,90 @@ def test_mixed_precision_bf16_rounded_values(orc):
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
     assert s.solve(dev(b), x)
     assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-10
+
+
+def test_cg_iteration_limit_matches_tnl(orc):
+    # the device-side stopping rule counts iterations as IterativeSolver::nextIteration does
+    off, cols, vals = tnl.laplace2d(32, 32)
+    n = len(off) - 1
+    b = np.random.default_rng(5).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    for maxit, minit in ((5, 0), (13, 0), (40, 30)):
+        xo, so = orc.cg(off, cols, vals, b, precond=1, tol=1e-3, maxit=maxit, minit=minit)
+        cg = S.CG()
+        cg.setMatrix(A)
+        cg.setPreconditioner(S.Diagonal(A))
+        cg.setConvergenceResidue(1e-3)
+        cg.setMaxIterations(maxit)
+        cg.setMinIterations(minit)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ok = cg.solve(dev(b), x)
+        assert ok == bool(so.converged) and cg.getIterations() == so.iterations, (maxit, minit)
+        assert abs(cg.getResidue() - so.residue) <= 1e-9 * max(1.0, so.residue)
+        np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("vt", ["f32", "bf16"])
+@pytest.mark.parametrize("fmt,param", [(tnl.CSR, 0), (tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32),
+                                       (tnl.BIELLPACK, 32)])
+def test_reliable_mixed_cg(orc, vt, fmt, param):
+    # FP32 CG with reliable updates: the FP64 true residual reaches 1e-10 in
+    # about as many iterations as FP64 CG (the Krylov process is not restarted)
+    off, cols, vals = tnl.poisson3d(20, 18, 16)
+    n = len(off) - 1
+    b = np.random.default_rng(21).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+    Alo = tnl.SparseMatrix(A.rows, A.columns, A.segments,
+                           A.values.to(torch.float32 if vt == "f32" else torch.bfloat16), A.column_indexes)
+    s = S.ReliableMixedCG()
+    s.setMatrix(A)
+    s.setInnerMatrix(Alo)
+    s.setConvergenceResidue(1e-10)
+    s.setMaxIterations(3000)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    r = orc.true_residual(off, cols, vals, x.cpu().numpy(), b)
+    assert r <= 1e-10 and abs(r - s.getResidue()) < 1e-13
+    xo, so = orc.cg(off, cols, vals, b, precond=1, tol=1e-12)
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-8, atol=1e-9)
+    assert s.getIterations() <= 1.25 * so.iterations + 10, (s.getIterations(), so.iterations)
+    assert 2 <= s.getUpdates() <= 40
+
+
+def test_reliable_mixed_cg_rounded_bf16(orc):
+    off, cols, vals = tnl.laplace2d(40, 40)
+    vals = vals * 1.0137  # not BF16 numbers: the inner matrix differs from A
+    n = len(off) - 1
+    b = np.random.default_rng(22).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    Ab = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, value_dtype=torch.bfloat16)
+    s = S.ReliableMixedCG()
+    s.setMatrix(A)
+    s.setInnerMatrix(Ab)
+    s.setConvergenceResidue(1e-10)
+    s.setMaxIterations(5000)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-10
+
+
+def test_reliable_mixed_cg_errors():
+    off, cols, vals = tnl.laplace2d(8, 8)
+    n = len(off) - 1
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    s = S.ReliableMixedCG()
+    s.setMatrix(A)
+    with pytest.raises(tnl._lib.LogicError):
+        s.solve(dev(np.ones(n)), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    s.setInnerMatrix(A)  # FP64 values: not an inner matrix
+    with pytest.raises(ValueError):
+        s.solve(dev(np.ones(n)), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    s.setInnerMatrix(tnl.SparseMatrix.from_csr(n, n, off, cols, vals, value_dtype=torch.float32))
+    s.setDelta(1.5)
+    with pytest.raises(ValueError):
+        s.solve(dev(np.ones(n)), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    # zero right-hand side: x = 0, converged without iterations
+    s.setDelta(0.1)
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    assert s.solve(torch.zeros(n, dtype=torch.float64, device="cuda"), x)
+    assert s.getIterations() == 0 and float(x.abs().max()) == 0.0

@@ -147,6 +147,8 @@ _PROTOS = {
     "tnlmi_ir_cg_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_double,
                                          c_i64, c_vp, c_vp, ctypes.POINTER(SolverState), ctypes.POINTER(c_i64),
                                          c_vp]),
+    "tnlmi_mixed_cg_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, ctypes.c_double,
+                                            c_vp, c_vp, ctypes.POINTER(SolverState), ctypes.POINTER(c_i64), c_vp]),
     "tnlmi_batched_cg_create": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_i64,
                                                ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)]),
     "tnlmi_batched_cg_destroy": (None, [c_vp]),

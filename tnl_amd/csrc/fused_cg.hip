@@ -1,0 +1,795 @@
+// fused_cg.hip -- Jacobi-preconditioned CG with the dot products fused into
+// the kernels that produce their operands, and the reliable-update
+// mixed-precision CG built from the same kernels.
+//
+// One CG iteration (CG.hpp:60-105) is three launches:
+//   spmv_dot   Ap = A p  and the partial sums of (Ap, p)            (SpMV epilogue)
+//   cg_upd     alpha = s1/s2;  x += alpha p;  r -= alpha Ap;  z = r / d (not stored);
+//              partial sums of (r, z) [and (r, r)]
+//   cg_dir     beta, the residue, TNL's stopping rule (IterativeSolver.hpp:98-121)
+//              and the iteration counter;  p = r / d + beta p
+// Every block of a kernel reduces the previous kernels' 1024 partial sums
+// in the same fixed order, so the scalars are identical in every block and
+// from run to run.  The iteration state lives on the device, double-buffered
+// by iteration parity (iteration k reads state k & 1, block 0 of its last
+// kernel writes state (k + 1) & 1); every kernel returns at once when the
+// state says the solver has stopped, so the host enqueues iterations in
+// batches and reads the state back once per batch.
+//
+// Mixed precision (tnlmi_mixed_cg_solve): the same iteration in FP32 on the
+// FP32- or BF16-valued matrix, FP32 vectors and accumulation, FP64 dot
+// products; x is accumulated in FP64.  Reliable updates (residual
+// replacement, Clark et al., "Solving lattice QCD systems of equations using
+// mixed precision solvers on GPUs", 2010): when the FP32 residual has dropped
+// by `delta` since the last update (or its estimate reaches the tolerance),
+// x += correction and r = b - A x are recomputed in FP64 and the search
+// direction p is kept, so the Krylov process is not restarted.  The stopping
+// test is the FP64 true residual ||b - A x|| / ||b||
+// (LinearResidueGetter.hpp:11-23).
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <type_traits>
+
+#include "blas1.hpp"
+#include "krylov.hpp"
+#include "segments.hpp"
+#include "spmv.hpp"
+
+namespace tnlmi {
+namespace {
+
+constexpr int kFB = 1024;  // blocks of every fused vector kernel = number of partial sums
+constexpr int kFT = 256;
+constexpr int kBatch = 8;  // iterations enqueued between two host reads of the state
+static_assert( kFB % kXcds == 0, "the SpMV kernels split the grid evenly over the XCDs" );
+
+struct CgDev
+{
+   double s1, normb, res, beta;
+   double rr, rr_max;  // mixed: (r, r) now and its maximum since the last update
+   double s1_prev;     // mixed: (r, z) of the previous iteration (the beta denominator)
+   double proj;        // mixed: (r, p) / (r, z) after a residual replacement
+   int64_t it;         // TNL's currentIteration
+   int32_t active;     // the iteration being run was admitted by nextIteration()
+   int32_t s2zero;     // s2 == 0 ended the solve (CG.hpp:66-70)
+   int32_t want_update;
+   int32_t pad;
+};
+
+struct StopRule
+{
+   int64_t maxit, minit;
+   double tol, div;
+};
+
+__device__ __forceinline__ bool check_next( const StopRule& R, double res, int64_t it )
+{
+   return ! ( isnan( res ) || it > R.maxit || ( res > R.div && it >= R.minit ) || ( res < R.tol && it >= R.minit ) );
+}
+
+// Sum of kFB partial sums in a fixed order; the same value in every block.
+__device__ __forceinline__ double sum_parts( const double* __restrict__ part, double* sm )
+{
+   double a = 0;
+   for( int i = threadIdx.x; i < kFB; i += kFT )
+      a += part[ i ];
+   return block_sum< kFT >( a, sm );
+}
+
+__device__ __forceinline__ void put_part( double v, double* sm, double* part )
+{
+   v = block_sum< kFT >( v, sm );
+   if( threadIdx.x < 64 )
+      part[ blockIdx.x ] = v;
+}
+
+// ---- fused SpMV + (Ap, p) partial sums
+// Loops stride over the grid kU rows / elements at a time, so that every
+// thread has kU independent loads in flight.
+constexpr int kU = 4;
+constexpr int64_t kStride = (int64_t) kFB * kFT;
+
+// Ellpack / SlicedEllpack, column-major: one thread per row, coalesced
+// slots, a full grid of row blocks placed XCD-contiguously (the rows that
+// share x entries -- a stencil's neighbouring rows and planes -- are read by
+// one XCD); one partial sum per block.
+template< typename V, typename T >
+__global__ void __launch_bounds__( kFT ) spmv_dot_ell( SegView v, const int32_t* __restrict__ cols,
+                                                       const V* __restrict__ vals, const T* __restrict__ p,
+                                                       T* __restrict__ Ap, double* __restrict__ part,
+                                                       const CgDev* __restrict__ st )
+{
+   __shared__ double sm[ kFT / 64 ];
+   if( ! st->active )
+      return;
+   const int64_t blk = xcd_contiguous( blockIdx.x, gridDim.x );
+   const int64_t r = blk * kFT + threadIdx.x;
+   double acc = 0;
+   if( r < v.n ) {
+      int64_t g, stride;
+      int32_t w;
+      if( v.format == TNLMI_ELLPACK ) {
+         g = r;
+         stride = v.ell_aligned;
+         w = v.ell_width;
+      }
+      else {
+         const int64_t sl = r / v.slice_size;
+         g = v.slice_offsets[ sl ] + ( r - sl * v.slice_size );
+         stride = v.slice_size;
+         w = v.slice_seg_sizes[ sl ];
+      }
+      T y = 0;
+      for( int32_t k = 0; k < w; k++, g += stride ) {
+         const int32_t c = __builtin_nontemporal_load( cols + g );
+         if( c >= 0 )
+            y += nt_load_val< V, T >( vals, g ) * p[ c ];
+      }
+      Ap[ r ] = y;
+      acc = (double) y * (double) p[ r ];
+   }
+   const double bs = block_sum< kFT >( acc, sm );
+   if( threadIdx.x < 64 )
+      part[ blk ] = bs;
+}
+
+// CSR with short rows: G lanes per row, a full grid of row blocks placed
+// XCD-contiguously, one partial sum per block
+template< int G, typename V, typename T >
+__global__ void __launch_bounds__( kFT ) spmv_dot_csr( int64_t n, const int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ cols, const V* __restrict__ vals,
+                                                       const T* __restrict__ p, T* __restrict__ Ap,
+                                                       double* __restrict__ part, const CgDev* __restrict__ st )
+{
+   __shared__ double sm[ kFT / 64 ];
+   if( ! st->active )
+      return;
+   const int lane = threadIdx.x & ( G - 1 );
+   const int64_t blk = xcd_contiguous( blockIdx.x, gridDim.x );
+   const int64_t r = blk * ( kFT / G ) + threadIdx.x / G;
+   double acc = 0;
+   if( r < n ) {
+      const int32_t e1 = off[ r + 1 ];
+      T y = 0;
+      for( int32_t e = off[ r ] + lane; e < e1; e += G ) {
+         const int32_t c = __builtin_nontemporal_load( cols + e );
+         if( c >= 0 )
+            y += nt_load_val< V, T >( vals, e ) * p[ c ];
+      }
+      y = group_sum< G >( y );
+      if( lane == 0 ) {
+         Ap[ r ] = y;
+         acc = (double) y * (double) p[ r ];
+      }
+   }
+   const double bs = block_sum< kFT >( acc, sm );
+   if( threadIdx.x < 64 )
+      part[ blk ] = bs;
+}
+
+// The SpMV kernels' nin block partial sums folded, in order, into the kFB
+// partial sums the following kernels read.
+__global__ void __launch_bounds__( kFT ) fold_parts( const double* __restrict__ in, int64_t nin,
+                                                     double* __restrict__ out, const CgDev* __restrict__ st )
+{
+   __shared__ double sm[ kFT / 64 ];
+   if( ! st->active )
+      return;
+   const int64_t chunk = ( nin + kFB - 1 ) / kFB;
+   const int64_t b = blockIdx.x * chunk, e = min( nin, b + chunk );
+   double a = 0;
+   for( int64_t i = b + threadIdx.x; i < e; i += kFT )
+      a += in[ i ];
+   put_part( a, sm, out );
+}
+
+// any other format: the format's own SpMV kernel, then this partial dot
+template< typename T >
+__global__ void __launch_bounds__( kFT ) dot_part( int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                   double* __restrict__ part, const CgDev* __restrict__ st )
+{
+   __shared__ double sm[ kFT / 64 ];
+   if( ! st->active )
+      return;
+   double acc = 0;
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+#pragma unroll
+      for( int u = 0; u < kU; u++ ) {
+         const int64_t i = i0 + u * kStride;
+         if( i < n )
+            acc += (double) a[ i ] * (double) b[ i ];
+      }
+   put_part( acc, sm, part );
+}
+
+// ---- alpha = s1 / (Ap, p);  x += alpha p;  r -= alpha Ap;  partial sums of
+// (r, z) and, for the mixed solver, (r, r);  z = r / d
+template< typename T, typename X, typename D >
+__global__ void __launch_bounds__( kFT ) cg_upd( int64_t n, const double* __restrict__ part1, T* __restrict__ r,
+                                                 const T* __restrict__ p, const T* __restrict__ Ap,
+                                                 X* __restrict__ x, const D* __restrict__ d,
+                                                 double* __restrict__ part2, double* __restrict__ part3,
+                                                 const CgDev* __restrict__ st )
+{
+   __shared__ double sm[ kFT / 64 ];
+   if( ! st->active )
+      return;
+   const double s2 = sum_parts( part1, sm );
+   if( s2 == 0.0 )
+      return;  // cg_dir ends the solve
+   const T alpha = (T) ( st->s1 / s2 );
+   double a2 = 0, a3 = 0;
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+#pragma unroll
+      for( int u = 0; u < kU; u++ ) {
+         const int64_t i = i0 + u * kStride;
+         if( i < n ) {
+            const T pi = p[ i ];
+            x[ i ] += (X) ( alpha * pi );
+            const T ri = r[ i ] - alpha * Ap[ i ];
+            r[ i ] = ri;
+            const T zi = d ? ri / (T) d[ i ] : ri;
+            a2 += (double) ri * (double) zi;
+            a3 += (double) ri * (double) ri;
+         }
+      }
+   put_part( a2, sm, part2 );
+   if( part3 )
+      put_part( a3, sm, part3 );
+}
+
+// ---- the end of one iteration: every block computes the scalars from the
+// partial sums (the same values in every block), block 0 writes the next
+// state, every block updates its share of p = z + beta p.
+// FP64 CG: residue sqrt(s1) / normb and TNL's stopping rule
+// (IterativeSolver.hpp:98-121; nextIteration() counts every check).
+template< typename T, typename D >
+__global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __restrict__ part1,
+                                                 const double* __restrict__ part2, const T* __restrict__ r,
+                                                 const D* __restrict__ d, T* __restrict__ p,
+                                                 const CgDev* __restrict__ cur, CgDev* __restrict__ nxt, StopRule R )
+{
+   __shared__ double sm[ kFT / 64 ];
+   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+   if( ! cur->active ) {
+      if( lead )
+         *nxt = *cur;
+      return;
+   }
+   const double s2 = sum_parts( part1, sm );
+   const double s1n = sum_parts( part2, sm );
+   if( s2 == 0.0 ) {
+      if( lead ) {
+         *nxt = *cur;
+         nxt->res = 0.0;
+         nxt->s2zero = 1;
+         nxt->active = 0;
+      }
+      return;
+   }
+   const double s1 = cur->s1;
+   const double beta = s1 == 0.0 ? 0.0 : s1n / s1;
+   const double res = sqrt( s1n ) / cur->normb;
+   const bool go = check_next( R, res, cur->it );
+   if( lead ) {
+      *nxt = *cur;
+      nxt->s1 = s1n;
+      nxt->beta = beta;
+      nxt->res = res;
+      nxt->active = go ? 1 : 0;
+      nxt->it = cur->it + 1;
+   }
+   if( ! go )
+      return;
+   const T tb = (T) beta;
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+#pragma unroll
+      for( int u = 0; u < kU; u++ ) {
+         const int64_t i = i0 + u * kStride;
+         if( i < n ) {
+            const T ri = r[ i ];
+            p[ i ] = ( d ? ri / (T) d[ i ] : ri ) + tb * p[ i ];
+         }
+      }
+}
+
+// Mixed: the FP32 iteration's end; asks for a reliable update (and leaves p
+// to it) when the residual has dropped by delta since the last update, when
+// its estimate reaches the tolerance, or when the iteration budget is spent.
+__global__ void __launch_bounds__( kFT ) mx_dir( int64_t n, const double* __restrict__ part1,
+                                                 const double* __restrict__ part2, const double* __restrict__ part3,
+                                                 const float* __restrict__ r, const float* __restrict__ d,
+                                                 float* __restrict__ p, const CgDev* __restrict__ cur,
+                                                 CgDev* __restrict__ nxt, double delta2, double tol, int64_t maxit )
+{
+   __shared__ double sm[ kFT / 64 ];
+   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+   if( ! cur->active ) {
+      if( lead )
+         *nxt = *cur;
+      return;
+   }
+   const double s2 = sum_parts( part1, sm );
+   const double s1n = sum_parts( part2, sm );
+   const double rr = sum_parts( part3, sm );
+   const double s1 = cur->s1;
+   const int64_t it = cur->it + 1;
+   const double beta = s1 == 0.0 ? 0.0 : s1n / s1;
+   const double rr_max = fmax( cur->rr_max, rr );
+   const bool want = s2 == 0.0 || rr < delta2 * rr_max || sqrt( rr ) < tol * cur->normb || it >= maxit || isnan( rr );
+   if( lead ) {
+      *nxt = *cur;
+      nxt->it = it;
+      nxt->s1_prev = s1;
+      if( s2 != 0.0 ) {
+         nxt->s1 = s1n;
+         nxt->beta = beta;
+         nxt->rr = rr;
+         nxt->rr_max = rr_max;
+      }
+      if( want ) {
+         nxt->want_update = 1;
+         nxt->active = 0;
+      }
+   }
+   if( want )
+      return;
+   const float tb = (float) beta;
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+#pragma unroll
+      for( int u = 0; u < kU; u++ ) {
+         const int64_t i = i0 + u * kStride;
+         if( i < n )
+            p[ i ] = r[ i ] / d[ i ] + tb * p[ i ];
+      }
+}
+
+// ---- set-up kernels
+__global__ void cg_start( CgDev* st, const double* s1, const double* normb, StopRule R )
+{
+   st->s1 = s1[ 0 ];
+   st->normb = normb[ 0 ];
+   st->res = sqrt( s1[ 0 ] ) / normb[ 0 ];
+   st->beta = 0;
+   st->s2zero = 0;
+   st->want_update = 0;
+   st->active = check_next( R, st->res, 0 ) ? 1 : 0;
+   st->it = 1;  // nextIteration() counts the admitted (or refused) iteration
+}
+
+// r = b - Ap (FP64), partial sums of r.r
+__global__ void __launch_bounds__( kFT ) resid_part( int64_t n, const double* __restrict__ b, double* __restrict__ r,
+                                                     double* __restrict__ part )
+{
+   __shared__ double sm[ kFT / 64 ];
+   double acc = 0;
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT ) {
+      const double ri = b[ i ] - r[ i ];
+      r[ i ] = ri;
+      acc += ri * ri;
+   }
+   put_part( acc, sm, part );
+}
+
+__global__ void __launch_bounds__( kFT ) sum_into( const double* __restrict__ part, double* out )
+{
+   __shared__ double sm[ kFT / 64 ];
+   const double v = sum_parts( part, sm );
+   if( threadIdx.x == 0 )
+      out[ 0 ] = v;
+}
+
+// Mixed restart after a residual replacement: r = (float) r64, xc = 0, and
+// the partial sums of (r, z), (r, r) and (r, p), z = r / d.
+__global__ void __launch_bounds__( kFT ) mx_restart( int64_t n, const double* __restrict__ r64,
+                                                     float* __restrict__ r, const float* __restrict__ d,
+                                                     const float* __restrict__ p, float* __restrict__ xc,
+                                                     double* __restrict__ part2, double* __restrict__ part3,
+                                                     double* __restrict__ part4 )
+{
+   __shared__ double sm[ kFT / 64 ];
+   double a2 = 0, a3 = 0, a4 = 0;
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT ) {
+      const float ri = (float) r64[ i ];
+      const float zi = ri / d[ i ];
+      r[ i ] = ri;
+      xc[ i ] = 0.0f;
+      a2 += (double) ri * (double) zi;
+      a3 += (double) ri * (double) ri;
+      a4 += (double) ri * (double) p[ i ];
+   }
+   put_part( a2, sm, part2 );
+   put_part( a3, sm, part3 );
+   put_part( a4, sm, part4 );
+}
+
+// The scalars after a replacement.  first: p = z.  Otherwise the kept
+// direction is made orthogonal to the new residual (p -= ((r, p) / (r, z)) z,
+// as exact CG has (r_{k+1}, p_k) = 0) and p = z + beta p with
+// beta = (r, z) / (r, z)_previous -- the residual replacement of Clark et al.
+__global__ void __launch_bounds__( kFT ) mx_state( CgDev* st, const double* __restrict__ part2,
+                                                   const double* __restrict__ part3, const double* __restrict__ part4,
+                                                   const double* normb, int64_t it, int first )
+{
+   __shared__ double sm[ kFT / 64 ];
+   const double s1 = sum_parts( part2, sm );
+   const double rr = sum_parts( part3, sm );
+   const double rp = sum_parts( part4, sm );
+   if( threadIdx.x != 0 )
+      return;
+   if( first ) {
+      st->beta = 0.0;
+      st->proj = 0.0;
+   }
+   else {
+      st->proj = s1 != 0.0 ? rp / s1 : 0.0;
+      st->beta = st->s1_prev != 0.0 ? s1 / st->s1_prev : 0.0;
+   }
+   st->s1 = s1;
+   st->rr = rr;
+   st->rr_max = rr;
+   st->normb = normb[ 0 ];
+   st->want_update = 0;
+   st->it = it;
+   st->active = 1;
+}
+
+// p = z + beta (p - proj z), z = r / d (p = z on the first start)
+__global__ void __launch_bounds__( kFT ) mx_redir( int64_t n, const float* __restrict__ r, const float* __restrict__ d,
+                                                   float* __restrict__ p, const CgDev* __restrict__ st, int first )
+{
+   const float beta = (float) st->beta, proj = (float) st->proj;
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT ) {
+      const float zi = r[ i ] / d[ i ];
+      p[ i ] = first ? zi : zi + beta * ( p[ i ] - proj * zi );
+   }
+}
+
+// x += xc (FP64 accumulation of the FP32 correction)
+__global__ void __launch_bounds__( kFT ) mx_accumulate( int64_t n, const float* __restrict__ xc, double* __restrict__ x )
+{
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT )
+      x[ i ] += (double) xc[ i ];
+}
+
+template< typename D >
+__global__ void to_f32( int64_t n, const D* __restrict__ a, float* __restrict__ o )
+{
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT )
+      o[ i ] = (float) a[ i ];
+}
+
+__global__ void max_len( const int32_t* __restrict__ sizes, int64_t n, int* out )
+{
+   int m = 0;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
+      m = max( m, sizes[ i ] );
+#pragma unroll
+   for( int o = 32; o > 0; o /= 2 )
+      m = max( m, __shfl_xor( m, o, 64 ) );
+   if( ( threadIdx.x & 63 ) == 0 )
+      atomicMax( out, m );
+}
+
+// ---- which SpMV stage a matrix gets
+struct Stage
+{
+   const tnlmi_matrix* A = nullptr;
+   const tnlmi_spmv_plan* plan = nullptr;
+   SegView v{};
+   int kind = 0;  // 0: the format's SpMV + dot_part, 1: Ellpack/SlicedEllpack column-major, 2: CSR G lanes
+   int G = 8;
+   int64_t grid = 0;                      // blocks of the fused SpMV kernel
+   std::shared_ptr< DeviceArray< double > > part;  // its per-block partial sums
+};
+
+Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_t s )
+{
+   Stage S;
+   S.A = A;
+   S.plan = plan;
+   S.v = make_view( *A->segments );
+   if( plan )
+      return S;
+   const int fmt = S.v.format;
+   if( ( fmt == TNLMI_ELLPACK || fmt == TNLMI_SLICED_ELLPACK ) && S.v.col_major ) {
+      S.kind = 1;
+      S.grid = ceil_div( S.v.n, kFT );
+      S.part = std::make_shared< DeviceArray< double > >();
+      S.part->allocate( S.grid );
+      return S;
+   }
+   if( fmt == TNLMI_CSR && S.v.n > 0 ) {
+      // short rows only: G lanes per row, G the power of two at or above the
+      // mean row length, and no row longer than 8 G
+      DeviceArray< int > m;
+      m.allocate( 1 );
+      TNLMI_HIP( hipMemsetAsync( m.ptr, 0, sizeof( int ), s ) );
+      hipLaunchKernelGGL( max_len, dim3( 1024 ), dim3( 256 ), 0, s, S.v.sizes, S.v.n, m.ptr );
+      int mx = 0, nnz = 0;
+      TNLMI_HIP( hipMemcpyAsync( &mx, m.ptr, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipMemcpyAsync( &nnz, S.v.offsets + S.v.n, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipStreamSynchronize( s ) );
+      const double mean = (double) nnz / (double) S.v.n;
+      int g = 2;
+      while( g < 64 && g < mean )
+         g *= 2;
+      if( mx <= 8 * g ) {
+         S.kind = 2;
+         S.G = g;
+         S.grid = ceil_div( S.v.n, kFT / g );
+         S.part = std::make_shared< DeviceArray< double > >();
+         S.part->allocate( S.grid );
+      }
+   }
+   return S;
+}
+
+template< typename V, typename T >
+void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDev* st, hipStream_t s )
+{
+   const V* vals = (const V*) S.A->values;
+   double* bp = S.part->ptr;
+   if( S.kind == 1 )
+      hipLaunchKernelGGL( ( spmv_dot_ell< V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
+                          S.A->column_indexes, vals, p, Ap, bp, st );
+   else {
+      switch( S.G ) {
+#define TNLMI_CSR_G( GG )                                                                                            \
+   case GG:                                                                                                          \
+      hipLaunchKernelGGL( ( spmv_dot_csr< GG, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v.n,        \
+                          S.v.offsets, S.A->column_indexes, vals, p, Ap, bp, st );                                   \
+      break;
+         TNLMI_CSR_G( 2 )
+         TNLMI_CSR_G( 4 )
+         TNLMI_CSR_G( 8 )
+         TNLMI_CSR_G( 16 )
+         TNLMI_CSR_G( 32 )
+         TNLMI_CSR_G( 64 )
+#undef TNLMI_CSR_G
+         default:
+            fail( TNLMI_LOGIC_ERROR, "fused CG: bad lane count" );
+      }
+   }
+   hipLaunchKernelGGL( fold_parts, dim3( kFB ), dim3( kFT ), 0, s, bp, S.grid, part, st );
+}
+
+// Ap = A p and the partial sums of (Ap, p)
+template< typename T >
+void stage_launch( const Stage& S, const T* p, T* Ap, double* part, const CgDev* st, hipStream_t s )
+{
+   constexpr tnlmi_dtype xt = std::is_same_v< T, double > ? TNLMI_F64 : TNLMI_F32;
+   if( S.kind == 0 ) {
+      if( S.plan )
+         plan_execute( *S.plan, xt, p, Ap, xt, 1.0, 0.0, 0, S.v.n, s );
+      else {
+         const tnlmi_launch cfg{ TNLMI_MAP_AUTO, 1 };
+         spmv_dispatch( *S.A, xt, p, Ap, xt, 1.0, 0.0, 0, S.v.n, cfg, s );
+      }
+      hipLaunchKernelGGL( dot_part< T >, dim3( kFB ), dim3( kFT ), 0, s, S.v.n, Ap, p, part, st );
+      return;
+   }
+   switch( S.A->value_type ) {
+      case TNLMI_F64:
+         stage_launch_v< double, T >( S, p, Ap, part, st, s );
+         return;
+      case TNLMI_F32:
+         stage_launch_v< float, T >( S, p, Ap, part, st, s );
+         return;
+      case TNLMI_BF16:
+         stage_launch_v< bf16, T >( S, p, Ap, part, st, s );
+         return;
+   }
+   fail( TNLMI_INVALID_ARGUMENT, "fused CG: unsupported value type" );
+}
+
+struct Parts
+{
+   DeviceArray< double > a, b, c;
+   Parts()
+   {
+      a.allocate( kFB );
+      b.allocate( kFB );
+      c.allocate( kFB );
+   }
+};
+
+// pinned host copy of the device state
+struct HostState
+{
+   CgDev* h = nullptr;
+   HostState() { TNLMI_HIP( hipHostMalloc( (void**) &h, sizeof( CgDev ), hipHostMallocDefault ) ); }
+   ~HostState()
+   {
+      if( h )
+         (void) hipHostFree( h );
+   }
+   const CgDev& read( const CgDev* d, hipStream_t s )
+   {
+      TNLMI_HIP( hipMemcpyAsync( h, d, sizeof( CgDev ), hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipStreamSynchronize( s ) );
+      return *h;
+   }
+};
+
+}  // namespace
+
+void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x,
+                     tnlmi_solver_state* st, hipStream_t s )
+{
+   const int64_t n = A.rows;
+   const double* d = precond_diagonal( M );
+   DeviceArray< double > r, p, Ap, sc;
+   DeviceArray< CgDev > dev;
+   r.allocate( n );
+   p.allocate( n );
+   Ap.allocate( n );
+   sc.allocate( 4 );
+   dev.allocate( 2 );
+   Parts parts;
+   RedWork w;
+   HostState hs;
+   const StopRule R{ st->max_iterations, st->min_iterations, st->convergence_residue, st->divergence_residue };
+   // normb = ||M^{-1} b|| (CG.hpp:24-29)
+   if( d ) {
+      precond_apply( M, n, b, r.ptr, s );
+      nrm2( n, r.ptr, sc.ptr + 1, w, s );
+   }
+   else
+      nrm2( n, b, sc.ptr + 1, w, s );
+   double normb = 0;
+   TNLMI_HIP( hipMemcpyAsync( &normb, sc.ptr + 1, sizeof( double ), hipMemcpyDeviceToHost, s ) );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+   if( normb == 0.0 ) {
+      fill( n, 0.0, x, s );
+      st->iterations = 0;
+      st->residue = 0.0;
+      st->converged = 1;
+      return;
+   }
+   // r = b - A x; p = z = M^{-1} r; s1 = (r, z)
+   A.apply( x, r.ptr, s );
+   rsub( n, b, r.ptr, s );
+   precond_apply( M, n, r.ptr, p.ptr, s );
+   dot( n, r.ptr, p.ptr, sc.ptr + 0, w, s );
+   hipLaunchKernelGGL( cg_start, dim3( 1 ), dim3( 1 ), 0, s, dev.ptr, sc.ptr + 0, sc.ptr + 1, R );
+   TNLMI_HIP( hipGetLastError() );
+   const Stage S = make_stage( A.A, A.plan, s );
+   // the state is double-buffered by iteration parity: iteration k reads
+   // dev[k & 1] and its last kernel writes dev[(k + 1) & 1]
+   int par = 0;
+   const CgDev* h = &hs.read( dev.ptr, s );
+   while( h->active ) {
+      for( int k = 0; k < kBatch; k++, par ^= 1 ) {
+         CgDev* cur = dev.ptr + par;
+         stage_launch< double >( S, p.ptr, Ap.ptr, parts.a.ptr, cur, s );
+         hipLaunchKernelGGL( ( cg_upd< double, double, double > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
+                             r.ptr, p.ptr, Ap.ptr, x, d, parts.b.ptr, (double*) nullptr, cur );
+         hipLaunchKernelGGL( ( cg_dir< double, double > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
+                             parts.b.ptr, r.ptr, d, p.ptr, cur, dev.ptr + ( par ^ 1 ), R );
+      }
+      TNLMI_HIP( hipGetLastError() );
+      h = &hs.read( dev.ptr + par, s );
+   }
+   Stop S2{ st };
+   S2.it = h->it;
+   S2.res = h->res;
+   st->iterations = h->it;
+   st->residue = h->res;
+   st->converged = S2.converged();
+}
+
+}  // namespace tnlmi
+
+using namespace tnlmi;
+
+extern "C" {
+
+tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const tnlmi_spmv_plan* plan,
+                                   double delta, const double* d_b, double* d_x, tnlmi_solver_state* st,
+                                   int64_t* updates, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( ! A || ! A_lo || ! A->segments || ! A_lo->segments || ! d_b || ! d_x || ! st )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed_cg_solve: NULL argument" );
+      if( A->value_type != TNLMI_F64 )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed_cg_solve: the outer matrix must hold FP64 values" );
+      if( A_lo->value_type != TNLMI_F32 && A_lo->value_type != TNLMI_BF16 )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed_cg_solve: the inner matrix must hold FP32 or BF16 values" );
+      if( A_lo->rows != A->rows || A_lo->columns != A->columns || A->rows != A->columns )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed_cg_solve: the matrices must be square and of one size" );
+      if( ! ( delta > 0.0 && delta < 1.0 ) )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed_cg_solve: delta must lie in (0, 1)" );
+      hipStream_t s = as_stream( stream );
+      const int64_t n = A->rows;
+      const Operator op{ A, plan, n };
+      DeviceArray< double > r64, d64, sc;
+      DeviceArray< float > r, p, Ap, xc, d32;
+      DeviceArray< CgDev > dev;
+      for( auto* v : { &r64, &d64 } )
+         v->allocate( n );
+      for( auto* v : { &r, &p, &Ap, &xc, &d32 } )
+         v->allocate( n );
+      sc.allocate( 4 );
+      dev.allocate( 2 );
+      TNLMI_HIP( hipMemsetAsync( p.ptr, 0, sizeof( float ) * n, s ) );
+      Parts parts;
+      RedWork w;
+      HostState hs;
+      // Jacobi: d = diag(A) (Diagonal.hpp:27-30); d32 = (float) d
+      {
+         tnlmi_preconditioner* M = nullptr;
+         const tnlmi_status rc = tnlmi_preconditioner_create( TNLMI_PRECOND_DIAGONAL, A, stream, &M );
+         if( rc != TNLMI_OK )
+            fail( rc, tnlmi_last_error() );
+         hipLaunchKernelGGL( to_f32< double >, dim3( kFB ), dim3( kFT ), 0, s, n, precond_diagonal( M ), d32.ptr );
+         TNLMI_HIP( hipStreamSynchronize( s ) );
+         tnlmi_preconditioner_destroy( M );
+      }
+      nrm2( n, d_b, sc.ptr + 0, w, s );
+      double normb = 0;
+      TNLMI_HIP( hipMemcpyAsync( &normb, sc.ptr, sizeof( double ), hipMemcpyDeviceToHost, s ) );
+      TNLMI_HIP( hipStreamSynchronize( s ) );
+      int64_t nupd = 0;
+      if( normb == 0.0 ) {
+         fill( n, 0.0, d_x, s );
+         st->iterations = 0;
+         st->residue = 0.0;
+         st->converged = 1;
+         if( updates )
+            *updates = 0;
+         return;
+      }
+      const Stage S = make_stage( A_lo, nullptr, s );
+      const StopRule R{ st->max_iterations, st->min_iterations, st->convergence_residue, st->divergence_residue };
+      int64_t it = 0;
+      double res = 0;
+      bool first = true;
+      int par = 0;  // the state is double-buffered by iteration parity
+      while( true ) {
+         // reliable update: r = b - A x in FP64 and the true residual
+         op.apply( d_x, r64.ptr, s );
+         hipLaunchKernelGGL( resid_part, dim3( kFB ), dim3( kFT ), 0, s, n, d_b, r64.ptr, parts.c.ptr );
+         hipLaunchKernelGGL( sum_into, dim3( 1 ), dim3( kFT ), 0, s, parts.c.ptr, sc.ptr + 1 );
+         double rr = 0;
+         TNLMI_HIP( hipMemcpyAsync( &rr, sc.ptr + 1, sizeof( double ), hipMemcpyDeviceToHost, s ) );
+         TNLMI_HIP( hipStreamSynchronize( s ) );
+         res = std::sqrt( rr ) / normb;
+         nupd++;
+         if( std::isnan( res ) || it > R.maxit || ( res > R.div && it >= R.minit ) || ( res < R.tol && it >= R.minit ) )
+            break;
+         // replace the FP32 residual by the FP64 one and form the next direction
+         CgDev* cur = dev.ptr + par;
+         hipLaunchKernelGGL( mx_restart, dim3( kFB ), dim3( kFT ), 0, s, n, r64.ptr, r.ptr, d32.ptr, p.ptr, xc.ptr,
+                             parts.a.ptr, parts.b.ptr, parts.c.ptr );
+         hipLaunchKernelGGL( mx_state, dim3( 1 ), dim3( kFT ), 0, s, cur, parts.a.ptr, parts.b.ptr, parts.c.ptr,
+                             sc.ptr + 0, it, first ? 1 : 0 );
+         hipLaunchKernelGGL( mx_redir, dim3( kFB ), dim3( kFT ), 0, s, n, r.ptr, d32.ptr, p.ptr, cur,
+                             first ? 1 : 0 );
+         first = false;
+         const CgDev* h = nullptr;
+         do {
+            for( int k = 0; k < kBatch; k++, par ^= 1 ) {
+               cur = dev.ptr + par;
+               stage_launch< float >( S, p.ptr, Ap.ptr, parts.a.ptr, cur, s );
+               hipLaunchKernelGGL( ( cg_upd< float, float, float > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
+                                   r.ptr, p.ptr, Ap.ptr, xc.ptr, d32.ptr, parts.b.ptr, parts.c.ptr, cur );
+               hipLaunchKernelGGL( mx_dir, dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr, parts.b.ptr, parts.c.ptr,
+                                   r.ptr, d32.ptr, p.ptr, cur, dev.ptr + ( par ^ 1 ), delta * delta, R.tol, R.maxit );
+            }
+            TNLMI_HIP( hipGetLastError() );
+            h = &hs.read( dev.ptr + par, s );
+         } while( h->active );
+         it = h->it;
+         hipLaunchKernelGGL( mx_accumulate, dim3( kFB ), dim3( kFT ), 0, s, n, xc.ptr, d_x );
+      }
+      st->iterations = it;
+      st->residue = res;
+      st->converged = ! std::isnan( res ) && res <= R.tol && it <= R.maxit ? 1 : 0;
+      if( updates )
+         *updates = nupd;
+   } );
+}
+
+}  // extern "C"

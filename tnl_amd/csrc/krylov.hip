@@ -283,6 +283,16 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
    }
 }
 
+tnlmi_precond_kind precond_kind( const tnlmi_preconditioner* P )
+{
+   return P ? P->kind : TNLMI_PRECOND_NONE;
+}
+
+const double* precond_diagonal( const tnlmi_preconditioner* P )
+{
+   return P && P->kind == TNLMI_PRECOND_DIAGONAL ? P->diag.ptr : nullptr;
+}
+
 // y = A x (FP64 vectors and accumulation) with the matrix or its plan
 void Operator::apply( const double* x, double* y, hipStream_t s ) const
 {
@@ -294,59 +304,7 @@ void Operator::apply( const double* x, double* y, hipStream_t s ) const
    }
 }
 
-// ---------------------------------------------------------- solver state
 namespace {
-
-// IterativeSolver.hpp:98-143
-struct Stop
-{
-   tnlmi_solver_state* st;
-   int64_t it = 0;
-   double res = 0;
-   bool check() const
-   {
-      return ! ( std::isnan( res ) || it > st->max_iterations
-                 || ( res > st->divergence_residue && it >= st->min_iterations )
-                 || ( res < st->convergence_residue && it >= st->min_iterations ) );
-   }
-   bool next()
-   {
-      const bool r = check();
-      it++;
-      return r;
-   }
-   bool converged() const
-   {
-      if( std::isnan( res ) )
-         return false;
-      if( res > st->divergence_residue && it > st->min_iterations )
-         return false;
-      if( it >= st->max_iterations )
-         return false;
-      if( res > st->convergence_residue )
-         return false;
-      return true;
-   }
-};
-
-// pinned host mirror of a few device scalars
-struct HostScalars
-{
-   double* h = nullptr;
-   HostScalars() { TNLMI_HIP( hipHostMalloc( (void**) &h, 64 * sizeof( double ), hipHostMallocDefault ) ); }
-   ~HostScalars()
-   {
-      if( h )
-         (void) hipHostFree( h );
-   }
-   // synchronous read of k device scalars starting at d
-   const double* read( const double* d, int k, hipStream_t s )
-   {
-      TNLMI_HIP( hipMemcpyAsync( h, d, k * sizeof( double ), hipMemcpyDeviceToHost, s ) );
-      TNLMI_HIP( hipStreamSynchronize( s ) );
-      return h;
-   }
-};
 
 // ---- CG kernels (CG.hpp:60-105), scalars in sc[]:
 // 0 s1, 1 s2, 2 normb, 3 residue, 4 alpha, 5 beta, 6 s1_new
@@ -396,6 +354,10 @@ inline dim3 vgrid( int64_t n )
 void cg_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x, tnlmi_solver_state* st,
                hipStream_t s )
 {
+   if( precond_kind( M ) != TNLMI_PRECOND_ILU0 ) {
+      fused_cg_solve( A, M, b, x, st, s );
+      return;
+   }
    const int64_t n = A.rows;
    DeviceArray< double > r, p, Ap, z, sc;
    r.allocate( n );

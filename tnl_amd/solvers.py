@@ -212,6 +212,40 @@ class MixedCG(IterativeSolver):
         return self._finish(st)
 
 
+class ReliableMixedCG(IterativeSolver):
+    """Jacobi-CG in FP32 on the FP32- or BF16-valued copy of the matrix with
+    reliable updates: x accumulated in FP64, the residual replaced by the FP64
+    b - A x whenever the FP32 residual has fallen by `delta` since the last
+    replacement (tnlmi_mixed_cg_solve).  Stops on the FP64 true residual;
+    getIterations() counts FP32 iterations, getUpdates() the FP64 residuals."""
+
+    def __init__(self):
+        super().__init__()
+        self.matrix_lo = None
+        self.delta = 0.1
+        self.updates = 0
+
+    def setInnerMatrix(self, matrix_lo):
+        self.matrix_lo = matrix_lo
+
+    def setDelta(self, delta):
+        self.delta = float(delta)
+
+    def getUpdates(self):
+        return self.updates
+
+    def solve(self, b, x):
+        m, plan, _ = self._handles()
+        if self.matrix_lo is None:
+            raise _lib.LogicError(_lib.LOGIC_ERROR, "No inner matrix was set. Call setInnerMatrix() first.")
+        st = self._state()
+        upd = ctypes.c_int64()
+        _lib.check(_lib.lib().tnlmi_mixed_cg_solve(m, ctypes.byref(self.matrix_lo._m), plan, self.delta, _f64(b),
+                                                   _f64(x), ctypes.byref(st), ctypes.byref(upd), _stream()))
+        self.updates = upd.value
+        return self._finish(st)
+
+
 def true_residual(matrix, x, b):
     """LinearResidueGetter::getResidue: ||A x - b|| / ||b||."""
     r = ctypes.c_double()
