@@ -217,7 +217,13 @@ typedef struct {
    int32_t converged;  /* out: the bool LinearSolver::solve returns */
 } tnlmi_solver_state;
 
-typedef enum { TNLMI_PRECOND_NONE = 0, TNLMI_PRECOND_DIAGONAL = 1, TNLMI_PRECOND_ILU0 = 2 } tnlmi_precond_kind;
+typedef enum {
+   TNLMI_PRECOND_NONE = 0,
+   TNLMI_PRECOND_DIAGONAL = 1,
+   TNLMI_PRECOND_ILU0 = 2,
+   TNLMI_PRECOND_EXPLICIT = 3 /* x = M b with the given matrix as M (e.g. a sparse approximate inverse);
+                                 the matrix must outlive the preconditioner.  No reference counterpart. */
+} tnlmi_precond_kind;
 typedef struct tnlmi_preconditioner tnlmi_preconditioner;
 
 /* Preconditioner::update(matrix) -- Preconditioners/Diagonal.hpp:12-31
@@ -290,6 +296,40 @@ tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_
                                    double delta, const double* d_b, double* d_x, tnlmi_solver_state* state,
                                    int64_t* updates, tnlmi_stream stream );
 
+/* ------------------------------------------- learned sparse approximate inverse */
+/* A message-passing network over the graph of a square CSR matrix A (FP64
+ * values, nonzero diagonal; an edge per stored entry, column -1 = padding)
+ * that predicts M = D^-1 (I + O) on A's own pattern, a left preconditioner
+ * (use M through TNLMI_PRECOND_EXPLICIT).  Hidden width 64, 3 layers: edge
+ * update E = relu(E We + (H Wu)[col] + (H Wv)[row] + be), node update
+ * H = relu(H Wh + mean_row(E) Wg + bh), decoder o = E wo + bo; the 64 x 64
+ * layers are BF16 GEMMs on the matrix cores with FP32 accumulation.
+ * Parameters: one FP32 vector of tnlmi_spai_parameter_count() entries
+ * (encoder Ae[4x64], ae, An[4x64], an; per layer We, Wu, Wv, Wh, Wg [64x64,
+ * row-major, y = x W], be, bh; wo[64], bo).  Training minimises the probe
+ * loss sum_k ||z_k - M A z_k||^2 / (K n) with Adam.  No reference
+ * counterpart (TNL has no learned preconditioner). */
+typedef struct tnlmi_spai tnlmi_spai;
+int64_t tnlmi_spai_parameter_count( void );
+tnlmi_status tnlmi_spai_create( const float* h_parameters, tnlmi_spai** out );
+void tnlmi_spai_destroy( tnlmi_spai* net );
+/* Host copies of the parameters and of the last computed gradients (either may be NULL). */
+tnlmi_status tnlmi_spai_get_parameters( const tnlmi_spai* net, float* h_parameters, float* h_gradients );
+/* M's values on A's pattern (storage order of A) into d_M (FP64).  Synchronous. */
+tnlmi_status tnlmi_spai_apply( const tnlmi_spai* net, const tnlmi_matrix* A, double* d_M, tnlmi_stream stream );
+/* Probe loss for the K probes d_Z (K x n, FP64) and its gradients (kept in
+ * the object); d_M (may be NULL) receives M.  Synchronous. */
+tnlmi_status tnlmi_spai_loss_grad( tnlmi_spai* net, const tnlmi_matrix* A, const double* d_Z, int32_t K,
+                                   double* loss, double* d_M, tnlmi_stream stream );
+/* One training step on A: K Rademacher probes drawn from probe_seed, loss,
+ * gradients, Adam (beta 0.9 / 0.999).  *loss = the loss before the step. */
+tnlmi_status tnlmi_spai_train_step( tnlmi_spai* net, const tnlmi_matrix* A, int32_t K, uint64_t probe_seed,
+                                    float learning_rate, double* loss, tnlmi_stream stream );
+/* Y[rows x 64] (FP32) = X[rows x 64] W[64 x 64] with BF16 X and W (row-major)
+ * on the matrix cores -- the dense layer of the network.  Synchronous. */
+tnlmi_status tnlmi_bf16_gemm64( int64_t rows, const uint16_t* d_X, const uint16_t* d_W, float* d_Y,
+                                tnlmi_stream stream );
+
 /* ------------------------------------------------------------ batched CG */
 /* Many independent SPD systems, the diagonal blocks of one block-diagonal
  * matrix A (FP64 values, any segment format; system k owns rows
@@ -338,6 +378,14 @@ void tnlmi_gen_zipf_sizes( int64_t n, double s, int32_t kmax, double mean, int32
  * row, for CSR offsets `offsets` (n+1 entries). */
 void tnlmi_gen_random_columns( int64_t n, int64_t ncols, const int32_t* offsets, uint64_t seed, int32_t* columns );
 /* out[i] ~ U[lo, hi) or N(0, 1) for i < n, from (seed, stream). */
+/* 2D convection-diffusion, m x m interior grid of the unit square, Dirichlet
+ * boundary, 5-point upwind finite differences, rows scaled by h^2/eps
+ * (diagonal 4 + h Pe (|vx| + |vy|), neighbours -1 - h Pe max(+-v, 0)),
+ * lexicographic rows, sorted columns; 5m^2 - 4m nonzeros.  Velocity: the
+ * curl of a stream function of 8 random sine modes (divergence-free, zero
+ * normal flow on the boundary) scaled to max |v| = 1; Peclet number
+ * Pe ~ U[1, 100].  Returns Pe; with offsets == NULL only returns it. */
+double tnlmi_gen_convdiff( int64_t m, uint64_t seed, int32_t* offsets, int32_t* columns, double* values );
 void tnlmi_gen_uniform( int64_t n, double lo, double hi, uint64_t seed, uint64_t stream, double* out );
 void tnlmi_gen_normal( int64_t n, uint64_t seed, uint64_t stream, double* out );
 /* Batch of 2D 5-point variable-coefficient diffusion systems as one

@@ -316,3 +316,30 @@ def test_reliable_mixed_cg_errors():
     x = torch.ones(n, dtype=torch.float64, device="cuda")
     assert s.solve(torch.zeros(n, dtype=torch.float64, device="cuda"), x)
     assert s.getIterations() == 0 and float(x.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("fmt,param", [(tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32), (tnl.SLICED_ELLPACK, 2),
+                                       (tnl.CSR, 0)])
+def test_fused_cg_ragged_rows(orc, fmt, param):
+    # 1073 rows: the multi-row SpMV kernels end inside a row group
+    off, cols, vals = tnl.laplace2d(37, 29)
+    n = len(off) - 1
+    b = np.random.default_rng(23).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+    cg = S.CG()
+    cg.setMatrix(A)
+    cg.setPreconditioner(S.Diagonal(A))
+    cg.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert cg.solve(dev(b), x)
+    xo, so = orc.cg(off, cols, vals, b, precond=1, tol=1e-10)
+    assert abs(cg.getIterations() - so.iterations) <= 1
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-9)
+    m = S.ReliableMixedCG()
+    m.setMatrix(A)
+    m.setInnerMatrix(tnl.SparseMatrix(A.rows, A.columns, A.segments, A.values.float(), A.column_indexes))
+    m.setConvergenceResidue(1e-10)
+    m.setMaxIterations(3000)
+    x.zero_()
+    assert m.solve(dev(b), x)
+    assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-10

@@ -277,6 +277,19 @@ def poisson3d(nx, ny, nz):
     return off, col, val
 
 
+def convdiff(m, seed=0):
+    """2D upwind convection-diffusion on an m x m grid (tnlmi_gen_convdiff):
+    (offsets, columns, values, peclet)."""
+    L = _lib.lib()
+    n = m * m
+    nnz = 5 * n - 4 * m
+    off = np.zeros(n + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float64)
+    pe = L.tnlmi_gen_convdiff(m, seed, _ptr(off), _ptr(col), _ptr(val))
+    return off, col, val, pe
+
+
 def zipf_csr(n, s=2.0, kmax=4096, mean=16.0, seed=0):
     """The power-law matrix: n x n, row lengths ~ Zipf(s) on [1, kmax] scaled to
     mean `mean` (clipped to [1, kmax]), uniform distinct sorted columns, N(0,1)

@@ -149,6 +149,16 @@ _PROTOS = {
                                          c_vp]),
     "tnlmi_mixed_cg_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, ctypes.c_double,
                                             c_vp, c_vp, ctypes.POINTER(SolverState), ctypes.POINTER(c_i64), c_vp]),
+    "tnlmi_spai_parameter_count": (c_i64, []),
+    "tnlmi_spai_create": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "tnlmi_spai_destroy": (None, [c_vp]),
+    "tnlmi_spai_get_parameters": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "tnlmi_spai_apply": (ctypes.c_int, [c_vp, ctypes.POINTER(Matrix), c_vp, c_vp]),
+    "tnlmi_spai_loss_grad": (ctypes.c_int, [c_vp, ctypes.POINTER(Matrix), c_vp, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_double), c_vp, c_vp]),
+    "tnlmi_spai_train_step": (ctypes.c_int, [c_vp, ctypes.POINTER(Matrix), ctypes.c_int32, ctypes.c_uint64,
+                                             ctypes.c_float, ctypes.POINTER(ctypes.c_double), c_vp]),
+    "tnlmi_bf16_gemm64": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "tnlmi_batched_cg_create": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_i64,
                                                ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)]),
     "tnlmi_batched_cg_destroy": (None, [c_vp]),
@@ -156,6 +166,7 @@ _PROTOS = {
                                               ctypes.c_double, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "tnlmi_gen_batched_diffusion": (None, [c_i64, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_vp, c_vp, c_vp,
                                            c_vp, c_vp, c_vp]),
+    "tnlmi_gen_convdiff": (ctypes.c_double, [c_i64, ctypes.c_uint64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_laplace2d": (c_i64, [c_i64, c_i64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_poisson3d": (c_i64, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "tnlmi_gen_zipf_sizes": (None, [c_i64, ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,

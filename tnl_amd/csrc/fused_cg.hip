@@ -84,84 +84,94 @@ __device__ __forceinline__ void put_part( double v, double* sm, double* part )
       part[ blockIdx.x ] = v;
 }
 
-// ---- fused SpMV + (Ap, p) partial sums
+// ---- fused SpMV + (Ap, p) partial sums (column-major Ellpack and
+// SlicedEllpack; every other format runs its own SpMV kernel, then dot_part)
 // Loops stride over the grid kU rows / elements at a time, so that every
 // thread has kU independent loads in flight.
 constexpr int kU = 4;
 constexpr int64_t kStride = (int64_t) kFB * kFT;
 
-// Ellpack / SlicedEllpack, column-major: one thread per row, coalesced
-// slots, a full grid of row blocks placed XCD-contiguously (the rows that
-// share x entries -- a stencil's neighbouring rows and planes -- are read by
-// one XCD); one partial sum per block.
-template< typename V, typename T >
+// Ellpack / SlicedEllpack, column-major: each thread R adjacent rows, whose
+// slots are adjacent in every slot column (vector loads of R column indexes
+// and R values), a full grid of row blocks placed XCD-contiguously (the rows
+// that share x entries -- a stencil's neighbouring rows and planes -- are
+// read by one XCD); one partial sum per block.  R divides the Ellpack row
+// alignment (32) and the slice size, so a group never straddles a slice and
+// its loads stay inside the padded storage (padding slots hold column -1).
+template< typename E, int R >
+struct Vec
+{
+   typedef E type __attribute__( ( ext_vector_type( R ) ) );
+};
+template< typename V, int R >
+struct ValVec
+{
+   using type = typename Vec< V, R >::type;
+   static __device__ __forceinline__ type load( const V* p ) { return __builtin_nontemporal_load( (const type*) p ); }
+   template< typename T >
+   static __device__ __forceinline__ T get( const type& v, int j )
+   {
+      return (T) v[ j ];
+   }
+};
+template< int R >
+struct ValVec< bf16, R >
+{
+   using type = typename Vec< uint16_t, R >::type;
+   static __device__ __forceinline__ type load( const bf16* p )
+   {
+      return __builtin_nontemporal_load( (const type*) p );
+   }
+   template< typename T >
+   static __device__ __forceinline__ T get( const type& v, int j )
+   {
+      return (T) bf16_to_float( v[ j ] );
+   }
+};
+
+template< int R, typename V, typename T >
 __global__ void __launch_bounds__( kFT ) spmv_dot_ell( SegView v, const int32_t* __restrict__ cols,
                                                        const V* __restrict__ vals, const T* __restrict__ p,
                                                        T* __restrict__ Ap, double* __restrict__ part,
                                                        const CgDev* __restrict__ st )
 {
+   using CV = typename Vec< int32_t, R >::type;
+   using VV = ValVec< V, R >;
    __shared__ double sm[ kFT / 64 ];
    if( ! st->active )
       return;
    const int64_t blk = xcd_contiguous( blockIdx.x, gridDim.x );
-   const int64_t r = blk * kFT + threadIdx.x;
+   const int64_t r0 = ( blk * kFT + threadIdx.x ) * R;
    double acc = 0;
-   if( r < v.n ) {
+   if( r0 < v.n ) {
       int64_t g, stride;
       int32_t w;
       if( v.format == TNLMI_ELLPACK ) {
-         g = r;
+         g = r0;
          stride = v.ell_aligned;
          w = v.ell_width;
       }
       else {
-         const int64_t sl = r / v.slice_size;
-         g = v.slice_offsets[ sl ] + ( r - sl * v.slice_size );
+         const int64_t sl = r0 / v.slice_size;
+         g = v.slice_offsets[ sl ] + ( r0 - sl * v.slice_size );
          stride = v.slice_size;
          w = v.slice_seg_sizes[ sl ];
       }
-      T y = 0;
+      T y[ R ] = {};
       for( int32_t k = 0; k < w; k++, g += stride ) {
-         const int32_t c = __builtin_nontemporal_load( cols + g );
-         if( c >= 0 )
-            y += nt_load_val< V, T >( vals, g ) * p[ c ];
+         const CV c = __builtin_nontemporal_load( (const CV*) ( cols + g ) );
+         const typename VV::type a = VV::load( vals + g );
+#pragma unroll
+         for( int j = 0; j < R; j++ )
+            if( c[ j ] >= 0 )
+               y[ j ] += VV::template get< T >( a, j ) * p[ c[ j ] ];
       }
-      Ap[ r ] = y;
-      acc = (double) y * (double) p[ r ];
-   }
-   const double bs = block_sum< kFT >( acc, sm );
-   if( threadIdx.x < 64 )
-      part[ blk ] = bs;
-}
-
-// CSR with short rows: G lanes per row, a full grid of row blocks placed
-// XCD-contiguously, one partial sum per block
-template< int G, typename V, typename T >
-__global__ void __launch_bounds__( kFT ) spmv_dot_csr( int64_t n, const int32_t* __restrict__ off,
-                                                       const int32_t* __restrict__ cols, const V* __restrict__ vals,
-                                                       const T* __restrict__ p, T* __restrict__ Ap,
-                                                       double* __restrict__ part, const CgDev* __restrict__ st )
-{
-   __shared__ double sm[ kFT / 64 ];
-   if( ! st->active )
-      return;
-   const int lane = threadIdx.x & ( G - 1 );
-   const int64_t blk = xcd_contiguous( blockIdx.x, gridDim.x );
-   const int64_t r = blk * ( kFT / G ) + threadIdx.x / G;
-   double acc = 0;
-   if( r < n ) {
-      const int32_t e1 = off[ r + 1 ];
-      T y = 0;
-      for( int32_t e = off[ r ] + lane; e < e1; e += G ) {
-         const int32_t c = __builtin_nontemporal_load( cols + e );
-         if( c >= 0 )
-            y += nt_load_val< V, T >( vals, e ) * p[ c ];
-      }
-      y = group_sum< G >( y );
-      if( lane == 0 ) {
-         Ap[ r ] = y;
-         acc = (double) y * (double) p[ r ];
-      }
+#pragma unroll
+      for( int j = 0; j < R; j++ )
+         if( r0 + j < v.n ) {
+            Ap[ r0 + j ] = y[ j ];
+            acc += (double) y[ j ] * (double) p[ r0 + j ];
+         }
    }
    const double bs = block_sum< kFT >( acc, sm );
    if( threadIdx.x < 64 )
@@ -460,27 +470,15 @@ __global__ void to_f32( int64_t n, const D* __restrict__ a, float* __restrict__ 
       o[ i ] = (float) a[ i ];
 }
 
-__global__ void max_len( const int32_t* __restrict__ sizes, int64_t n, int* out )
-{
-   int m = 0;
-   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x )
-      m = max( m, sizes[ i ] );
-#pragma unroll
-   for( int o = 32; o > 0; o /= 2 )
-      m = max( m, __shfl_xor( m, o, 64 ) );
-   if( ( threadIdx.x & 63 ) == 0 )
-      atomicMax( out, m );
-}
-
 // ---- which SpMV stage a matrix gets
 struct Stage
 {
    const tnlmi_matrix* A = nullptr;
    const tnlmi_spmv_plan* plan = nullptr;
    SegView v{};
-   int kind = 0;  // 0: the format's SpMV + dot_part, 1: Ellpack/SlicedEllpack column-major, 2: CSR G lanes
-   int G = 8;
+   int kind = 0;  // 0: the format's own SpMV kernel + dot_part, 1: fused Ellpack/SlicedEllpack column-major
    int64_t grid = 0;                      // blocks of the fused SpMV kernel
+   int rows_per_thread = 1;               // Ellpack / SlicedEllpack kernel
    std::shared_ptr< DeviceArray< double > > part;  // its per-block partial sums
 };
 
@@ -495,33 +493,14 @@ Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_
    const int fmt = S.v.format;
    if( ( fmt == TNLMI_ELLPACK || fmt == TNLMI_SLICED_ELLPACK ) && S.v.col_major ) {
       S.kind = 1;
-      S.grid = ceil_div( S.v.n, kFT );
+      // 16-byte loads of the column indexes for 4-byte values, of the values for FP64
+      S.rows_per_thread = A->value_type == TNLMI_F64 ? 2 : 4;
+      if( fmt == TNLMI_SLICED_ELLPACK && S.v.slice_size % S.rows_per_thread != 0 )
+         S.rows_per_thread = 1;
+      S.grid = ceil_div( ceil_div( S.v.n, S.rows_per_thread ), kFT );
       S.part = std::make_shared< DeviceArray< double > >();
       S.part->allocate( S.grid );
       return S;
-   }
-   if( fmt == TNLMI_CSR && S.v.n > 0 ) {
-      // short rows only: G lanes per row, G the power of two at or above the
-      // mean row length, and no row longer than 8 G
-      DeviceArray< int > m;
-      m.allocate( 1 );
-      TNLMI_HIP( hipMemsetAsync( m.ptr, 0, sizeof( int ), s ) );
-      hipLaunchKernelGGL( max_len, dim3( 1024 ), dim3( 256 ), 0, s, S.v.sizes, S.v.n, m.ptr );
-      int mx = 0, nnz = 0;
-      TNLMI_HIP( hipMemcpyAsync( &mx, m.ptr, sizeof( int ), hipMemcpyDeviceToHost, s ) );
-      TNLMI_HIP( hipMemcpyAsync( &nnz, S.v.offsets + S.v.n, sizeof( int ), hipMemcpyDeviceToHost, s ) );
-      TNLMI_HIP( hipStreamSynchronize( s ) );
-      const double mean = (double) nnz / (double) S.v.n;
-      int g = 2;
-      while( g < 64 && g < mean )
-         g *= 2;
-      if( mx <= 8 * g ) {
-         S.kind = 2;
-         S.G = g;
-         S.grid = ceil_div( S.v.n, kFT / g );
-         S.part = std::make_shared< DeviceArray< double > >();
-         S.part->allocate( S.grid );
-      }
    }
    return S;
 }
@@ -531,26 +510,16 @@ void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDe
 {
    const V* vals = (const V*) S.A->values;
    double* bp = S.part->ptr;
-   if( S.kind == 1 )
-      hipLaunchKernelGGL( ( spmv_dot_ell< V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
-                          S.A->column_indexes, vals, p, Ap, bp, st );
-   else {
-      switch( S.G ) {
-#define TNLMI_CSR_G( GG )                                                                                            \
-   case GG:                                                                                                          \
-      hipLaunchKernelGGL( ( spmv_dot_csr< GG, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v.n,        \
-                          S.v.offsets, S.A->column_indexes, vals, p, Ap, bp, st );                                   \
-      break;
-         TNLMI_CSR_G( 2 )
-         TNLMI_CSR_G( 4 )
-         TNLMI_CSR_G( 8 )
-         TNLMI_CSR_G( 16 )
-         TNLMI_CSR_G( 32 )
-         TNLMI_CSR_G( 64 )
-#undef TNLMI_CSR_G
-         default:
-            fail( TNLMI_LOGIC_ERROR, "fused CG: bad lane count" );
-      }
+   if( S.kind == 1 ) {
+      if( S.rows_per_thread == 4 )
+         hipLaunchKernelGGL( ( spmv_dot_ell< 4, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
+                             S.A->column_indexes, vals, p, Ap, bp, st );
+      else if( S.rows_per_thread == 2 )
+         hipLaunchKernelGGL( ( spmv_dot_ell< 2, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
+                             S.A->column_indexes, vals, p, Ap, bp, st );
+      else
+         hipLaunchKernelGGL( ( spmv_dot_ell< 1, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
+                             S.A->column_indexes, vals, p, Ap, bp, st );
    }
    hipLaunchKernelGGL( fold_parts, dim3( kFB ), dim3( kFT ), 0, s, bp, S.grid, part, st );
 }

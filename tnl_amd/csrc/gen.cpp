@@ -268,6 +268,79 @@ void tnlmi_gen_batched_diffusion( int64_t nsys, int32_t smin, int32_t smax, uint
    offsets[ row0[ nsys ] ] = (int32_t) nz0[ nsys ];
 }
 
+double tnlmi_gen_convdiff( int64_t m, uint64_t seed, int32_t* off, int32_t* col, double* val )
+{
+   // Peclet number and the eight stream-function modes of this instance
+   const double pe = 1.0 + 99.0 * unit( rnd( seed, 201, 0 ) );
+   double amp[ 8 ], kp[ 8 ], kq[ 8 ];
+   for( int k = 0; k < 8; k++ ) {
+      kp[ k ] = (double) ( 1 + rnd( seed, 202, 2 * k ) % 4 );
+      kq[ k ] = (double) ( 1 + rnd( seed, 202, 2 * k + 1 ) % 4 );
+      const double u1 = 1.0 - unit( rnd( seed, 203, 2 * k ) ), u2 = unit( rnd( seed, 203, 2 * k + 1 ) );
+      amp[ k ] = std::sqrt( -2.0 * std::log( u1 ) ) * std::cos( 6.283185307179586 * u2 )
+               / std::sqrt( kp[ k ] * kp[ k ] + kq[ k ] * kq[ k ] );
+   }
+   const double h = 1.0 / (double) ( m + 1 ), pi = 3.141592653589793;
+   auto vel = [ & ]( int64_t ix, int64_t iy, double& vx, double& vy ) {
+      const double x = h * (double) ( ix + 1 ), y = h * (double) ( iy + 1 );
+      vx = 0.0;
+      vy = 0.0;
+      for( int k = 0; k < 8; k++ ) {
+         const double sx = std::sin( pi * kp[ k ] * x ), cx = std::cos( pi * kp[ k ] * x );
+         const double sy = std::sin( pi * kq[ k ] * y ), cy = std::cos( pi * kq[ k ] * y );
+         vx += amp[ k ] * pi * kq[ k ] * sx * cy;   //  d psi / dy
+         vy -= amp[ k ] * pi * kp[ k ] * cx * sy;   // -d psi / dx
+      }
+   };
+   if( ! off )
+      return pe;
+   const int64_t n = m * m;
+   double vmax = 0.0;
+#pragma omp parallel for schedule( static ) reduction( max : vmax )
+   for( int64_t i = 0; i < n; i++ ) {
+      double vx, vy;
+      vel( i % m, i / m, vx, vy );
+      vmax = std::max( vmax, std::sqrt( vx * vx + vy * vy ) );
+   }
+   // rows scaled by h^2 / eps: diffusion stencil (4, -1), upwind convection h * Pe * v / max|v|
+   const double c = vmax > 0.0 ? h * pe / vmax : 0.0;
+   int64_t acc = 0;
+   for( int64_t i = 0; i < n; i++ ) {
+      const int64_t ix = i % m, iy = i / m;
+      off[ i ] = (int32_t) acc;
+      acc += 1 + ( ix > 0 ) + ( ix < m - 1 ) + ( iy > 0 ) + ( iy < m - 1 );
+   }
+   off[ n ] = (int32_t) acc;
+#pragma omp parallel for schedule( static )
+   for( int64_t i = 0; i < n; i++ ) {
+      const int64_t ix = i % m, iy = i / m;
+      double vx, vy;
+      vel( ix, iy, vx, vy );
+      vx *= c;
+      vy *= c;
+      int64_t q = off[ i ];
+      if( iy > 0 ) {
+         col[ q ] = (int32_t) ( i - m );
+         val[ q++ ] = -1.0 - std::max( vy, 0.0 );
+      }
+      if( ix > 0 ) {
+         col[ q ] = (int32_t) ( i - 1 );
+         val[ q++ ] = -1.0 - std::max( vx, 0.0 );
+      }
+      col[ q ] = (int32_t) i;
+      val[ q++ ] = 4.0 + std::fabs( vx ) + std::fabs( vy );
+      if( ix < m - 1 ) {
+         col[ q ] = (int32_t) ( i + 1 );
+         val[ q++ ] = -1.0 - std::max( -vx, 0.0 );
+      }
+      if( iy < m - 1 ) {
+         col[ q ] = (int32_t) ( i + m );
+         val[ q++ ] = -1.0 - std::max( -vy, 0.0 );
+      }
+   }
+   return pe;
+}
+
 void tnlmi_gen_uniform( int64_t n, double lo, double hi, uint64_t seed, uint64_t stream, double* out )
 {
 #pragma omp parallel for schedule( static )

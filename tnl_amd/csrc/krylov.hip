@@ -31,6 +31,7 @@ struct tnlmi_preconditioner
    DeviceArray< int32_t > lp, lc, up, uc, lrows, urows;
    DeviceArray< double > lv, uv;
    std::vector< int64_t > llev, ulev;  // level boundaries into lrows / urows
+   tnlmi_matrix M{};                   // explicit: x = M b
 };
 
 namespace tnlmi {
@@ -268,6 +269,11 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
       divide( n, b, P->diag.ptr, x, s );  // Diagonal.hpp:38-43: x = b / diagonal
       return;
    }
+   if( P->kind == TNLMI_PRECOND_EXPLICIT ) {
+      const tnlmi_launch cfg{ TNLMI_MAP_AUTO, 1 };
+      spmv_dispatch( P->M, TNLMI_F64, b, x, TNLMI_F64, 1.0, 0.0, 0, n, cfg, s );
+      return;
+   }
    // ILU0.hpp:99-113: triangularSolveLower then triangularSolveUpper<reversed rows>
    for( size_t l = 0; l + 1 < P->llev.size(); l++ ) {
       const int64_t cnt = P->llev[ l + 1 ] - P->llev[ l ];
@@ -354,7 +360,7 @@ inline dim3 vgrid( int64_t n )
 void cg_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x, tnlmi_solver_state* st,
                hipStream_t s )
 {
-   if( precond_kind( M ) != TNLMI_PRECOND_ILU0 ) {
+   if( precond_kind( M ) == TNLMI_PRECOND_NONE || precond_kind( M ) == TNLMI_PRECOND_DIAGONAL ) {
       fused_cg_solve( A, M, b, x, st, s );
       return;
    }
@@ -826,6 +832,8 @@ tnlmi_status tnlmi_preconditioner_create( tnlmi_precond_kind kind, const tnlmi_m
          }
          else if( kind == TNLMI_PRECOND_ILU0 )
             ilu0_build( *P, *A, s );
+         else if( kind == TNLMI_PRECOND_EXPLICIT )
+            P->M = *A;
          else if( kind != TNLMI_PRECOND_NONE )
             fail( TNLMI_INVALID_ARGUMENT, "preconditioner: unknown kind" );
       }

@@ -58,6 +58,16 @@ class ILU0(Preconditioner):
     kind = ILU0_KIND
 
 
+class Explicit(Preconditioner):
+    """x = M b with an explicit sparse M (e.g. a learned sparse approximate
+    inverse, tnl_amd.spai); M must stay alive while the preconditioner is used."""
+    kind = 3
+
+    def update(self, matrix):
+        self.M = matrix
+        return super().update(matrix)
+
+
 class IterativeSolver:
     def __init__(self):
         self.max_iterations = 1_000_000_000
