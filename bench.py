@@ -20,6 +20,9 @@ Other workloads (same JSON contract, their own `config.workload`):
   batched-cg     4,096 variable-coefficient diffusion systems (sides log-uniform on
                  [16, 128]) in one persistent graph-launched batched CG, vs a loop of
                  per-system CG solves
+  gmres-convdiff GMRES(30) to a 1e-10 true residual on 1024^2 convection-diffusion
+                 with the learned sparse approximate inverse (trained first on 2,000
+                 seeded 256^2 instances) vs Jacobi and ILU(0)
 
 All inputs are synthetic, from the package's seeded host generators.
 `--gpus N` with N > 1 exits: every GPU box of this build has one GPU.
@@ -520,8 +523,89 @@ def wl_batched_cg(args):
     return out
 
 
+def wl_gmres_convdiff(args):
+    import numpy as np
+    import torch
+    import tnl_amd
+    from tnl_amd import solvers
+    from tnl_amd import spai
+
+    tol = 1e-10
+    # training: seeded side x side instances, one Adam step each, on the GPU
+    net = spai.Network(seed=args.spai_seed)
+    t0 = time.perf_counter()
+    hist = spai.train(net, sizes=(args.spai_side,), instances=args.spai_instances, steps_per_instance=1,
+                      probes=args.spai_probes, lr=args.spai_lr, seed=100_000, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # the solve: an unseen m x m instance
+    m = args.side
+    off, cols, vals, pe = tnl_amd.convdiff(m, seed=args.solve_seed)
+    n, nnz = len(off) - 1, int(off[-1])
+    A = tnl_amd.SparseMatrix.from_csr(n, n, off, cols, vals)
+    b = torch.from_numpy(tnl_amd.normal(n, seed=3)).cuda()
+    log(f"convection-diffusion {m}^2: {n} rows, {nnz} nnz, Peclet {pe:.1f}")
+    res = {}
+
+    def run(label, make_pre, max_its):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pre, extra = make_pre()
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        ok, its, r = spai.gmres_solve(A, b, x, pre, tolerance=tol, restart=30, max_iterations=max_its,
+                                      variant=args.gmres_variant)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        rec = {"seconds": round(t, 4), "setup_seconds": round(t_setup, 4), "converged": ok, "iterations": its,
+               "true_residual": r, "ms_per_iteration": round((t - t_setup) / max(its, 1) * 1e3, 4)}
+        rec.update(extra)
+        log(f"{label}: {rec}")
+        res[label] = rec
+        return rec
+
+    # warm-up of the GMRES kernels
+    w = solvers.GMRES()
+    w.setMatrix(A)
+    w.setMaxIterations(5)
+    w.solve(b, torch.zeros(n, dtype=torch.float64, device="cuda"))
+    run("jacobi", lambda: (solvers.Diagonal(A), {}), args.max_iterations)
+
+    def learned():
+        return solvers.Explicit(net.inverse(A)), {}
+    run("learned_spai", learned, args.max_iterations)
+    run("ilu0", lambda: (solvers.ILU0(A), {}), args.ilu_max_iterations)
+    # probe loss of the two inverses on the solve matrix
+    Z = torch.from_numpy(np.random.default_rng(5).integers(0, 2, (4, n)) * 2.0 - 1.0).cuda()
+    L_learned, _ = net.loss_grad(A, Z)
+    L_jacobi, _ = spai.Network(seed=args.spai_seed).loss_grad(A, Z)
+    t_j, t_l = res["jacobi"]["seconds"], res["learned_spai"]["seconds"]
+    ok = res["jacobi"]["converged"] and res["learned_spai"]["converged"]
+    speedup = t_j / t_l if ok else 0.0
+    nb = spmv_bytes(n, nnz)
+    out = base_record(args, round(speedup, 4), "x (Jacobi-GMRES(30) time / learned-SPAI-GMRES(30) time, to 1e-10)",
+                      t_l * 1e3, "f64 solve, bf16 network (f32 accumulation)",
+                      "synthetic (seeded convection-diffusion instances; training on the GPU from random init)",
+                      {"workload": f"GMRES(30) to ||b-Ax||/||b|| <= 1e-10 on {m}^2 upwind convection-diffusion",
+                       "rows": n, "nnz": nnz, "peclet": pe, "results": res,
+                       "training": {"instances": args.spai_instances, "side": args.spai_side,
+                                    "probes": args.spai_probes, "lr": args.spai_lr, "seconds": round(t_train, 2),
+                                    "loss_first_50": float(np.mean(hist[:50])),
+                                    "loss_last_50": float(np.mean(hist[-50:]))},
+                       "probe_loss_on_solve_matrix": {"jacobi": L_jacobi, "learned": L_learned},
+                       "gmres_variant": args.gmres_variant,
+                       "note": "left-preconditioned GMRES(30); the solve time of the learned preconditioner includes "
+                               "the network's inference on the solve matrix; ILU(0) capped at --ilu-max-iterations"})
+    its_l = res["learned_spai"]["iterations"]
+    out["roofline"] = roofline(2 * nb * its_l / (t_l - res["learned_spai"]["setup_seconds"]) / 1e9)
+    out["roofline"]["kernel"] = "SpMV with A and with M per GMRES iteration (orthogonalisation not counted)"
+    out["cpu_baseline"] = None
+    return out
+
+
 WORKLOADS = {"spmv-powerlaw": wl_spmv_powerlaw, "spmv-laplace": wl_spmv_laplace, "spmv-formats": wl_spmv_formats,
-             "cg-poisson": wl_cg_poisson, "batched-cg": wl_batched_cg}
+             "cg-poisson": wl_cg_poisson, "batched-cg": wl_batched_cg, "gmres-convdiff": wl_gmres_convdiff}
 
 
 def main():
@@ -531,10 +615,19 @@ def main():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", choices=sorted(WORKLOADS), default="spmv-powerlaw")
     p.add_argument("--rows", type=int, default=N_ROWS)
-    p.add_argument("--side", type=int, default=256, help="cg-poisson grid side")
+    p.add_argument("--side", type=int, default=None, help="grid side (cg-poisson 256, gmres-convdiff 1024)")
     p.add_argument("--delta", type=float, default=0.1, help="cg-poisson: reliable-update threshold")
     p.add_argument("--systems", type=int, default=4096, help="batched-cg system count")
     p.add_argument("--loop-sample", type=int, default=256, help="batched-cg: systems timed in the per-system loop")
+    p.add_argument("--max-iterations", type=int, default=100_000, help="gmres-convdiff: GMRES iteration cap")
+    p.add_argument("--ilu-max-iterations", type=int, default=3000, help="gmres-convdiff: cap for ILU(0)")
+    p.add_argument("--solve-seed", type=int, default=7)
+    p.add_argument("--gmres-variant", default="CGSR", choices=["CGS", "CGSR", "MGS", "MGSR"])
+    p.add_argument("--spai-instances", type=int, default=2000)
+    p.add_argument("--spai-side", type=int, default=256)
+    p.add_argument("--spai-probes", type=int, default=4)
+    p.add_argument("--spai-lr", type=float, default=1e-3)
+    p.add_argument("--spai-seed", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-budget", type=float, default=15.0)
     p.add_argument("--copy-calibration", action="store_true",
@@ -545,6 +638,8 @@ def main():
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU (the value is the one "
               "GPU's throughput)", file=sys.stderr)
         sys.exit(2)
+    if args.side is None:
+        args.side = 1024 if args.workload == "gmres-convdiff" else 256
     import torch
     torch.cuda.set_device(0)
     out = WORKLOADS[args.workload](args)
