@@ -155,6 +155,9 @@ int main()
                                            { 3, 4, -1 },   { 4, 3, -1 },  { 4, 4, 2.5 } } );
    solveExample< Solvers::Linear::CG< Matrix > >( A, "CG" );
    solveExample< Solvers::Linear::CG< Matrix > >( A, "CG + Diagonal", true );
+   // GMRES with the class default orthogonalisation (CWY, GMRES.h:212)
+   solveExample< Solvers::Linear::GMRES< Matrix > >( A, "GMRES (default CWY)" );
+   solveExample< Solvers::Linear::GMRES< Matrix > >( A, "GMRES (default CWY) + Diagonal", true );
    {
       Solvers::Linear::GMRES< Matrix > g;
       Config::ParameterContainer p;

@@ -100,7 +100,7 @@ def test_cg_split_plan_and_stopping(orc):
     assert cg.solve(dev(np.zeros(n)), x) and float(x.abs().max()) == 0.0
 
 
-@pytest.mark.parametrize("variant", ["CGS", "CGSR", "MGS", "MGSR"])
+@pytest.mark.parametrize("variant", ["CGS", "CGSR", "MGS", "MGSR", "CWY"])
 @pytest.mark.parametrize("pre", [0, 1, 2], ids=["none", "diagonal", "ilu0"])
 def test_gmres(orc, variant, pre):
     off, cols, vals = nonsym(500, 7)
@@ -131,12 +131,27 @@ def test_gmres_errors():
     off, cols, vals = nonsym(50, 1)
     A = tnl.SparseMatrix.from_csr(50, 50, off, cols, vals)
     g.setMatrix(A)
-    assert g.setup(variant="CWY")
-    with pytest.raises(NotImplementedError):
-        g.solve(dev(np.ones(50)), dev(np.zeros(50)))
     assert not g.setup(variant="XYZ")
     g.setup(variant="MGS", restarting_min=20, restarting_max=10)
     assert not g.solve(dev(np.ones(50)), dev(np.zeros(50)))  # wrong restarting parameters -> false
+
+
+def test_gmres_cwy_is_the_cpp_default_and_matches_tnl(orc):
+    # CWY (compact-WY Householder) is GMRES's class default in the reference
+    # (GMRES.h:212); check it against TNL's own Host GMRES where that is built
+    off, cols, vals = nonsym(300, 11)
+    b = np.random.default_rng(12).standard_normal(300)
+    A = tnl.SparseMatrix.from_csr(300, 300, off, cols, vals)
+    g = S.GMRES()
+    g.setMatrix(A)
+    g.setup(variant="CWY", restarting_min=30, restarting_max=30)
+    g.setConvergenceResidue(1e-11)
+    x = torch.zeros(300, dtype=torch.float64, device="cuda")
+    assert g.solve(dev(b), x)
+    assert S.true_residual(A, x, dev(b)) < 1e-9
+    xo, so = orc.gmres(off, cols, vals, b, variant=4, rmin=30, rmax=30, tol=1e-11)
+    assert abs(g.getIterations() - so.iterations) <= 1
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-9)
 
 
 def test_jacobi(orc):

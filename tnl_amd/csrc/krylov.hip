@@ -635,11 +635,157 @@ __global__ void gm_normalize( int64_t n, const double* __restrict__ w, double* _
       v[ i ] = inv * w[ i ];
 }
 
+// ---- CWY: compact-WY Householder orthogonalisation (GMRES.hpp:345-431, 467-643)
+// Y: the Householder vectors (columns of length ld), T: the upper-triangular
+// compact-WY factor (R x R, R = rmax + 1), YL: the first R entries of each
+// Householder vector; the small arrays sit in one device buffer cw.
+struct CwyLayout
+{
+   int R;
+   int64_t T, YL, aux, total;
+   explicit CwyLayout( int rmax ) : R( rmax + 1 )
+   {
+      T = 0;
+      YL = T + (int64_t) R * R;
+      aux = YL + (int64_t) R * R;
+      total = aux + 64;
+   }
+};
+
+// y_i = z with the entries above i zeroed (hauseholder_generate, :467-478)
+__global__ void cwy_mask( int64_t n, const double* __restrict__ z, double* __restrict__ y, int i )
+{
+   for( int64_t j = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t) gridDim.x * blockDim.x )
+      y[ j ] = j < i ? 0.0 : z[ j ];
+}
+
+// y_i[i] += sign(y_ii) ||y_i||;  t_i = 2 / ||y_i||^2 (new y_i);  T[i, i] = t_i  (:480-492)
+__global__ void cwy_gen( double* __restrict__ y, int i, const double* __restrict__ normz, double* __restrict__ cw,
+                         CwyLayout C )
+{
+   const double nz = normz[ 0 ];
+   const double yii = y[ i ];
+   y[ i ] = yii > 0.0 ? yii + nz : yii - nz;
+   const double nys = 2.0 * ( nz * nz + fabs( yii ) * nz );
+   cw[ C.T + i + (int64_t) i * C.R ] = 2.0 / nys;
+}
+
+// T[k, i] = -sum_{j=k}^{i-1} T[k, j] t_i aux[j], aux[j] = (y_j, y_i)  (:494-510)
+__global__ void cwy_tcol( double* __restrict__ cw, CwyLayout C, int i, const double* __restrict__ aux )
+{
+   const int k = threadIdx.x;
+   if( k >= i )
+      return;
+   const double ti = cw[ C.T + i + (int64_t) i * C.R ];
+   double t = 0.0;
+   for( int j = k; j < i; j++ )
+      t -= cw[ C.T + k + (int64_t) j * C.R ] * ( ti * aux[ j ] );
+   cw[ C.T + k + (int64_t) i * C.R ] = t;
+}
+
+// YL[i] = the first R entries of y_i;  out[k] = z[k] - y_i[k], k <= i (hauseholder_apply_trunc, :512-538)
+__global__ void cwy_trunc( int64_t n, const double* __restrict__ y, int i, const double* __restrict__ z,
+                           double* __restrict__ cw, CwyLayout C, double* __restrict__ out )
+{
+   if( threadIdx.x != 0 )
+      return;
+   for( int k = 0; k < C.R && k < n; k++ )
+      cw[ C.YL + (int64_t) i * C.R + k ] = y[ k ];
+   for( int k = 0; k <= i; k++ )
+      out[ k ] = z[ k ] - cw[ C.YL + (int64_t) i * C.R + k ];
+}
+
+// aux = T (YL[k][i])_k  (upper-triangular product, hauseholder_cwy :540-563)
+__global__ void cwy_aux_upper( double* __restrict__ cw, CwyLayout C, int i )
+{
+   if( threadIdx.x != 0 )
+      return;
+   double* aux = cw + C.aux;
+   for( int k = 0; k <= i; k++ )
+      aux[ k ] = cw[ C.YL + i + (int64_t) k * C.R ];
+   for( int k = 0; k <= i; k++ ) {
+      double a2 = 0.0;
+      for( int j = k; j <= i; j++ )
+         a2 += cw[ C.T + k + (int64_t) j * C.R ] * aux[ j ];
+      aux[ k ] = a2;
+   }
+}
+
+// aux = T^T aux  (lower-triangular product, hauseholder_cwy_transposed :565-592)
+__global__ void cwy_aux_lower( double* __restrict__ cw, CwyLayout C, int i )
+{
+   if( threadIdx.x != 0 )
+      return;
+   double* aux = cw + C.aux;
+   for( int k = i; k >= 0; k-- ) {
+      double a2 = 0.0;
+      for( int j = 0; j <= k; j++ )
+         a2 += cw[ C.T + j + (int64_t) k * C.R ] * aux[ j ];
+      aux[ k ] = a2;
+   }
+}
+
+// v[i] += 1
+__global__ void cwy_unit( double* v, int i )
+{
+   v[ i ] += 1.0;
+}
+
+// apply_givens_rotations to column i of H (filled by cwy_trunc) and s;
+// residue = |s[i + 1]| / normb  (:758-770)
+__global__ void cwy_givens( double* g, GLayout L, int i, int m )
+{
+   double* H = g + L.H;
+   double* s = g + L.s;
+   double* cs = g + L.cs;
+   double* sn = g + L.sn;
+   for( int k = 0; k < i; k++ )
+      app_rot( H[ k + i * ( m + 1 ) ], H[ k + 1 + i * ( m + 1 ) ], cs[ k ], sn[ k ] );
+   if( H[ i + 1 + i * ( m + 1 ) ] != 0.0 ) {
+      gen_rot( H[ i + i * ( m + 1 ) ], H[ i + 1 + i * ( m + 1 ) ], cs[ i ], sn[ i ] );
+      app_rot( H[ i + i * ( m + 1 ) ], H[ i + 1 + i * ( m + 1 ) ], cs[ i ], sn[ i ] );
+      app_rot( s[ i ], s[ i + 1 ], cs[ i ], sn[ i ] );
+   }
+   g[ L.sc + 3 ] = fabs( s[ i + 1 ] ) / g[ L.sc + 0 ];
+}
+
+// residue = |s[0]| / normb (the first Householder step)
+__global__ void cwy_res0( double* g, GLayout L )
+{
+   g[ L.sc + 3 ] = fabs( g[ L.s ] ) / g[ L.sc + 0 ];
+}
+
+// update (:684-727): aux[i] = sum_{j=i}^{k} y[j] YL[i][j], aux = T aux (upper)
+__global__ void cwy_update_aux( double* __restrict__ cw, CwyLayout C, const double* __restrict__ g, GLayout L, int k )
+{
+   if( threadIdx.x != 0 )
+      return;
+   const double* y = g + L.y;
+   double* aux = cw + C.aux;
+   for( int i = 0; i <= k; i++ )
+      aux[ i ] = 0.0;
+   for( int j = 0; j <= k; j++ )
+      for( int i = 0; i <= j; i++ )
+         aux[ i ] += y[ j ] * cw[ C.YL + j + (int64_t) i * C.R ];
+   for( int i = 0; i <= k; i++ ) {
+      double a2 = 0.0;
+      for( int j = i; j <= k; j++ )
+         a2 += cw[ C.T + i + (int64_t) j * C.R ] * aux[ j ];
+      aux[ i ] = a2;
+   }
+}
+
+// x[j] += y[j], j <= k
+__global__ void cwy_add_y( double* __restrict__ x, const double* __restrict__ g, GLayout L, int k )
+{
+   const int j = threadIdx.x;
+   if( j <= k )
+      x[ j ] += g[ L.y + j ];
+}
+
 void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x, int variant, int rmin,
                   int rmax, int dmin, int dmax, tnlmi_solver_state* st, hipStream_t s )
 {
-   if( variant == 4 )
-      fail( TNLMI_NOT_IMPLEMENTED, "GMRES: the CWY variant is not implemented on this device" );
    if( rmin <= 0 || rmax <= 0 || rmin > rmax || dmin < 0 || dmax < 0 || dmin > dmax ) {
       st->converged = 0;
       st->iterations = 0;
@@ -706,11 +852,26 @@ void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double
    int m = rmax;
    const bool classical = variant <= 1;
    const int passes = ( variant == 1 || variant == 3 ) ? 2 : 1;
+   const bool cwy = variant == 4;
+   const CwyLayout C( rmax );
+   DeviceArray< double > Y, z, cwbuf;
+   if( cwy ) {
+      Y.allocate( ld * ( rmax + 1 ) );
+      z.allocate( n );
+      cwbuf.allocate( C.total );
+   }
+   double* cw = cwbuf.ptr;
    auto update = [ & ]( int k ) {
       hipLaunchKernelGGL( gm_backsub, dim3( 1 ), dim3( 1 ), 0, s, G, L, k, m );
       if( hs.read( G + L.sc + 4, 1, s )[ 0 ] != 0.0 )
          fail( TNLMI_RUNTIME_ERROR, "GMRES: zero on the diagonal of the Hessenberg matrix" );
-      gemv_add( n, k + 1, V.ptr, ld, G + L.y, x, s );
+      if( ! cwy )
+         gemv_add( n, k + 1, V.ptr, ld, G + L.y, x, s );
+      else {  // x += (I - Y T Y^T) applied to y (:684-727)
+         hipLaunchKernelGGL( cwy_update_aux, dim3( 1 ), dim3( 64 ), 0, s, cw, C, G, L, k );
+         gemv_sub( n, k + 1, Y.ptr, ld, cw + C.aux, x, s );
+         hipLaunchKernelGGL( cwy_add_y, dim3( 1 ), dim3( 64 ), 0, s, x, G, L, k );
+      }
    };
    while( S.check() ) {
       if( rmax > rmin && cycles > 0 ) {  // Baker-Jessup-Kolev adaptive restart, :121-146
@@ -724,11 +885,53 @@ void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double
                m = rmax;
          }
       }
+      int o = m;
+      if( cwy ) {
+         // z = r; H = 0; s = 0; T = 0   (orthogonalize_CWY, :345-360)
+         copy( n, r.ptr, z.ptr, s );
+         hipLaunchKernelGGL( gm_start, dim3( 1 ), dim3( 256 ), 0, s, G, L, m );
+         fill( (int64_t) C.R * C.R, 0.0, cw + C.T, s );
+         for( int i = 0; i <= m && S.next(); i++ ) {
+            double* yi = Y.ptr + i * ld;
+            const unsigned vg = (unsigned) std::min< int64_t >( 8192, ceil_div( n, 256 ) );
+            hipLaunchKernelGGL( cwy_mask, dim3( vg ), dim3( 256 ), 0, s, n, z.ptr, yi, i );
+            nrm2( n, yi, G + L.sc + 5, rw, s );
+            hipLaunchKernelGGL( cwy_gen, dim3( 1 ), dim3( 1 ), 0, s, yi, i, G + L.sc + 5, cw, C );
+            if( i > 0 ) {
+               multi_dot( n, i, Y.ptr, ld, yi, G + L.tmp, rw, s );
+               hipLaunchKernelGGL( cwy_tcol, dim3( 1 ), dim3( 64 ), 0, s, cw, C, i, G + L.tmp );
+            }
+            hipLaunchKernelGGL( cwy_trunc, dim3( 1 ), dim3( 64 ), 0, s, n, yi, i, z.ptr, cw, C,
+                                i == 0 ? G + L.s : G + L.H + (int64_t) ( i - 1 ) * ( m + 1 ) );
+            // v = (I - Y T Y^T) e_i
+            hipLaunchKernelGGL( cwy_aux_upper, dim3( 1 ), dim3( 64 ), 0, s, cw, C, i );
+            fill( n, 0.0, V.ptr, s );
+            gemv_sub( n, i + 1, Y.ptr, ld, cw + C.aux, V.ptr, s );
+            hipLaunchKernelGGL( cwy_unit, dim3( 1 ), dim3( 1 ), 0, s, V.ptr, i );
+            if( i < m ) {
+               // z = (I - Y T^T Y^T) M^{-1} A v
+               matvec_prec( V.ptr, wv.ptr );
+               multi_dot( n, i + 1, Y.ptr, ld, wv.ptr, cw + C.aux, rw, s );
+               hipLaunchKernelGGL( cwy_aux_lower, dim3( 1 ), dim3( 64 ), 0, s, cw, C, i );
+               copy( n, wv.ptr, z.ptr, s );
+               gemv_sub( n, i + 1, Y.ptr, ld, cw + C.aux, z.ptr, s );
+            }
+            if( i > 0 )
+               hipLaunchKernelGGL( cwy_givens, dim3( 1 ), dim3( 1 ), 0, s, G, L, i - 1, m );
+            else
+               hipLaunchKernelGGL( cwy_res0, dim3( 1 ), dim3( 1 ), 0, s, G, L );
+            S.res = hs.read( G + L.sc + 3, 1, s )[ 0 ];
+            if( i > 0 && ! S.check() ) {
+               o = i - 1;
+               break;
+            }
+         }
+      }
+      else {
       // v_0 = r / beta; H = 0; s = 0; s[0] = beta   (:190-202, :268-280)
       hipLaunchKernelGGL( gm_normalize, dim3( (unsigned) ceil_div( n, 256 ) ), dim3( 256 ), 0, s, n, r.ptr, V.ptr, G,
                           L.sc + 1 );
       hipLaunchKernelGGL( gm_start, dim3( 1 ), dim3( 256 ), 0, s, G, L, m );
-      int o = m;
       for( int i = 0; i < m && S.next(); i++ ) {
          const double* vi = V.ptr + i * ld;
          matvec_prec( vi, wv.ptr );
@@ -755,6 +958,7 @@ void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double
             o = i;
             break;
          }
+      }
       }
       if( o < m ) {
          update( o );
