@@ -172,18 +172,24 @@ tnlmi_status tnlmi_spmv( const tnlmi_matrix* A, tnlmi_dtype x_type, const void* 
                          tnlmi_dtype compute_type, double alpha, double beta, int64_t begin, int64_t end,
                          const tnlmi_launch* launch, tnlmi_stream stream );
 
-/* ------------------------------------------------ XCD column-split plan */
+/* ------------------------------------------- column-split / windowed plan */
 /* An execution plan of y = alpha*A*x + beta*y for a CSR matrix whose x
- * gathers are random (power-law graphs): the columns are cut into `parts`
- * ranges (8: one per XCD; 16: two per XCD) and a reordered copy of the CSR
+ * gathers are random (power-law graphs).  parts = 8 or 16: the columns are
+ * cut into `parts` ranges (one or two per XCD) and a reordered copy of the CSR
  * arrays is kept per range, so that every XCD gathers from one L2-sized slice
- * of x; per-range partial row sums are added in range order.  Built once
- * (synchronous); the plan owns its copy (the matrix may be freed), so value
- * changes of the matrix need a new plan.  Results equal vectorProduct up to
- * the FP summation order.  No reference counterpart (TNL's vectorProduct has
- * no analysis phase); it serves SparseMatrix::vectorProduct of CSR matrices.
- * Concurrent executions of one plan on different streams are not allowed
- * (the plan holds the partial sums). */
+ * of x; per-range partial row sums are added in range order.  parts = 0: the
+ * windowed plan -- the columns are cut into windows of 8192 entries, each
+ * window's products are formed with x staged in LDS and summed per row block
+ * in LDS; every row of at most 256 nonzeros is summed left to right in
+ * column order from zero, bit for bit as the host CSR vectorProduct, longer
+ * rows in a fixed order (rows longer than 8160 FP64 / 16320 FP32 nonzeros:
+ * TNLMI_NOT_IMPLEMENTED at creation).  Built once (synchronous); the plan
+ * owns its copy (the matrix may be freed), so value changes of the matrix
+ * need a new plan.  Results equal vectorProduct up to the FP summation order.
+ * No reference counterpart (TNL's vectorProduct has no analysis phase); it
+ * serves SparseMatrix::vectorProduct of CSR matrices.  Concurrent executions
+ * of one plan on different streams are not allowed (the plan holds the
+ * partial sums / products). */
 typedef struct tnlmi_spmv_plan tnlmi_spmv_plan;
 tnlmi_status tnlmi_spmv_plan_create( const tnlmi_matrix* A, int32_t parts, tnlmi_stream stream,
                                      tnlmi_spmv_plan** out );

@@ -182,7 +182,7 @@ def test_errors():
         A.vectorProduct(x, x.clone(), begin=10, end=5)
 
 
-@pytest.mark.parametrize("parts", [8, 16])
+@pytest.mark.parametrize("parts", [8, 16, 0], ids=["split8", "split16", "windowed"])
 def test_column_split_plan(orc, parts):
     rng = np.random.default_rng(parts)
     for n, m, maxlen, longs in [(1, 1, 1, ()), (300, 700, 30, ()), (20000, 30000, 40, [(3, 5000), (19999, 2500)])]:
@@ -216,3 +216,51 @@ def test_column_split_plan(orc, parts):
     P.vectorProduct(xd, yd)
     torch.cuda.synchronize()
     assert relerr(yd.cpu().numpy(), orc.spmv_omp(off, cols, vals, x, 16)) <= 1e-12
+
+
+def test_windowed_plan_matches_tnl_bit_for_bit(orc):
+    # rows summed by one thread: products rounded to FP64, then summed from
+    # zero left to right in column order -- TNL's host loop exactly
+    # (rows of at most 256 nonzeros); longer rows are summed by a wave in a
+    # fixed order and agree to rounding
+    rng = np.random.default_rng(5)
+    n, m = 60000, 300001  # 37 column windows, the last one partial
+    off, cols, vals = random_csr(n, m, 24, rng, long_rows=[(11, 256), (12, 257), (4000, 3000), (59999, 700)])
+    A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals)
+    P = tnl.SpmvPlan(A, 0)
+    x = rng.standard_normal(m)
+    xd, yd = gpu(x), torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.vectorProduct(xd, yd)
+    torch.cuda.synchronize()
+    got, want = yd.cpu().numpy(), orc.spmv(off, cols, vals, x)
+    short = np.diff(off) <= 256
+    assert short[11] and not short[12]
+    assert np.array_equal(got[short], want[short])
+    assert relerr(got[~short], want[~short]) <= 1e-13
+    ref = OL.Reference.load()  # TNL's own Host CSR vectorProduct, where built
+    if ref is not None:
+        assert np.array_equal(got[short], ref.spmv("CSR", off, cols, vals, x)[short])
+
+
+def test_windowed_plan_fp32_and_errors(orc):
+    rng = np.random.default_rng(6)
+    n, m = 5000, 20000
+    off, cols, vals = random_csr(n, m, 30, rng, long_rows=[(7, 9000)])
+    A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, value_dtype=torch.float32)
+    P = tnl.SpmvPlan(A, 0)
+    x = rng.standard_normal(m).astype(np.float32)
+    xd, yd = gpu(x).float(), torch.zeros(n, dtype=torch.float32, device="cuda")
+    P.vectorProduct(xd, yd)
+    torch.cuda.synchronize()
+    v32 = vals.astype(np.float32).astype(np.float64)
+    want = orc.spmv(off, cols, v32, x.astype(np.float64))
+    bound = 1e-6 * np.bincount(np.repeat(np.arange(n), np.diff(off)), weights=np.abs(v32 * np.where(
+        cols >= 0, x.astype(np.float64)[np.maximum(cols, 0)], 0.0)), minlength=n) * 64
+    assert np.all(np.abs(yd.cpu().numpy() - want) <= bound + 1e-30)
+    with pytest.raises(NotImplementedError):  # FP32 plan, FP64 vectors
+        P.vectorProduct(gpu(x.astype(np.float64)), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    # a row longer than a row block holds (8160 FP64 nonzeros) is refused at creation
+    off2, cols2, vals2 = random_csr(3, 20000, 4, rng, long_rows=[(1, 9000)])
+    B = tnl.SparseMatrix.from_csr(3, 20000, off2, cols2, vals2)
+    with pytest.raises(NotImplementedError):
+        tnl.SpmvPlan(B, 0)

@@ -223,7 +223,10 @@ class SparseMatrix:
 
 
 class SpmvPlan:
-    """XCD column-split execution plan of a CSR SparseMatrix (tnlmi_spmv_plan)."""
+    """Execution plan of a CSR SparseMatrix's vectorProduct (tnlmi_spmv_plan):
+    parts = 8 or 16, the XCD column-split plan; parts = 0, the windowed plan
+    (x staged in LDS per 8192-column window, rows of at most 256 nonzeros
+    summed bit for bit as the host CSR loop)."""
 
     def __init__(self, A, parts=8, stream=None):
         h = ctypes.c_void_p()

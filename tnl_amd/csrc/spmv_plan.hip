@@ -15,10 +15,12 @@
 // No reference counterpart: TNL's vectorProduct has no analysis phase.  The
 // plain CSR path (spmv.hip) reads the CSR arrays themselves.
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "segments.hpp"
 #include "spmv.hpp"
+#include "spmv_window.hpp"
 #include "spmv_tile.hpp"
 
 constexpr int kMaxParts = 16;
@@ -40,6 +42,7 @@ struct tnlmi_spmv_plan
    int64_t tiles[ kMaxParts ] = {};
    int64_t tile_base[ kMaxParts + 1 ] = {};     // first global tile of each range
    int64_t tilerow_base[ kMaxParts + 1 ] = {};  // first tile_rows entry of each range
+   std::unique_ptr< tnlmi::WindowPlan > window;  // parts == 0: the windowed plan
 };
 
 namespace tnlmi {
@@ -285,19 +288,23 @@ tnlmi_status tnlmi_spmv_plan_create( const tnlmi_matrix* A, int32_t parts, tnlmi
       *out = nullptr;
       if( A->segments->format != TNLMI_CSR )
          fail( TNLMI_INVALID_ARGUMENT, "spmv_plan_create: the column-split plan is built from a CSR matrix" );
-      if( parts != 8 && parts != 16 )
-         fail( TNLMI_INVALID_ARGUMENT, "spmv_plan_create: parts must be 8 or 16" );
+      if( parts != 0 && parts != 8 && parts != 16 )
+         fail( TNLMI_INVALID_ARGUMENT, "spmv_plan_create: parts must be 0 (windowed plan), 8 or 16" );
       if( A->rows != A->segments->n )
          fail( TNLMI_INVALID_ARGUMENT, "spmv_plan_create: matrix rows do not match the segments count" );
       auto* P = new tnlmi_spmv_plan;
       P->rows = A->rows;
       P->columns = A->columns;
       P->parts = parts;
-      P->part_width = std::max< int64_t >( 1, ceil_div( A->columns, parts ) );
+      P->part_width = parts > 0 ? std::max< int64_t >( 1, ceil_div( A->columns, parts ) ) : 0;
       P->value_type = A->value_type;
       try {
          hipStream_t s = as_stream( stream );
-         if( A->value_type == TNLMI_F64 )
+         if( parts == 0 ) {
+            P->window = std::make_unique< tnlmi::WindowPlan >();
+            tnlmi::window_build( *P->window, *A, s );
+         }
+         else if( A->value_type == TNLMI_F64 )
             build_typed< double >( *P, *A, s );
          else if( A->value_type == TNLMI_F32 )
             build_typed< float >( *P, *A, s );
@@ -324,6 +331,10 @@ namespace tnlmi {
 void plan_execute( const tnlmi_spmv_plan& P, tnlmi_dtype xt, const void* d_x, void* d_y, tnlmi_dtype ct,
                    double alpha, double beta, int64_t begin, int64_t end, hipStream_t s )
 {
+   if( P.window ) {
+      window_execute( *P.window, xt, d_x, d_y, ct, alpha, beta, begin, end, s );
+      return;
+   }
    const tnlmi_dtype vt = P.value_type;
    if( vt == TNLMI_F64 && xt == TNLMI_F64 && ct == TNLMI_F64 )
       execute_typed< double, double, double >( P, d_x, d_y, alpha, beta, begin, end, s );

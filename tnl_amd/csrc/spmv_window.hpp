@@ -1,0 +1,31 @@
+// spmv_window.hpp -- the windowed SpMV plan (spmv_window.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace tnlmi {
+
+struct WindowPlan
+{
+   int64_t rows = 0, columns = 0, nnz = 0;
+   tnlmi_dtype value_type = TNLMI_F64;
+   int32_t K = 0;    // column windows
+   int64_t B = 0;    // row blocks
+   int64_t cap = 0;  // nonzeros per row block at most
+   DeviceArray< int64_t > wstart;  // K + 2: first sorted nonzero of each window (window K: padding slots)
+   DeviceArray< int64_t > seg;     // (B + 1) x K: first nonzero of row block b in window k
+   DeviceArray< int32_t > rb;      // B + 1: first row of each row block
+   DeviceArray< int32_t > offs;    // rows + 1: the CSR offsets
+   DeviceArray< int32_t > lstart;  // B + 1: first entry of each row block in lrows
+   DeviceArray< int32_t > lrows;   // rows longer than the one-thread limit, by row block
+   DeviceArray< char > pv;         // nonzeros sorted by window (rows ascending inside), value type
+   DeviceArray< uint16_t > pc;     // their column inside the window
+   DeviceArray< uint16_t > dest;   // their CSR slot inside their row block
+   DeviceArray< char > prod;       // phase-1 products (compute type)
+};
+
+void window_build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s );
+void window_execute( const WindowPlan& W, tnlmi_dtype xt, const void* x, void* y, tnlmi_dtype ct, double alpha,
+                     double beta, int64_t begin, int64_t end, hipStream_t s );
+
+}  // namespace tnlmi
