@@ -259,8 +259,49 @@ def test_windowed_plan_fp32_and_errors(orc):
     assert np.all(np.abs(yd.cpu().numpy() - want) <= bound + 1e-30)
     with pytest.raises(NotImplementedError):  # FP32 plan, FP64 vectors
         P.vectorProduct(gpu(x.astype(np.float64)), torch.zeros(n, dtype=torch.float64, device="cuda"))
-    # a row longer than a row block holds (8160 FP64 nonzeros) is refused at creation
-    off2, cols2, vals2 = random_csr(3, 20000, 4, rng, long_rows=[(1, 9000)])
-    B = tnl.SparseMatrix.from_csr(3, 20000, off2, cols2, vals2)
-    with pytest.raises(NotImplementedError):
-        tnl.SpmvPlan(B, 0)
+    with pytest.raises(tnl.TnlmiError):  # parts must be 0, 8 or 16
+        tnl.SpmvPlan(A, 4)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_windowed_plan_rows_longer_than_a_block(orc, dtype):
+    # a row block holds 8160 FP64 / 16320 FP32 nonzeros: longer rows are cut
+    # into pieces whose partial sums are added in order; empty rows around them
+    rng = np.random.default_rng(7)
+    m = 70000
+    longs = [(0, 9000), (5, 8160), (6, 8161), (9, 40000), (40, 16321), (77, 20000)]
+    off, cols, vals = random_csr(80, m, 12, rng, long_rows=longs)
+    sizes = np.diff(off).copy()
+    sizes[[1, 2, 78, 79]] = 0  # empty rows after the first long row and after the last one
+    off, cols, vals = _resize_rows(off, cols, vals, sizes)
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    v = vals.astype(npd).astype(np.float64)
+    A = tnl.SparseMatrix.from_csr(80, m, off, cols, vals, value_dtype=dtype)
+    P = tnl.SpmvPlan(A, 0)
+    x = rng.standard_normal(m).astype(npd)
+    y0 = rng.standard_normal(80).astype(npd)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for alpha, beta, b, e in [(1, 0, 0, 0), (-2.0, 0.5, 0, 0), (0.75, -1.25, 6, 78)]:
+        want = orc.spmv(off, cols, v, x.astype(np.float64), y=y0.astype(np.float64), alpha=alpha, beta=beta,
+                        begin=b, end=e)
+        xd, yd = gpu(x).to(dtype), gpu(y0).to(dtype)
+        P.vectorProduct(xd, yd, alpha=alpha, beta=beta, begin=b, end=e)
+        torch.cuda.synchronize()
+        got = yd.cpu().numpy().astype(np.float64)
+        assert relerr(got, want) <= tol, (alpha, beta, b, e)
+        if b or e:
+            assert np.array_equal(got[:b], y0[:b]) and np.array_equal(got[e:], y0[e:])
+    # all rows empty
+    Z = tnl.SparseMatrix.from_csr(5, 9, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    yd = gpu(np.ones(5))
+    tnl.SpmvPlan(Z, 0).vectorProduct(gpu(np.ones(9)), yd, alpha=1.0, beta=0.5)
+    torch.cuda.synchronize()
+    assert np.array_equal(yd.cpu().numpy(), np.full(5, 0.5))
+
+
+def _resize_rows(off, cols, vals, sizes):
+    """The CSR arrays with row r cut to its first sizes[r] entries."""
+    keep = np.concatenate([np.arange(off[r], off[r] + sizes[r]) for r in range(len(sizes))]).astype(np.int64)
+    new = np.zeros(len(sizes) + 1, np.int32)
+    new[1:] = np.cumsum(sizes)
+    return new, cols[keep], vals[keep]

@@ -4,15 +4,18 @@
 // Random column indexes make every nonzero an 8-byte gather into x; through
 // the L2 each one costs a 128-byte line and an L2 request.  The plan splits
 // the columns into windows of kWin = 8192 entries (64 KiB of x, one LDS
-// allocation) and the rows into blocks of at most 8160 (FP64) / 16320 (FP32)
-// nonzeros:
+// allocation) and the nonzeros into row blocks of at most `cap` (FP64 8160,
+// FP32 16320): runs of whole consecutive rows, or one piece of a row longer
+// than cap (such a row is cut into pieces of cap nonzeros):
 //
 //   phase 1  one workgroup per window: x[window] -> LDS, then stream the
-//            window's nonzeros (values + 13-bit local columns, sorted by row)
+//            window's nonzeros (values + 13-bit local columns, in CSR order)
 //            and write each product v * x[c] to prod[] (window-major)
 //   phase 2  one workgroup per row block: copy the block's products from each
 //            window's contiguous segment into LDS at their CSR position
-//            (16-bit slot), then sum every row in CSR order
+//            (16-bit slot), then sum every row in CSR order (a piece: its
+//            partial sum)
+//   fixup    rows cut into pieces: the pieces' partial sums, added in order
 //
 // Per nonzero: 8 + 2 bytes read and 8 written in phase 1, 8 + 2 read in
 // phase 2 -- about 1.9 GB for the 67 M-nonzero power-law matrix, all of it
@@ -21,7 +24,7 @@
 // TNL's host loop does (ReducingOperations_CSR.h:46-74 with
 // SparseMatrixBase.hpp:331-389), so every row of at most kLongRow = 256
 // nonzeros, summed by one thread, matches it bit for bit; longer rows are
-// summed by a wave in a fixed (deterministic) order.
+// summed by a wave (pieces: by the workgroup) in a fixed, deterministic order.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -39,22 +42,13 @@ constexpr int kWin = 1 << kWinBits;  // x entries per window
 constexpr int kP1 = 1024;            // phase-1 threads
 constexpr int kP2 = 512;             // phase-2 threads
 constexpr int kLongRow = 256;        // longer rows are summed by a whole wave
-constexpr int kSegU = 24;            // phase-2 loads in flight per thread (and array padding)
+constexpr int kPad = 32;             // padding entries of prod and dest (phase-2 reads past a segment)
 
 // row-block capacity: the phase-2 LDS buffer within 64 KiB
 template< typename C >
 constexpr int cap_of()
 {
    return sizeof( C ) == 8 ? 8160 : 16320;
-}
-
-__global__ void w_erow( int64_t n, const int32_t* __restrict__ off, int32_t* __restrict__ erow )
-{
-   const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-   if( r >= n )
-      return;
-   for( int32_t e = off[ r ]; e < off[ r + 1 ]; e++ )
-      erow[ e ] = (int32_t) r;
 }
 
 // window of each nonzero (padding slots, column -1, go to an extra window K)
@@ -81,37 +75,40 @@ __global__ void w_starts( int64_t nnz, int32_t K, const int32_t* __restrict__ sk
       wstart[ k ] = p;
 }
 
-// the sorted arrays: value, local column, CSR slot within the row block, row
+// the row block holding nonzero e: the last b < B with E[b] <= e
+__device__ __forceinline__ int64_t block_of( int64_t B, const int64_t* __restrict__ E, int64_t e )
+{
+   int64_t lo = 0, hi = B;
+   while( hi - lo > 1 ) {
+      const int64_t mid = ( lo + hi ) / 2;
+      if( E[ mid ] <= e )
+         lo = mid;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+
+// the sorted arrays: value, local column, slot within the row block
 template< typename V >
 __global__ void w_fill( int64_t nnz, const int32_t* __restrict__ perm, const int32_t* __restrict__ cols,
-                        const V* __restrict__ vals, const int32_t* __restrict__ erow, const int32_t* __restrict__ rblk,
-                        const int64_t* __restrict__ eb, V* __restrict__ pv, uint16_t* __restrict__ pc,
-                        uint16_t* __restrict__ dest, int32_t* __restrict__ srow )
+                        const V* __restrict__ vals, int64_t B, const int64_t* __restrict__ E, V* __restrict__ pv,
+                        uint16_t* __restrict__ pc, uint16_t* __restrict__ dest )
 {
    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    if( i >= nnz )
       return;
    const int32_t e = perm[ i ];
-   const int32_t r = erow[ e ];
    const int32_t c = cols[ e ];
    pv[ i ] = vals[ e ];
    pc[ i ] = (uint16_t) ( c >= 0 ? ( c & ( kWin - 1 ) ) : 0 );
-   dest[ i ] = (uint16_t) ( e - eb[ rblk[ r ] ] );
-   srow[ i ] = r;
+   dest[ i ] = (uint16_t) ( e - E[ block_of( B, E, e ) ] );
 }
 
-__global__ void w_rblk( int64_t B, const int32_t* __restrict__ rb, int32_t* __restrict__ rblk )
-{
-   const int64_t b = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-   if( b >= B )
-      return;
-   for( int32_t r = rb[ b ]; r < rb[ b + 1 ]; r++ )
-      rblk[ r ] = (int32_t) b;
-}
-
-// seg[b * K + k] = first position of window k whose row is >= rb[b]
-__global__ void w_seg( int64_t B, int32_t K, const int32_t* __restrict__ rb, const int64_t* __restrict__ wstart,
-                       const int32_t* __restrict__ srow, int64_t* __restrict__ seg )
+// seg[b * K + k] = first position of window k holding a nonzero e >= E[b]
+// (a window's nonzeros are in CSR order: the sort is stable)
+__global__ void w_seg( int64_t B, int32_t K, const int64_t* __restrict__ E, const int64_t* __restrict__ wstart,
+                       const int32_t* __restrict__ perm, int32_t* __restrict__ seg )
 {
    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    if( q >= ( B + 1 ) * K )
@@ -119,15 +116,15 @@ __global__ void w_seg( int64_t B, int32_t K, const int32_t* __restrict__ rb, con
    const int64_t b = q / K;
    const int32_t k = (int32_t) ( q % K );
    int64_t lo = wstart[ k ], hi = wstart[ k + 1 ];
-   const int32_t row = rb[ b ];
+   const int64_t e = E[ b ];
    while( lo < hi ) {
       const int64_t mid = ( lo + hi ) / 2;
-      if( srow[ mid ] < row )
+      if( perm[ mid ] < e )
          lo = mid + 1;
       else
          hi = mid;
    }
-   seg[ q ] = lo;
+   seg[ q ] = (int32_t) lo;
 }
 
 // phase 1: prod[i] = v[i] * x[window base + pc[i]] for the window's nonzeros
@@ -160,25 +157,39 @@ __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t*
    }
 }
 
-// phase 2: row block b -> LDS in CSR order -> row sums -> y.  Every global
-// load whose address is known at entry (the first kSegU terms of the thread's
-// first kJ segments, the bounds of its first kJ rows) is issued before the
-// LDS buffer is cleared, so the block waits for about one memory round trip.
-constexpr int kJ = 1;
+// phase 2: row block b -> LDS in CSR order -> row sums -> y.  One workgroup
+// per row block.  The products of a block's segment (window k) are loaded by
+// a group of kGrp lanes together, kPer consecutive terms each: one load
+// instruction covers kGrp * kPer terms of a segment, so the lanes of a group
+// touch consecutive cache lines (a lane per segment would touch a line of its
+// own in every load, and the L1 tag lookups, one per line, would bound the
+// kernel).  All kRounds rounds of a thread (kP2 / kGrp groups, kGrp segments
+// each) are issued before the first is stored.
+constexpr int kGrp = 16;
+constexpr int kPer = 2;
+constexpr int kSpan = kGrp * kPer;  // terms of a segment per load round
+constexpr int kRounds = kGrp;       // segments per group: the group's own lanes hold their bounds
+
+// per-block header: 8 int32 -- rows [r0, r1), nonzeros [e0, e1), the long
+// row a piece belongs to (-1: a run of whole rows), long-row list [l0, l1)
+struct BlockHdr
+{
+   int32_t r0, r1, e0, e1, piece, l0, l1, pad;
+};
 
 template< typename C >
 __device__ __forceinline__ C sum_row( const C* buf, int32_t q0, int32_t q1 )
 {
-   // from zero, left to right; the LDS reads of eight terms ahead of their adds
+   // from zero, left to right; the LDS reads of four terms ahead of their adds
    C sum = 0;
    int32_t q = q0;
-   for( ; q + 8 <= q1; q += 8 ) {
-      C t[ 8 ];
+   for( ; q + 4 <= q1; q += 4 ) {
+      C t[ 4 ];
 #pragma unroll
-      for( int u = 0; u < 8; u++ )
+      for( int u = 0; u < 4; u++ )
          t[ u ] = buf[ q + u ];
 #pragma unroll
-      for( int u = 0; u < 8; u++ )
+      for( int u = 0; u < 4; u++ )
          sum += t[ u ];
    }
    for( ; q < q1; q++ )
@@ -186,122 +197,154 @@ __device__ __forceinline__ C sum_row( const C* buf, int32_t q0, int32_t q1 )
    return sum;
 }
 
-// the stores of one step of a segment copy: terms i0 + u < z
+// kPer terms at i (products and their 16-bit slots), loaded unconditionally:
+// prod and dest carry kSpan entries of padding, so a read past a segment's
+// end stays in bounds (and what it reads is not stored)
 template< typename C >
-__device__ __forceinline__ void put_terms( C* buf, const C ( &v )[ kSegU ], const uint32_t ( &d )[ kSegU ],
-                                           int64_t i0, int64_t z )
+__device__ __forceinline__ void get_pair( const C* prod, const uint16_t* dest, int32_t i, C ( &v )[ kPer ],
+                                          uint32_t& d )
 {
 #pragma unroll
-   for( int u = 0; u < kSegU; u++ )
-      if( i0 + u < z )
-         buf[ d[ u ] ] = v[ u ];
+   for( int u = 0; u < kPer; u++ )
+      v[ u ] = prod[ i + u ];
+   __builtin_memcpy( &d, dest + i, sizeof( d ) );
 }
 
-// one step of a segment copy: all kSegU loads are issued unconditionally
-// (prod and dest carry kSegU entries of padding, so reading past a segment
-// stays in bounds, and what is read there is not stored).  Ordinary cached
-// loads: a thread's loads of one segment share cache lines in L1, and
-// neighbouring row blocks' segments share lines in L2.
 template< typename C >
-__device__ __forceinline__ void get_terms( const C* prod, const uint16_t* dest, int64_t i0, C ( &v )[ kSegU ],
-                                           uint32_t ( &d )[ kSegU ] )
+__device__ __forceinline__ void put_pair( C* buf, const C ( &v )[ kPer ], uint32_t d, int32_t i, int32_t z )
 {
 #pragma unroll
-   for( int u = 0; u < kSegU; u++ ) {
-      v[ u ] = prod[ i0 + u ];
-      d[ u ] = dest[ i0 + u ];
-   }
+   for( int u = 0; u < kPer; u++ )
+      if( i + u < z )
+         buf[ ( d >> ( 16 * u ) ) & 0xffff ] = v[ u ];
 }
 
-template< typename C, int CAP >
+template< typename C, int CAP, bool BETA >
 __global__ void __launch_bounds__( kP2 )
-   w_phase2( int32_t K, const int64_t* __restrict__ seg, const C* __restrict__ prod,
-             const uint16_t* __restrict__ dest, const int32_t* __restrict__ rb, const int32_t* __restrict__ off,
-             const int32_t* __restrict__ lstart, const int32_t* __restrict__ lrows, C* __restrict__ y, C alpha,
-             C beta, int64_t begin, int64_t end )
+   w_phase2( int64_t B, int32_t K, const BlockHdr* __restrict__ hdr, const int32_t* __restrict__ seg,
+             const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ off,
+             const int32_t* __restrict__ lrows, C* __restrict__ part, C* __restrict__ y, C alpha, C beta,
+             int64_t begin, int64_t end )
 {
    __shared__ C buf[ CAP ];
+   __shared__ C red[ kP2 / 64 ];
    // consecutive row blocks on one XCD: their chunks of a window's products
    // (about 120 bytes each, 30 of slot indexes) share cache lines in its L2
    const int64_t b = xcd_contiguous( blockIdx.x, gridDim.x );
-   const int32_t r0 = rb[ b ], r1 = rb[ b + 1 ];
-   if( r1 <= begin || r0 >= end )
-      return;  // uniform: no row of this block is asked for
+   const BlockHdr H = hdr[ b ];  // uniform
+   if( ( H.piece < 0 ? H.r1 : H.r0 + 1 ) <= begin || H.r0 >= end )
+      return;  // no row of this block is asked for
    const int t = threadIdx.x;
-   int64_t a[ kJ ], z[ kJ ];
-   C v[ kJ ][ kSegU ];
-   uint32_t d[ kJ ][ kSegU ];
+   const int lane = t & 63, wid = wave_uniform( (int32_t) ( t >> 6 ) );
+   const int gl = lane % kGrp, gb = lane - gl;
+   // segment t (window t) of the block: its bounds, held by thread t
+   const int32_t kt = t < K ? t : K - 1;
+   const int32_t a = seg[ b * K + kt ];
+   const int32_t z = t < K ? seg[ ( b + 1 ) * K + kt ] : a;
+   // the rows of this thread (a block has at most 2 kP2) and of its wave's
+   // long row (at most one per wave); unconditional loads at clamped indexes
+   int32_t ob[ 2 ], oe[ 2 ];
+   const int32_t last = std::max( H.r0, H.r1 - 1 );
 #pragma unroll
-   for( int j = 0; j < kJ; j++ ) {
-      const int k = t + j * kP2;
-      a[ j ] = k < K ? seg[ b * K + k ] : 0;
-      z[ j ] = k < K ? seg[ ( b + 1 ) * K + k ] : 0;
+   for( int j = 0; j < 2; j++ ) {
+      const int32_t r = std::min( H.r0 + t + j * kP2, last );
+      ob[ j ] = off[ r ];
+      oe[ j ] = off[ r + 1 ];
    }
+   const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
+   const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
+   // round i: the group's segment gb + i (of this wave's 64), kSpan terms
+   C v[ kRounds ][ kPer ];
+   uint32_t d[ kRounds ];
 #pragma unroll
-   for( int j = 0; j < kJ; j++ )
-      get_terms( prod, dest, a[ j ], v[ j ], d[ j ] );
-   int32_t ob[ kJ ], oe[ kJ ];
-#pragma unroll
-   for( int j = 0; j < kJ; j++ ) {
-      const int32_t r = r0 + t + j * kP2;
-      ob[ j ] = r < r1 ? off[ r ] : 0;
-      oe[ j ] = r < r1 ? off[ r + 1 ] : 0;
+   for( int i = 0; i < kRounds; i++ ) {
+      const int32_t ai = __shfl( a, gb + i, 64 );
+      get_pair( prod, dest, ai + kPer * gl, v[ i ], d[ i ] );
    }
-   const int32_t e0 = off[ r0 ], e1 = off[ r1 ];
-   // padding slots (column -1) have no product: they read as zero
-   for( int32_t q = t; q < e1 - e0; q += kP2 )
+   // clear (padding slots, column -1, have no product: they read as zero)
+   const int32_t len = H.e1 - H.e0;
+   for( int32_t q = t; q < len; q += kP2 )
       buf[ q ] = (C) 0;
    __syncthreads();
 #pragma unroll
-   for( int j = 0; j < kJ; j++ )
-      put_terms( buf, v[ j ], d[ j ], a[ j ], z[ j ] );
-   // the rest: terms past the first kSegU of a segment, segments past the first kJ
-#pragma unroll
-   for( int j = 0; j < kJ; j++ )
-      for( int64_t i0 = a[ j ] + kSegU; i0 < z[ j ]; i0 += kSegU ) {
-         get_terms( prod, dest, i0, v[ 0 ], d[ 0 ] );
-         put_terms( buf, v[ 0 ], d[ 0 ], i0, z[ j ] );
+   for( int i = 0; i < kRounds; i++ ) {
+      const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
+      put_pair( buf, v[ i ], d[ i ], ai + kPer * gl, zi );
+      // on demand: the rest of a segment longer than kSpan
+      for( int32_t i0 = ai + kSpan + kPer * gl; i0 < zi; i0 += kSpan ) {
+         C w[ kPer ];
+         uint32_t dw;
+         get_pair( prod, dest, i0, w, dw );
+         put_pair( buf, w, dw, i0, zi );
       }
-   for( int k = t + kJ * kP2; k < K; k += kP2 ) {
-      const int64_t ka = seg[ b * K + k ], kz = seg[ ( b + 1 ) * K + k ];
-      for( int64_t i0 = ka; i0 < kz; i0 += kSegU ) {
-         get_terms( prod, dest, i0, v[ 0 ], d[ 0 ] );
-         put_terms( buf, v[ 0 ], d[ 0 ], i0, kz );
+   }
+   // on demand: windows past kP2 (segment k handled by the group of thread k % kP2)
+   for( int32_t k0 = kP2; k0 < K; k0 += kP2 ) {
+      const int32_t k = std::min( k0 + t, K - 1 );
+      const int32_t ak = seg[ b * K + k ], zk = k0 + t < K ? seg[ ( b + 1 ) * K + k ] : ak;
+      for( int i = 0; i < kRounds; i++ ) {
+         const int32_t ai = __shfl( ak, gb + i, 64 ), zi = __shfl( zk, gb + i, 64 );
+         for( int32_t i0 = ai + kPer * gl; i0 < zi; i0 += kSpan ) {
+            C w[ kPer ];
+            uint32_t dw;
+            get_pair( prod, dest, i0, w, dw );
+            put_pair( buf, w, dw, i0, zi );
+         }
       }
    }
    __syncthreads();
+   const int32_t e0 = H.e0;
    auto keep = [ & ]( int32_t r, C sum ) {
       if( r >= begin && r < end )
-         y[ r ] = beta == (C) 0 ? alpha * sum : alpha * sum + beta * y[ r ];
+         y[ r ] = BETA ? alpha * sum + beta * y[ r ] : alpha * sum;
    };
+   if( H.piece >= 0 ) {
+      // a piece of a row longer than cap: its partial sum (strided parts,
+      // then a fixed-order reduction), added to the other pieces' by w_fixup
+      C p = 0;
+      for( int32_t q = t; q < len; q += kP2 )
+         p += buf[ q ];
+      const C sum = block_sum< kP2 >( p, red );
+      if( t == 0 )
+         part[ b ] = sum;
+      return;
+   }
    // rows of at most kLongRow nonzeros: one thread each, summed from zero
    // left to right
 #pragma unroll
-   for( int j = 0; j < kJ; j++ ) {
-      const int32_t r = r0 + t + j * kP2;
-      if( r < r1 && oe[ j ] - ob[ j ] <= kLongRow )
+   for( int j = 0; j < 2; j++ ) {
+      const int32_t r = H.r0 + t + j * kP2;
+      if( r < H.r1 && oe[ j ] - ob[ j ] <= kLongRow )
          keep( r, sum_row( buf, ob[ j ] - e0, oe[ j ] - e0 ) );
    }
-   for( int32_t r = r0 + t + kJ * kP2; r < r1; r += kP2 ) {
-      const int32_t q0 = off[ r ] - e0, q1 = off[ r + 1 ] - e0;
-      if( q1 - q0 <= kLongRow )
-         keep( r, sum_row( buf, q0, q1 ) );
-   }
-   // longer rows (listed per block): one wave each, in turn over the waves;
-   // lane l sums terms l, l + 64, ... and an xor butterfly adds the 64 parts
-   // (a fixed order)
-   const int lane = t & 63, wid = wave_uniform( (int32_t) ( t >> 6 ) );
-   const int32_t l1 = lstart[ b + 1 ];
-   for( int32_t i = lstart[ b ] + wid; i < l1; i += kP2 / 64 ) {
-      const int32_t r = lrows[ i ];
-      const int32_t q1 = off[ r + 1 ] - e0;
-      C part = 0;
-      for( int32_t q = off[ r ] - e0 + lane; q < q1; q += 64 )
-         part += buf[ q ];
-      const C sum = group_sum< 64 >( part );
+   // the long row of this wave (if any): lane l sums terms l, l + 64, ... and
+   // an xor butterfly adds the 64 parts
+   if( H.l0 + wid < H.l1 ) {
+      C p = 0;
+      for( int32_t q = lq0 + lane; q < lq1; q += 64 )
+         p += buf[ q ];
+      const C sum = group_sum< 64 >( p );
       if( lane == 0 )
-         keep( r, sum );
+         keep( lr, sum );
    }
+}
+
+// rows cut into pieces: y[row] from the pieces' partial sums, in piece order
+// (huge holds (row, first piece block, pieces) per row)
+template< typename C >
+__global__ void w_fixup( int64_t H, const int32_t* __restrict__ huge, const C* __restrict__ part, C* __restrict__ y,
+                         C alpha, C beta, int64_t begin, int64_t end )
+{
+   const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( i >= H )
+      return;
+   const int32_t r = huge[ 3 * i ], first = huge[ 3 * i + 1 ], count = huge[ 3 * i + 2 ];
+   if( r < begin || r >= end )
+      return;
+   C sum = 0;
+   for( int32_t j = 0; j < count; j++ )
+      sum += part[ first + j ];
+   y[ r ] = beta == (C) 0 ? alpha * sum : alpha * sum + beta * y[ r ];
 }
 
 template< typename V >
@@ -313,53 +356,78 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    const int32_t* cols = A.column_indexes;
    const V* vals = (const V*) A.values;
    W.K = (int32_t) std::max< int64_t >( 1, ceil_div( A.columns, kWin ) );
-   // row blocks: consecutive rows with at most cap nonzeros together
+   // row blocks: runs of whole consecutive rows with at most cap nonzeros
+   // together, or pieces of cap nonzeros of a longer row
    std::vector< int32_t > hoff( n + 1 );
    TNLMI_HIP( hipMemcpyAsync( hoff.data(), off, sizeof( int32_t ) * ( n + 1 ), hipMemcpyDeviceToHost, s ) );
    TNLMI_HIP( hipStreamSynchronize( s ) );
    const int64_t cap = W.cap;
-   std::vector< int32_t > rb{ 0 };
-   std::vector< int64_t > eb{ 0 };
+   std::vector< int32_t > rb, piece, huge;
+   std::vector< int64_t > E;
    for( int64_t r = 0; r < n; ) {
-      const int64_t start = hoff[ r ];
-      int64_t e = r;
-      while( e < n && hoff[ e + 1 ] - start <= cap )
+      const int64_t start = hoff[ r ], len = hoff[ r + 1 ] - start;
+      if( len > cap ) {
+         const int64_t count = ceil_div( len, cap );
+         const int32_t h = (int32_t) ( huge.size() / 3 );
+         huge.insert( huge.end(), { (int32_t) r, (int32_t) rb.size(), (int32_t) count } );
+         for( int64_t j = 0; j < count; j++ ) {
+            rb.push_back( (int32_t) r );
+            E.push_back( start + j * cap );
+            piece.push_back( h );
+         }
+         r++;
+         continue;
+      }
+      // at most cap nonzeros, 2 kP2 rows and kP2 / 64 rows longer than kLongRow
+      int64_t e = r, longs = 0;
+      while( e < n && hoff[ e + 1 ] - start <= cap && e - r < 2 * kP2 ) {
+         const bool lg = hoff[ e + 1 ] - hoff[ e ] > kLongRow;
+         if( lg && longs == kP2 / 64 )
+            break;
+         longs += lg;
          e++;
-      if( e == r )
-         fail( TNLMI_NOT_IMPLEMENTED, "windowed SpMV plan: a row is longer than the row-block capacity" );
-      rb.push_back( (int32_t) e );
-      eb.push_back( hoff[ e ] );
+      }
+      rb.push_back( (int32_t) r );
+      E.push_back( start );
+      piece.push_back( -1 );
       r = e;
    }
-   W.B = (int64_t) rb.size() - 1;
-   // rows longer than kLongRow, by row block (summed by a wave in phase 2)
-   std::vector< int32_t > lstart{ 0 }, lrows;
+   rb.push_back( (int32_t) n );
+   E.push_back( nnz );
+   W.B = (int64_t) piece.size();
+   W.H = (int64_t) huge.size() / 3;
+   // block headers; rows longer than kLongRow inside runs of whole rows (summed
+   // by a wave) as (row, first slot, end slot)
+   std::vector< int32_t > hdr, lrows;
    for( int64_t i = 0; i < W.B; i++ ) {
-      for( int32_t r = rb[ i ]; r < rb[ i + 1 ]; r++ )
-         if( hoff[ r + 1 ] - hoff[ r ] > kLongRow )
-            lrows.push_back( r );
-      lstart.push_back( (int32_t) lrows.size() );
+      const int32_t l0 = (int32_t) ( lrows.size() / 3 );
+      if( piece[ i ] < 0 )
+         for( int32_t r = rb[ i ]; r < rb[ i + 1 ]; r++ )
+            if( hoff[ r + 1 ] - hoff[ r ] > kLongRow )
+               lrows.insert( lrows.end(), { r, (int32_t) ( hoff[ r ] - E[ i ] ), (int32_t) ( hoff[ r + 1 ] - E[ i ] ) } );
+      hdr.insert( hdr.end(), { rb[ i ], rb[ i + 1 ], (int32_t) E[ i ], (int32_t) E[ i + 1 ], piece[ i ], l0,
+                               (int32_t) ( lrows.size() / 3 ), 0 } );
    }
-   W.lstart.allocate( W.B + 1 );
-   W.lrows.allocate( std::max< int64_t >( 1, (int64_t) lrows.size() ) );
-   TNLMI_HIP( hipMemcpyAsync( W.lstart.ptr, lstart.data(), sizeof( int32_t ) * ( W.B + 1 ), hipMemcpyHostToDevice,
-                              s ) );
-   if( !lrows.empty() )
-      TNLMI_HIP( hipMemcpyAsync( W.lrows.ptr, lrows.data(), sizeof( int32_t ) * lrows.size(), hipMemcpyHostToDevice,
-                                 s ) );
-   W.rb.allocate( W.B + 1 );
-   DeviceArray< int64_t > deb;
-   deb.allocate( W.B + 1 );
-   TNLMI_HIP( hipMemcpyAsync( W.rb.ptr, rb.data(), sizeof( int32_t ) * ( W.B + 1 ), hipMemcpyHostToDevice, s ) );
-   TNLMI_HIP( hipMemcpyAsync( deb.ptr, eb.data(), sizeof( int64_t ) * ( W.B + 1 ), hipMemcpyHostToDevice, s ) );
-   // sort the nonzeros by window (stable: rows ascending inside a window)
-   DeviceArray< int32_t > erow, key, skey, ids, perm, rblk, srow;
-   for( auto* a : { &erow, &key, &skey, &ids, &perm, &srow } )
+   auto upload = [ & ]( auto& dst, const auto& src ) {
+      dst.allocate( std::max< int64_t >( 1, (int64_t) src.size() ) );
+      if( !src.empty() )
+         TNLMI_HIP( hipMemcpyAsync( dst.ptr, src.data(), sizeof( src[ 0 ] ) * src.size(), hipMemcpyHostToDevice,
+                                    s ) );
+   };
+   DeviceArray< int64_t > dE;  // build only
+   upload( dE, E );
+   upload( W.hdr, hdr );
+   upload( W.huge, huge );
+   if( lrows.empty() )
+      lrows.assign( 3, 0 );  // phase 2 reads one entry at a clamped index
+   upload( W.lrows, lrows );
+   W.part.allocate( std::max< int64_t >( 1, W.B ) * (int64_t) sizeof( V ) );
+   // sort the nonzeros by window (stable: CSR order inside a window)
+   DeviceArray< int32_t > key, skey, ids, perm;
+   for( auto* a : { &key, &skey, &ids, &perm } )
       a->allocate( std::max< int64_t >( 1, nnz ) );
-   rblk.allocate( std::max< int64_t >( 1, n ) );
    const dim3 b256( 256 );
    auto g = [ & ]( int64_t c ) { return dim3( (unsigned) std::max< int64_t >( 1, ceil_div( c, 256 ) ) ); };
-   hipLaunchKernelGGL( w_erow, g( n ), b256, 0, s, n, off, erow.ptr );
    hipLaunchKernelGGL( w_key, g( nnz ), b256, 0, s, nnz, cols, W.K, key.ptr, ids.ptr );
    int bits = 1;
    while( ( 1 << bits ) <= W.K + 1 )
@@ -373,18 +441,17 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
                                                   bits, s ) );
    W.wstart.allocate( W.K + 2 );
    hipLaunchKernelGGL( w_starts, g( nnz + 1 ), b256, 0, s, nnz, W.K, skey.ptr, W.wstart.ptr );
-   hipLaunchKernelGGL( w_rblk, g( W.B ), b256, 0, s, W.B, W.rb.ptr, rblk.ptr );
    W.pv.allocate( std::max< int64_t >( 1, nnz ) * (int64_t) sizeof( V ) );
    W.pc.allocate( std::max< int64_t >( 1, nnz ) );
-   W.dest.allocate( nnz + kSegU );
-   hipLaunchKernelGGL( ( w_fill< V > ), g( nnz ), b256, 0, s, nnz, perm.ptr, cols, vals, erow.ptr, rblk.ptr, deb.ptr,
-                       (V*) W.pv.ptr, W.pc.ptr, W.dest.ptr, srow.ptr );
+   W.dest.allocate( nnz + kPad );
+   hipLaunchKernelGGL( ( w_fill< V > ), g( nnz ), b256, 0, s, nnz, perm.ptr, cols, vals, W.B, dE.ptr,
+                       (V*) W.pv.ptr, W.pc.ptr, W.dest.ptr );
    W.seg.allocate( ( W.B + 1 ) * W.K );
-   hipLaunchKernelGGL( w_seg, g( ( W.B + 1 ) * W.K ), b256, 0, s, W.B, W.K, W.rb.ptr, W.wstart.ptr, srow.ptr,
+   hipLaunchKernelGGL( w_seg, g( ( W.B + 1 ) * W.K ), b256, 0, s, W.B, W.K, dE.ptr, W.wstart.ptr, perm.ptr,
                        W.seg.ptr );
-   W.prod.allocate( ( nnz + kSegU ) * (int64_t) sizeof( V ) );
-   TNLMI_HIP( hipMemsetAsync( W.dest.ptr + nnz, 0, sizeof( uint16_t ) * kSegU, s ) );
-   TNLMI_HIP( hipMemsetAsync( W.prod.ptr + nnz * (int64_t) sizeof( V ), 0, sizeof( V ) * kSegU, s ) );
+   W.prod.allocate( ( nnz + kPad ) * (int64_t) sizeof( V ) );
+   TNLMI_HIP( hipMemsetAsync( W.dest.ptr + nnz, 0, sizeof( uint16_t ) * kPad, s ) );
+   TNLMI_HIP( hipMemsetAsync( W.prod.ptr + nnz * (int64_t) sizeof( V ), 0, sizeof( V ) * kPad, s ) );
    TNLMI_HIP( hipGetLastError() );
    TNLMI_HIP( hipStreamSynchronize( s ) );
 }
@@ -397,9 +464,13 @@ void run( const WindowPlan& W, const C* x, C* y, double alpha, double beta, int6
       hipLaunchKernelGGL( ( w_phase1< C, C > ), dim3( (unsigned) W.K ), dim3( kP1 ), 0, s, W.columns, W.wstart.ptr,
                           (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
    if( W.B > 0 )
-      hipLaunchKernelGGL( ( w_phase2< C, cap_of< C >() > ), dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.K, W.seg.ptr,
-                          (const C*) W.prod.ptr, W.dest.ptr, W.rb.ptr, W.offs.ptr, W.lstart.ptr, W.lrows.ptr, y,
+      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false > : w_phase2< C, cap_of< C >(), true > ),
+                          dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr,
+                          W.seg.ptr, (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y,
                           (C) alpha, (C) beta, begin, end );
+   if( W.H > 0 )
+      hipLaunchKernelGGL( ( w_fixup< C > ), dim3( (unsigned) ceil_div( W.H, 256 ) ), dim3( 256 ), 0, s, W.H,
+                          W.huge.ptr, (const C*) W.part.ptr, y, (C) alpha, (C) beta, begin, end );
 }
 
 }  // namespace

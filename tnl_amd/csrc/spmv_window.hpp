@@ -10,14 +10,16 @@ struct WindowPlan
    int64_t rows = 0, columns = 0, nnz = 0;
    tnlmi_dtype value_type = TNLMI_F64;
    int32_t K = 0;    // column windows
-   int64_t B = 0;    // row blocks
-   int64_t cap = 0;  // nonzeros per row block at most
+   int64_t B = 0;     // row blocks
+   int64_t H = 0;     // rows longer than cap (cut into pieces)
+   int64_t cap = 0;   // nonzeros per row block at most
    DeviceArray< int64_t > wstart;  // K + 2: first sorted nonzero of each window (window K: padding slots)
-   DeviceArray< int64_t > seg;     // (B + 1) x K: first nonzero of row block b in window k
-   DeviceArray< int32_t > rb;      // B + 1: first row of each row block
+   DeviceArray< int32_t > seg;     // (B + 1) x K: first nonzero of row block b in window k
    DeviceArray< int32_t > offs;    // rows + 1: the CSR offsets
-   DeviceArray< int32_t > lstart;  // B + 1: first entry of each row block in lrows
-   DeviceArray< int32_t > lrows;   // rows longer than the one-thread limit, by row block
+   DeviceArray< int32_t > hdr;     // 8 B: rows, nonzeros, piece and long-row range of each row block
+   DeviceArray< int32_t > lrows;   // 3 per row longer than the one-thread limit: row, first and end slot
+   DeviceArray< int32_t > huge;    // 3 H: (row, first piece block, pieces) of each row longer than cap
+   DeviceArray< char > part;       // B: partial sums of the piece blocks (compute type)
    DeviceArray< char > pv;         // nonzeros sorted by window (rows ascending inside), value type
    DeviceArray< uint16_t > pc;     // their column inside the window
    DeviceArray< uint16_t > dest;   // their CSR slot inside their row block
