@@ -294,6 +294,54 @@ def test_reliable_mixed_cg(orc, vt, fmt, param):
     assert 2 <= s.getUpdates() <= 40
 
 
+@pytest.mark.parametrize("fmt,param", [(tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32)])
+def test_mixed_cg_column_codes_identical(monkeypatch, fmt, param):
+    # the inner SpMV of an Ellpack / SlicedEllpack stencil matrix reads 8-bit
+    # column codes (column = row + dictionary[code]); the columns it decodes
+    # are the stored ones, so the solve is bit-identical to int32 columns
+    off, cols, vals = tnl.poisson3d(20, 18, 16)
+    n = len(off) - 1
+    b = np.random.default_rng(23).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+    Alo = tnl.SparseMatrix(A.rows, A.columns, A.segments, A.values.to(torch.bfloat16), A.column_indexes)
+    out = []
+    for codes in ("1", "0"):
+        monkeypatch.setenv("TNLMI_COLUMN_CODES", codes)
+        s = S.ReliableMixedCG()
+        s.setMatrix(A)
+        s.setInnerMatrix(Alo)
+        s.setConvergenceResidue(1e-10)
+        s.setMaxIterations(3000)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        assert s.solve(dev(b), x)
+        out.append((x.cpu().numpy(), s.getIterations(), s.getUpdates()))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1:] == out[1][1:]
+
+
+def test_mixed_cg_many_offsets_keeps_int32_columns(orc):
+    # more than 255 distinct offsets column - row: no codes, the int32 kernel
+    import scipy.sparse as sp
+    n = 3000
+    R = sp.random(n, n, density=4.0 / n, random_state=np.random.RandomState(3), format="csr")
+    R.data[:] = -np.abs(R.data)
+    M = (R + R.T).tocsr()
+    M = (M + sp.diags(np.asarray(abs(M).sum(axis=1)).ravel() + 1.0)).tocsr()
+    M.sort_indices()
+    assert len(np.unique(M.nonzero()[1] - M.nonzero()[0])) > 255
+    off, cols, vals = M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=tnl.ELLPACK)
+    Alo = tnl.SparseMatrix(A.rows, A.columns, A.segments, A.values.to(torch.float32), A.column_indexes)
+    b = np.random.default_rng(24).standard_normal(n)
+    s = S.ReliableMixedCG()
+    s.setMatrix(A)
+    s.setInnerMatrix(Alo)
+    s.setConvergenceResidue(1e-10)
+    s.setMaxIterations(3000)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-10
+
+
 def test_reliable_mixed_cg_rounded_bf16(orc):
     off, cols, vals = tnl.laplace2d(40, 40)
     vals = vals * 1.0137  # not BF16 numbers: the inner matrix differs from A

@@ -28,8 +28,11 @@
 // (LinearResidueGetter.hpp:11-23).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 #include "blas1.hpp"
 #include "krylov.hpp"
@@ -176,6 +179,201 @@ __global__ void __launch_bounds__( kFT ) spmv_dot_ell( SegView v, const int32_t*
    const double bs = block_sum< kFT >( acc, sm );
    if( threadIdx.x < 64 )
       part[ blk ] = bs;
+}
+
+// ---- 8-bit column codes: for a matrix whose entries take few distinct
+// offsets column - row (a stencil: 7 for the 3D 7-point Laplacian), every
+// slot stores a code into a dictionary of at most kCodes offsets instead of
+// its 32-bit column (code kPadCode: padding), so the SpMV reads 1 byte of
+// index per entry instead of 4.  The columns decoded are exactly the stored
+// ones: results are bit-identical to the int32 kernel above.
+constexpr int kCodes = 255;
+constexpr uint8_t kPadCode = 255;
+constexpr int kHash = 4096;  // open-addressing table of the distinct offsets
+constexpr uint64_t kEmpty = ~0ull;
+
+template< int R, typename V, typename T >
+__global__ void __launch_bounds__( kFT ) spmv_dot_ell_code( SegView v, const uint8_t* __restrict__ codes,
+                                                            const int32_t* __restrict__ dict, int32_t ndict,
+                                                            const V* __restrict__ vals, const T* __restrict__ p,
+                                                            T* __restrict__ Ap, double* __restrict__ part,
+                                                            const CgDev* __restrict__ st )
+{
+   using CB = typename Vec< uint8_t, R >::type;
+   using VV = ValVec< V, R >;
+   __shared__ double sm[ kFT / 64 ];
+   __shared__ int32_t sd[ kCodes ];
+   if( ! st->active )
+      return;
+   for( int i = threadIdx.x; i < ndict; i += kFT )
+      sd[ i ] = dict[ i ];
+   __syncthreads();
+   const int64_t blk = xcd_contiguous( blockIdx.x, gridDim.x );
+   const int64_t r0 = ( blk * kFT + threadIdx.x ) * R;
+   double acc = 0;
+   if( r0 < v.n ) {
+      int64_t g, stride;
+      int32_t w;
+      if( v.format == TNLMI_ELLPACK ) {
+         g = r0;
+         stride = v.ell_aligned;
+         w = v.ell_width;
+      }
+      else {
+         const int64_t sl = r0 / v.slice_size;
+         g = v.slice_offsets[ sl ] + ( r0 - sl * v.slice_size );
+         stride = v.slice_size;
+         w = v.slice_seg_sizes[ sl ];
+      }
+      // kKU entries at a time: their codes and values, then their reads of p
+      // (one contiguous read when the R rows share a code, as in the interior
+      // of a stencil: the columns r0 + j + offset are consecutive), then the
+      // products, added in entry order as the int32 kernel adds them
+      constexpr int kKU = 8;
+      T y[ R ] = {};
+      for( int32_t k0 = 0; k0 < w; k0 += kKU ) {
+         CB c[ kKU ];
+         typename VV::type a[ kKU ];
+#pragma unroll
+         for( int u = 0; u < kKU; u++ ) {
+            if( k0 + u < w ) {
+               c[ u ] = __builtin_nontemporal_load( (const CB*) ( codes + g + u * stride ) );
+               a[ u ] = VV::load( vals + g + u * stride );
+            }
+            else
+#pragma unroll
+               for( int j = 0; j < R; j++ )
+                  c[ u ][ j ] = kPadCode;
+         }
+         T pv[ kKU ][ R ];
+#pragma unroll
+         for( int u = 0; u < kKU; u++ ) {
+            const uint32_t c0 = c[ u ][ 0 ];
+            bool same = true;
+#pragma unroll
+            for( int j = 1; j < R; j++ )
+               same = same && c[ u ][ j ] == c0;
+            if( same && c0 != kPadCode )
+               __builtin_memcpy( pv[ u ], p + r0 + sd[ c0 ], sizeof( pv[ u ] ) );
+            else
+#pragma unroll
+               for( int j = 0; j < R; j++ )
+                  pv[ u ][ j ] = c[ u ][ j ] != kPadCode ? p[ r0 + j + sd[ c[ u ][ j ] ] ] : (T) 0;
+         }
+#pragma unroll
+         for( int u = 0; u < kKU; u++ )
+#pragma unroll
+            for( int j = 0; j < R; j++ )
+               if( c[ u ][ j ] != kPadCode )
+                  y[ j ] += VV::template get< T >( a[ u ], j ) * pv[ u ][ j ];
+         g += kKU * stride;
+      }
+#pragma unroll
+      for( int j = 0; j < R; j++ )
+         if( r0 + j < v.n ) {
+            Ap[ r0 + j ] = y[ j ];
+            acc += (double) y[ j ] * (double) p[ r0 + j ];
+         }
+   }
+   const double bs = block_sum< kFT >( acc, sm );
+   if( threadIdx.x < 64 )
+      part[ blk ] = bs;
+}
+
+__device__ __forceinline__ uint32_t offset_hash( uint32_t key )
+{
+   return ( key * 2654435761u ) >> 20;  // Knuth's multiplicative hash, 12 bits
+}
+
+// the slot of row r, entry k, and the row width (Ellpack / SlicedEllpack, column-major)
+__device__ __forceinline__ int64_t ell_slot( const SegView& v, int64_t r, int32_t k, int32_t* w )
+{
+   if( v.format == TNLMI_ELLPACK ) {
+      *w = v.ell_width;
+      return r + (int64_t) k * v.ell_aligned;
+   }
+   const int64_t sl = r / v.slice_size;
+   *w = v.slice_seg_sizes[ sl ];
+   return v.slice_offsets[ sl ] + ( r - sl * v.slice_size ) + (int64_t) k * v.slice_size;
+}
+
+// the distinct offsets column - row into the table (a few recent ones kept in
+// registers; a plain read before the atomic); more than kCodes, or a full
+// probe sequence, sets *overflow
+__global__ void __launch_bounds__( kFT ) code_collect( SegView v, const int32_t* __restrict__ cols,
+                                                       unsigned long long* table, int32_t* count, int32_t* overflow )
+{
+   uint32_t recent[ 4 ] = { 0, 0, 0, 0 };
+   bool have[ 4 ] = { false, false, false, false };
+   int next = 0;
+   for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < v.n; r += (int64_t) gridDim.x * blockDim.x ) {
+      int32_t w = 0;
+      ell_slot( v, r, 0, &w );
+      for( int32_t k = 0; k < w; k++ ) {
+         const int32_t c = cols[ ell_slot( v, r, k, &w ) ];
+         if( c < 0 )
+            continue;
+         const uint32_t key = (uint32_t) ( c - (int32_t) r );
+         bool seen = false;
+#pragma unroll
+         for( int i = 0; i < 4; i++ )
+            seen |= have[ i ] && recent[ i ] == key;
+         if( seen )
+            continue;
+         if( *(volatile int32_t*) overflow )
+            return;
+         uint32_t h = offset_hash( key );
+         bool done = false;
+         for( int probe = 0; probe < kHash && ! done; probe++, h = ( h + 1 ) & ( kHash - 1 ) ) {
+            const unsigned long long cur = ( (volatile unsigned long long*) table )[ h ];
+            if( cur == (unsigned long long) key ) {
+               done = true;
+               break;
+            }
+            if( cur != kEmpty )
+               continue;
+            const unsigned long long prev = atomicCAS( table + h, kEmpty, (unsigned long long) key );
+            if( prev == kEmpty ) {
+               if( atomicAdd( count, 1 ) >= kCodes )
+                  atomicExch( overflow, 1 );
+               done = true;
+            }
+            else if( prev == (unsigned long long) key )
+               done = true;
+         }
+         if( ! done ) {
+            atomicExch( overflow, 1 );
+            return;
+         }
+         recent[ next ] = key;
+         have[ next ] = true;
+         next = ( next + 1 ) & 3;
+      }
+   }
+}
+
+// codes[slot] = code of the slot's offset (slots of padding keep kPadCode)
+__global__ void __launch_bounds__( kFT ) code_encode( SegView v, const int32_t* __restrict__ cols,
+                                                      const unsigned long long* __restrict__ table,
+                                                      const uint8_t* __restrict__ code_of, uint8_t* __restrict__ codes )
+{
+   for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < v.n; r += (int64_t) gridDim.x * blockDim.x ) {
+      int32_t w = 0;
+      ell_slot( v, r, 0, &w );
+      for( int32_t k = 0; k < w; k++ ) {
+         const int64_t g = ell_slot( v, r, k, &w );
+         const int32_t c = cols[ g ];
+         if( c < 0 )
+            continue;
+         const uint32_t key = (uint32_t) ( c - (int32_t) r );
+         uint32_t h = offset_hash( key );
+         for( int probe = 0; probe < kHash; probe++, h = ( h + 1 ) & ( kHash - 1 ) )
+            if( table[ h ] == (unsigned long long) key ) {
+               codes[ g ] = code_of[ h ];
+               break;
+            }
+      }
+   }
 }
 
 // The SpMV kernels' nin block partial sums folded, in order, into the kFB
@@ -480,9 +678,69 @@ struct Stage
    int64_t grid = 0;                      // blocks of the fused SpMV kernel
    int rows_per_thread = 1;               // Ellpack / SlicedEllpack kernel
    std::shared_ptr< DeviceArray< double > > part;  // its per-block partial sums
+   // kind 1 with 8-bit column codes (the inner matrix of the mixed solver)
+   std::shared_ptr< DeviceArray< uint8_t > > codes;
+   std::shared_ptr< DeviceArray< int32_t > > dict;
+   int32_t ndict = 0;
 };
 
-Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_t s )
+// The column codes of a column-major Ellpack / SlicedEllpack matrix, or
+// nothing (more than kCodes distinct offsets).  Set TNLMI_COLUMN_CODES=0 in
+// the environment to keep the int32 columns (identical results, more bytes).
+void make_codes( Stage& S, hipStream_t s )
+{
+   const char* env = std::getenv( "TNLMI_COLUMN_CODES" );
+   if( env && std::string( env ) == "0" )
+      return;
+   const tnlmi_segments& seg = *S.A->segments;
+   const int64_t storage = seg.storage;
+   DeviceArray< unsigned long long > table;
+   DeviceArray< int32_t > flags;  // count, overflow
+   table.allocate( kHash );
+   flags.allocate( 2 );
+   TNLMI_HIP( hipMemsetAsync( table.ptr, 0xff, sizeof( unsigned long long ) * kHash, s ) );
+   TNLMI_HIP( hipMemsetAsync( flags.ptr, 0, sizeof( int32_t ) * 2, s ) );
+   const unsigned grid = (unsigned) std::min< int64_t >( 4096, std::max< int64_t >( 1, ceil_div( S.v.n, kFT ) ) );
+   hipLaunchKernelGGL( code_collect, dim3( grid ), dim3( kFT ), 0, s, S.v, S.A->column_indexes, table.ptr, flags.ptr,
+                       flags.ptr + 1 );
+   TNLMI_HIP( hipGetLastError() );
+   std::vector< unsigned long long > ht( kHash );
+   int32_t hf[ 2 ];
+   TNLMI_HIP( hipMemcpyAsync( ht.data(), table.ptr, sizeof( unsigned long long ) * kHash, hipMemcpyDeviceToHost, s ) );
+   TNLMI_HIP( hipMemcpyAsync( hf, flags.ptr, sizeof( hf ), hipMemcpyDeviceToHost, s ) );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+   if( hf[ 1 ] != 0 || hf[ 0 ] > kCodes )
+      return;
+   std::vector< int32_t > keys;
+   for( const auto k : ht )
+      if( k != kEmpty )
+         keys.push_back( (int32_t) (uint32_t) k );
+   std::sort( keys.begin(), keys.end() );
+   std::vector< uint8_t > code_of( kHash, kPadCode );
+   for( int h = 0; h < kHash; h++ )
+      if( ht[ h ] != kEmpty )
+         code_of[ h ] = (uint8_t) ( std::lower_bound( keys.begin(), keys.end(), (int32_t) (uint32_t) ht[ h ] )
+                                    - keys.begin() );
+   DeviceArray< uint8_t > dcode_of;
+   dcode_of.allocate( kHash );
+   S.dict = std::make_shared< DeviceArray< int32_t > >();
+   S.dict->allocate( std::max< int64_t >( 1, (int64_t) keys.size() ) );
+   S.codes = std::make_shared< DeviceArray< uint8_t > >();
+   // R bytes are read per row group: room for the last group's overhang
+   S.codes->allocate( storage + 16 );
+   TNLMI_HIP( hipMemcpyAsync( dcode_of.ptr, code_of.data(), kHash, hipMemcpyHostToDevice, s ) );
+   if( ! keys.empty() )
+      TNLMI_HIP( hipMemcpyAsync( S.dict->ptr, keys.data(), sizeof( int32_t ) * keys.size(), hipMemcpyHostToDevice,
+                                 s ) );
+   TNLMI_HIP( hipMemsetAsync( S.codes->ptr, kPadCode, storage + 16, s ) );
+   hipLaunchKernelGGL( code_encode, dim3( grid ), dim3( kFT ), 0, s, S.v, S.A->column_indexes, table.ptr, dcode_of.ptr,
+                       S.codes->ptr );
+   TNLMI_HIP( hipGetLastError() );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+   S.ndict = (int32_t) keys.size();
+}
+
+Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_t s, bool codes = false )
 {
    Stage S;
    S.A = A;
@@ -500,6 +758,9 @@ Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_
       S.grid = ceil_div( ceil_div( S.v.n, S.rows_per_thread ), kFT );
       S.part = std::make_shared< DeviceArray< double > >();
       S.part->allocate( S.grid );
+      if( codes ) {
+         make_codes( S, s );
+      }
       return S;
    }
    return S;
@@ -510,7 +771,20 @@ void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDe
 {
    const V* vals = (const V*) S.A->values;
    double* bp = S.part->ptr;
-   if( S.kind == 1 ) {
+   if( S.kind == 1 && S.codes ) {
+      const uint8_t* c = S.codes->ptr;
+      const int32_t* d = S.dict->ptr;
+      if( S.rows_per_thread == 4 )
+         hipLaunchKernelGGL( ( spmv_dot_ell_code< 4, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v, c, d,
+                             S.ndict, vals, p, Ap, bp, st );
+      else if( S.rows_per_thread == 2 )
+         hipLaunchKernelGGL( ( spmv_dot_ell_code< 2, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v, c, d,
+                             S.ndict, vals, p, Ap, bp, st );
+      else
+         hipLaunchKernelGGL( ( spmv_dot_ell_code< 1, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v, c, d,
+                             S.ndict, vals, p, Ap, bp, st );
+   }
+   else if( S.kind == 1 ) {
       if( S.rows_per_thread == 4 )
          hipLaunchKernelGGL( ( spmv_dot_ell< 4, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
                              S.A->column_indexes, vals, p, Ap, bp, st );
@@ -710,7 +984,7 @@ tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_
             *updates = 0;
          return;
       }
-      const Stage S = make_stage( A_lo, nullptr, s );
+      const Stage S = make_stage( A_lo, nullptr, s, true );
       const StopRule R{ st->max_iterations, st->min_iterations, st->convergence_residue, st->divergence_residue };
       int64_t it = 0;
       double res = 0;
