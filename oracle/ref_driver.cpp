@@ -7,6 +7,8 @@
  * serves as the measured CPU baseline ("kind": "reference").  It contains no
  * reference source: it only instantiates the reference's templates.
  */
+#include <omp.h>
+
 #include <TNL/Matrices/SparseMatrix.h>
 #include <TNL/Algorithms/Segments/CSR.h>
 #include <TNL/Algorithms/Segments/AdaptiveCSR.h>
@@ -471,6 +473,12 @@ void ref_ilu0_apply( int64_t n, const int32_t* rp, const int32_t* c, const doubl
    std::copy( b, b + n, bv.getData() );
    p.solve( bv.getConstView(), xv.getView() );
    std::copy( xv.getData(), xv.getData() + n, x );
+}
+
+// OpenMP threads of TNL's Host algorithms in the following calls
+void ref_set_threads( int threads )
+{
+   omp_set_num_threads( threads );
 }
 
 double ref_true_residual( int64_t n, const int32_t* rp, const int32_t* c, const double* v, const double* x,

@@ -334,6 +334,10 @@ class Reference:
     def jacobi(self, off, cols, vals, b, x0=None, omega=1.0, period=4, **kw):
         return self._solve(self.L.ref_jacobi, off, cols, vals, b, x0, ctypes.c_double(omega), I64(period), **kw)
 
+    def set_threads(self, threads):
+        """OpenMP threads of TNL's Host algorithms in later calls."""
+        self.L.ref_set_threads(ctypes.c_int(int(threads)))
+
     def ilu0_apply(self, off, cols, vals, b):
         off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
         x = np.zeros(len(b))

@@ -220,8 +220,12 @@ def test_cg_vs_reference(orc, ref):
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_gmres_vs_reference(orc, ref, variant):
+    # the reference runs single-threaded: TNL's Host GMRES-CWY with several
+    # OpenMP threads returned a wrong x (reporting convergence) in about 1 of
+    # 180 solves of these systems here, and never with one thread
     off, cols, vals = nonsym(300, np.random.default_rng(7))
     b = np.random.default_rng(8).standard_normal(300)
+    ref.set_threads(1)
     for pre, rmin, rmax in [(0, 10, 10), (1, 30, 30), (2, 5, 20)]:
         xo, so = orc.gmres(off, cols, vals, b, precond=pre, variant=variant, rmin=rmin, rmax=rmax, tol=1e-10)
         xr, sr = ref.gmres(off, cols, vals, b, precond=pre, variant=variant, rmin=rmin, rmax=rmax, tol=1e-10)
