@@ -147,6 +147,39 @@ tnlmi_status tnlmi_matrix_fill( const tnlmi_segments* segments, const int32_t* d
                                 void* d_values, tnlmi_dtype value_type, int32_t* d_column_indexes,
                                 tnlmi_stream stream );
 
+/* ------------------------------------------------- cross-format copy */
+/* copySparseMatrix( dst, src ) on the device (Matrices/SparseOperations.hpp:
+ * 600-690): tnlmi_matrix_row_lengths gives the target's row capacities -- the
+ * leading entries of each source row whose column is below the column count
+ * (padding slots, column -1, are counted, as the reference counts them) --
+ * synchronously into d_lengths (rows int32).  The caller builds the target's
+ * segments from them (tnlmi_segments_create) and allocates its storage;
+ * tnlmi_matrix_copy clears that storage (value 0, column -1) and copies
+ * element k of every row (column and value) through both formats'
+ * getGlobalIndex.  Same value type and dimensions (TNLMI_INVALID_ARGUMENT
+ * otherwise, and for a row length above a segment size).  Synchronous. */
+tnlmi_status tnlmi_matrix_row_lengths( const tnlmi_matrix* A, int32_t* d_lengths, tnlmi_stream stream );
+tnlmi_status tnlmi_matrix_copy( const tnlmi_matrix* src, tnlmi_matrix* dst, const int32_t* d_lengths,
+                                tnlmi_stream stream );
+
+/* ---------------------------------------------------- MatrixMarket input */
+/* A MatrixMarket coordinate file (real / integer / pattern, general /
+ * symmetric) read into a host CSR matrix: rows ascending, columns ascending
+ * in each row, a repeated position keeps the last value, symmetric files are
+ * mirrored, pattern entries are 1.  Replaces MatrixReader< Matrix, Host >::
+ * readMtx (Matrices/MatrixReader.hpp:52-66, 158-306) followed by
+ * SparseMatrix::setElements of its map; the same header and line checks and
+ * messages (TNLMI_RUNTIME_ERROR; indexes below 1 are rejected as well).
+ * tnlmi_mtx_parse reads the file's text from memory (the istream overload,
+ * MatrixReader.hpp:26-32).  The arrays returned by tnlmi_host_csr_get stay
+ * valid until tnlmi_host_csr_destroy. */
+typedef struct tnlmi_host_csr tnlmi_host_csr;
+tnlmi_status tnlmi_mtx_read( const char* path, tnlmi_host_csr** out );
+tnlmi_status tnlmi_mtx_parse( const char* text, int64_t length, tnlmi_host_csr** out );
+tnlmi_status tnlmi_host_csr_get( const tnlmi_host_csr* m, int64_t* rows, int64_t* columns, int64_t* nnz,
+                                 const int32_t** offsets, const int32_t** columns_out, const double** values );
+void tnlmi_host_csr_destroy( tnlmi_host_csr* m );
+
 /* Segments/LaunchConfiguration.h: ThreadsToSegmentsMapping (same values) plus
  * TNLMI_MAP_AUTO, the MI355X-tuned default (tiled nnz-balanced CSR). */
 typedef enum {

@@ -23,11 +23,13 @@
 #include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
 #include <TNL/Solvers/Linear/Preconditioners/ILU0.h>
 #include <TNL/Solvers/Linear/Utils/LinearResidueGetter.h>
+#include <TNL/Matrices/MatrixReader.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 using namespace TNL;
@@ -473,6 +475,47 @@ void ref_ilu0_apply( int64_t n, const int32_t* rp, const int32_t* c, const doubl
    std::copy( b, b + n, bv.getData() );
    p.solve( bv.getConstView(), xv.getView() );
    std::copy( xv.getData(), xv.getData() + n, x );
+}
+
+// MatrixReader< CSR Host matrix >::readMtx; the result is kept for
+// ref_read_mtx_arrays.  Returns 0, or -1 with the exception's message in
+// ref_last_error().
+static std::vector< int > g_off, g_col;
+static std::vector< double > g_val;
+static std::string g_err;
+
+int ref_read_mtx( const char* path, int64_t* rows, int64_t* cols, int64_t* nnz )
+{
+   try {
+      CsrMat A;
+      Matrices::MatrixReader< CsrMat >::readMtx( std::string( path ), A );
+      const auto off = A.getSegments().getOffsets();
+      g_off.assign( off.getData(), off.getData() + off.getSize() );
+      const auto ci = A.getColumnIndexes();
+      const auto va = A.getValues();
+      g_col.assign( ci.getData(), ci.getData() + ci.getSize() );
+      g_val.assign( va.getData(), va.getData() + va.getSize() );
+      *rows = A.getRows();
+      *cols = A.getColumns();
+      *nnz = (int64_t) g_col.size();
+      return 0;
+   }
+   catch( const std::exception& e ) {
+      g_err = e.what();
+      return -1;
+   }
+}
+
+void ref_read_mtx_arrays( int32_t* off, int32_t* col, double* val )
+{
+   std::copy( g_off.begin(), g_off.end(), off );
+   std::copy( g_col.begin(), g_col.end(), col );
+   std::copy( g_val.begin(), g_val.end(), val );
+}
+
+const char* ref_last_error()
+{
+   return g_err.c_str();
 }
 
 // OpenMP threads of TNL's Host algorithms in the following calls

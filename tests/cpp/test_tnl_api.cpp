@@ -3,6 +3,8 @@
 // Devices::Host (src/UnitTests/Matrices/SparseMatrixVectorProductTest.hpp,
 // Documentation/Examples/Solvers/Linear/IterativeLinearSolverExample.cpp).
 // Prints "ALL OK" and exits 0 when every check passes.
+#include <sstream>
+
 #include <TNL/MI355X.h>
 
 #include <cmath>
@@ -172,6 +174,57 @@ int main()
       std::cout << "GMRES(MGSR): " << g.getIterations() << " iterations\n";
    }
    solveExample< Solvers::Linear::Jacobi< Matrix > >( A, "Jacobi" );
+
+   // copySparseMatrix into BiEllpack and ChunkedEllpack (column-major) and back to CSR
+   {
+      using BMatrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Seg::BiEllpack >;
+      using CMatrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Seg::ChunkedEllpack >;
+      BMatrix Bm;
+      CMatrix Cm;
+      Matrix Am;
+      Matrices::copySparseMatrix( Bm, *A );
+      Matrices::copySparseMatrix( Cm, Bm );
+      Matrices::copySparseMatrix( Am, Cm );
+      Vector x( 5 ), y( 5 ), z( 5 ), w( 5 );
+      for( int i = 0; i < 5; i++ )
+         x.setElement( i, 1.0 + i );
+      A->vectorProduct( x, y );
+      Cm.vectorProduct( x, z );
+      Am.vectorProduct( x, w );
+      EXPECT_EQ( y == z, true );
+      EXPECT_EQ( y == w, true );
+      EXPECT_EQ( Cm.getElement( 2, 3 ), -1.0 );
+      EXPECT_EQ( Am.getRowCapacity( 0 ), 2 );
+      std::cout << "copySparseMatrix CSR -> BiEllpack -> ChunkedEllpack -> CSR done\n";
+   }
+
+   // MatrixReader::readMtx: the same matrix as a symmetric MatrixMarket file
+   // (lower triangle, one repeated entry whose last value counts), into
+   // SlicedEllpack, then the CG example on it
+   {
+      using SMatrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Seg::SlicedEllpack >;
+      std::stringstream mtx( "%%MatrixMarket matrix coordinate real symmetric\n% tridiag(-1, 2.5, -1)\n5 5 10\n"
+                             "1 1 2.5\n2 1 -1\n2 2 2.5\n3 2 -1\n3 3 9\n3 3 2.5\n4 3 -1\n4 4 2.5\n5 4 -1\n"
+                             "5 5 2.5\n" );
+      auto B = std::make_shared< SMatrix >();
+      Matrices::MatrixReader< SMatrix >::readMtx( mtx, *B );
+      EXPECT_EQ( B->getRows(), 5 );
+      EXPECT_EQ( B->getColumns(), 5 );
+      Vector x( 5, 1.0 ), y( 5 ), z( 5 );
+      B->vectorProduct( x, y );
+      A->vectorProduct( x, z );
+      EXPECT_EQ( y == z, true );
+      solveExample< Solvers::Linear::CG< SMatrix > >( B, "CG on readMtx (SlicedEllpack)" );
+      bool thrown = false;
+      try {
+         std::stringstream bad( "%%MatrixMarket matrix coordinate complex general\n2 2 1\n1 1 1 0\n" );
+         Matrices::MatrixReader< SMatrix >::readMtx( bad, *B );
+      }
+      catch( const std::runtime_error& ) {
+         thrown = true;
+      }
+      EXPECT_EQ( thrown, true );
+   }
 
    // error behaviour of the reference
    {

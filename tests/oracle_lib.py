@@ -334,6 +334,19 @@ class Reference:
     def jacobi(self, off, cols, vals, b, x0=None, omega=1.0, period=4, **kw):
         return self._solve(self.L.ref_jacobi, off, cols, vals, b, x0, ctypes.c_double(omega), I64(period), **kw)
 
+    def read_mtx(self, path):
+        """MatrixReader< CSR Host >::readMtx: (rows, columns, offsets, columns, values);
+        RuntimeError with the reference's message on a bad file."""
+        r, c, z = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        if self.L.ref_read_mtx(os.fsencode(path), ctypes.byref(r), ctypes.byref(c), ctypes.byref(z)) != 0:
+            self.L.ref_last_error.restype = ctypes.c_char_p
+            raise RuntimeError(self.L.ref_last_error().decode())
+        off = np.zeros(r.value + 1, np.int32)
+        col = np.zeros(z.value, np.int32)
+        val = np.zeros(z.value)
+        self.L.ref_read_mtx_arrays(P(off), P(col), P(val))
+        return r.value, c.value, off, col, val
+
     def set_threads(self, threads):
         """OpenMP threads of TNL's Host algorithms in later calls."""
         self.L.ref_set_threads(ctypes.c_int(int(threads)))

@@ -15,6 +15,7 @@ runs in the library's own HIP kernels.  Errors map to the reference's
 exception types (see ``_lib``).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -193,6 +194,37 @@ class SparseMatrix:
                                                 dtype_code(vdt), ctypes.c_void_p(colidx.data_ptr()), _stream()))
         return cls(rows, columns, seg, values, colidx)
 
+    def rowLengths(self):
+        """The leading entries of each row with a column below getColumns() (padding
+        slots, column -1, included): copySparseMatrix's row capacities
+        (tnlmi_matrix_row_lengths), an int32 device tensor."""
+        torch = _torch()
+        out = torch.empty(max(1, self.rows), dtype=torch.int32, device="cuda")
+        _lib.check(_lib.lib().tnlmi_matrix_row_lengths(ctypes.byref(self._m), ctypes.c_void_p(out.data_ptr()),
+                                                       _stream()))
+        return out[:self.rows]
+
+    def convert(self, fmt, param=0, organization=COLUMN_MAJOR):
+        """copySparseMatrix into a new matrix of another format (Matrices/SparseOperations.hpp:
+        600-690): same rows, values and columns, on the GPU (tnlmi_matrix_copy)."""
+        torch = _torch()
+        lengths = self.rowLengths()
+        seg = Segments(fmt, lengths, organization, param)
+        storage = max(1, seg.getStorageSize())
+        values = torch.empty(storage, dtype=self.values.dtype, device="cuda")
+        colidx = torch.empty(storage, dtype=torch.int32, device="cuda")
+        B = SparseMatrix(self.rows, self.columns, seg, values, colidx)
+        _lib.check(_lib.lib().tnlmi_matrix_copy(ctypes.byref(self._m), ctypes.byref(B._m),
+                                                ctypes.c_void_p(lengths.data_ptr()), _stream()))
+        return B
+
+    @classmethod
+    def from_mtx(cls, path, fmt=CSR, value_dtype=None, organization=COLUMN_MAJOR, param=0):
+        """MatrixReader::readMtx into this format (tnlmi_mtx_read, then assembly on the GPU)."""
+        rows, columns, off, cols, vals = read_mtx(path)
+        return cls.from_csr(rows, columns, off, cols, vals, fmt=fmt, value_dtype=value_dtype,
+                            organization=organization, param=param)
+
     def getRows(self):
         return self.rows
 
@@ -257,6 +289,42 @@ class SpmvPlan:
 # ------------------------------------------------------------------ generators
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def _host_csr(h):
+    """(rows, columns, offsets, columns, values) numpy copies of a tnlmi_host_csr, destroyed."""
+    import numpy as np
+    try:
+        r, c, z = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        po, pc, pv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(_lib.lib().tnlmi_host_csr_get(h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), ctypes.byref(po),
+                                                 ctypes.byref(pc), ctypes.byref(pv)))
+        n, nnz = r.value, z.value
+
+        def arr(p, count, ct, dt):
+            if count == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(count,)).copy()
+        return (n, c.value, arr(po, n + 1, ctypes.c_int32, np.int32), arr(pc, nnz, ctypes.c_int32, np.int32),
+                arr(pv, nnz, ctypes.c_double, np.float64))
+    finally:
+        _lib.lib().tnlmi_host_csr_destroy(h)
+
+
+def read_mtx(path):
+    """MatrixReader::readMtx (MatrixMarket coordinate file) as host CSR arrays:
+    (rows, columns, offsets, columns, values)."""
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().tnlmi_mtx_read(os.fsencode(path), ctypes.byref(h)))
+    return _host_csr(h)
+
+
+def parse_mtx(text):
+    """read_mtx of a MatrixMarket file's text (str or bytes)."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().tnlmi_mtx_parse(data, len(data), ctypes.byref(h)))
+    return _host_csr(h)
 
 
 def laplace2d(nx, ny):

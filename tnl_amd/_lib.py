@@ -164,6 +164,13 @@ _PROTOS = {
     "tnlmi_batched_cg_destroy": (None, [c_vp]),
     "tnlmi_batched_cg_solve": (ctypes.c_int, [c_vp, ctypes.c_int32, c_vp, c_vp, ctypes.c_double, c_i64,
                                               ctypes.c_double, c_vp, c_vp, ctypes.c_int32, c_vp]),
+    "tnlmi_matrix_row_lengths": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp]),
+    "tnlmi_matrix_copy": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp]),
+    "tnlmi_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
+    "tnlmi_mtx_parse": (ctypes.c_int, [ctypes.c_char_p, c_i64, ctypes.POINTER(c_vp)]),
+    "tnlmi_host_csr_get": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                                          ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "tnlmi_host_csr_destroy": (None, [c_vp]),
     "tnlmi_gen_batched_diffusion": (None, [c_i64, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_vp, c_vp, c_vp,
                                            c_vp, c_vp, c_vp]),
     "tnlmi_gen_convdiff": (ctypes.c_double, [c_i64, ctypes.c_uint64, c_vp, c_vp, c_vp]),

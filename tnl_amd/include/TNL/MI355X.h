@@ -20,6 +20,8 @@
 
 #include <cmath>
 #include <initializer_list>
+#include <iostream>
+#include <iterator>
 #include <limits>
 #include <map>
 #include <memory>
@@ -413,6 +415,38 @@ public:
    SparseMatrix& operator=( const SparseMatrix& ) = delete;
    ~SparseMatrix() { release(); }
 
+   //! copySparseMatrix( *this, B ) (Matrices/SparseOperations.hpp:600-690) from
+   //! a matrix of any format on the device: row capacities = B's row lengths
+   //! (tnlmi_matrix_row_lengths), then element k of every row copied
+   //! (tnlmi_matrix_copy)
+   template< typename Matrix2 >
+   void copyFrom( const Matrix2& B )
+   {
+      static_assert( std::is_same_v< Real, typename Matrix2::RealType >, "The matrices must have the same RealType." );
+      static_assert( std::is_same_v< typename Matrix2::DeviceType, Devices::MI355X >,
+                     "The matrices must be allocated on the same device." );
+      release();
+      rows_ = B.getRows();
+      columns_ = B.getColumns();
+      const tnlmi_matrix src = B.abi();
+      int* d_len = nullptr;
+      detail::hip( hipMalloc( (void**) &d_len, std::max( 1, rows_ ) * sizeof( int ) ) );
+      std::unique_ptr< int, void ( * )( int* ) > own( d_len, []( int* p ) { (void) hipFree( p ); } );
+      detail::check( tnlmi_matrix_row_lengths( &src, d_len, nullptr ) );
+      detail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, d_len, rows_, nullptr,
+                                            &segments_ ) );
+      const int64_t storage = std::max< int64_t >( 1, tnlmi_segments_storage_size( segments_ ) );
+      detail::hip( hipMalloc( (void**) &values_, storage * sizeof( Real ) ) );
+      detail::hip( hipMalloc( (void**) &columnIndexes_, storage * sizeof( int ) ) );
+      tnlmi_matrix dst = abi();
+      detail::check( tnlmi_matrix_copy( &src, &dst, d_len, nullptr ) );
+      hoff_ = B.hoff_;  // the host mirror in CSR order is the same
+      hcols_ = B.hcols_;
+      hvals_ = B.hvals_;
+   }
+   template< typename, typename, typename, typename, template< typename, typename, typename > class, typename >
+   friend class SparseMatrix;
+
    void setDimensions( int rows, int columns )
    {
       if( rows < 0 || columns < 0 )
@@ -565,6 +599,58 @@ private:
    int* columnIndexes_ = nullptr;
    std::vector< int > hoff_, hcols_;
    std::vector< Real > hvals_;
+};
+
+//! copySparseMatrix (Matrices/SparseOperations.hpp:716-721) between MI355X
+//! matrices of any formats
+template< typename Matrix1, typename Matrix2 >
+void copySparseMatrix( Matrix1& A, const Matrix2& B )
+{
+   A.copyFrom( B );
+}
+
+template< typename Matrix, typename Device = typename Matrix::DeviceType >
+class MatrixReader;
+
+//! MatrixReader::readMtx (Matrices/MatrixReader.hpp:16-32): a MatrixMarket
+//! coordinate file (real / integer / pattern, general / symmetric) into the
+//! matrix -- the reference reads into a host matrix and copies it; here the
+//! library parses (tnlmi_mtx_read) and the matrix is assembled on the GPU
+//! from the CSR arrays, with the same dimensions, entries and exceptions
+//! (std::runtime_error).
+template< typename Matrix >
+class MatrixReader< Matrix, Devices::MI355X >
+{
+public:
+   static void readMtx( const std::string& fileName, Matrix& matrix, bool verbose = false )
+   {
+      tnlmi_host_csr* h = nullptr;
+      detail::check( tnlmi_mtx_read( fileName.c_str(), &h ) );
+      load( h, matrix, verbose );
+   }
+   static void readMtx( std::istream& str, Matrix& matrix, bool verbose = false )
+   {
+      const std::string text( ( std::istreambuf_iterator< char >( str ) ), std::istreambuf_iterator< char >() );
+      tnlmi_host_csr* h = nullptr;
+      detail::check( tnlmi_mtx_parse( text.data(), (int64_t) text.size(), &h ) );
+      load( h, matrix, verbose );
+   }
+
+private:
+   static void load( tnlmi_host_csr* h, Matrix& matrix, bool verbose )
+   {
+      std::unique_ptr< tnlmi_host_csr, void ( * )( tnlmi_host_csr* ) > own( h, tnlmi_host_csr_destroy );
+      int64_t rows = 0, columns = 0, nnz = 0;
+      const int32_t *off = nullptr, *cols = nullptr;
+      const double* vals = nullptr;
+      detail::check( tnlmi_host_csr_get( h, &rows, &columns, &nnz, &off, &cols, &vals ) );
+      if( verbose )
+         std::cout << "Matrix dimensions are " << rows << " x " << columns << '\n';
+      matrix.setDimensions( (int) rows, (int) columns );
+      using Real = typename Matrix::RealType;
+      matrix.setFromCSR( std::vector< int >( off, off + rows + 1 ), std::vector< int >( cols, cols + nnz ),
+                         std::vector< Real >( vals, vals + nnz ) );
+   }
 };
 
 }  // namespace Matrices
