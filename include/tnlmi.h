@@ -278,6 +278,18 @@ void tnlmi_preconditioner_destroy( tnlmi_preconditioner* P );
 tnlmi_status tnlmi_preconditioner_solve( const tnlmi_preconditioner* P, const double* d_b, double* d_x,
                                          tnlmi_stream stream );
 
+/* BiCGStab (Solvers/Linear/BICGStab.hpp:33-127; exact_residue != 0: the
+ * bicgstab-exact-residue option, BICGStab.hpp:14-31) and TFQMR
+ * (Solvers/Linear/TFQMR.hpp:12-121), left-preconditioned by M (NULL: none),
+ * with the reference's residues (||r|| / ||M^-1 b||, the exact preconditioned
+ * residual, TFQMR's quasi-residual tau sqrt(it + 1) / ||M^-1 b||), its r*
+ * restart and its stopping rule (IterativeSolver.hpp:98-143).  Optional plan
+ * of A.  Synchronous. */
+tnlmi_status tnlmi_bicgstab_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                                   int32_t exact_residue, const double* d_b, double* d_x, tnlmi_solver_state* state,
+                                   tnlmi_stream stream );
+tnlmi_status tnlmi_tfqmr_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                                const double* d_b, double* d_x, tnlmi_solver_state* state, tnlmi_stream stream );
 /* LinearSolver<Matrix>::solve(b, x) -> bool (synchronous; FP64 vectors; any
  * matrix format and value type, FP64 accumulation).  plan: optional
  * column-split plan of A used for its products (NULL: A's own kernels).

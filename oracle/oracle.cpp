@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <vector>
 #ifdef _OPENMP
@@ -1198,3 +1199,167 @@ extern "C" void orc_gmres( int64_t n, const int32_t* row_ptr, const int32_t* col
    const bool conv = g.solve( b, x );
    g.S.finish( conv );
 }
+
+/* ------------------------------------------------------ BiCGStab / TFQMR */
+namespace {
+
+// M^{-1} A src (PreconditionedMatvec of BICGStab.hpp:147-157 / TFQMR.hpp:46-53)
+void pmv( const Csr& A, const Precond& M, int precond, const double* src, double* dst, std::vector< double >& tmp )
+{
+   if( precond ) {
+      A.mv( src, tmp.data() );
+      M.solve( tmp.data(), dst, A.n );
+   }
+   else
+      A.mv( src, dst );
+}
+
+// r = M^{-1} (b - A x) (BICGStab.hpp:131-145)
+void pres( const Csr& A, const Precond& M, int precond, const double* x, const double* b, double* r,
+           std::vector< double >& tmp )
+{
+   const int64_t n = A.n;
+   A.mv( x, tmp.data() );
+   for( int64_t i = 0; i < n; i++ )
+      tmp[ i ] = b[ i ] - tmp[ i ];
+   if( precond )
+      M.solve( tmp.data(), r, n );
+   else
+      std::memcpy( r, tmp.data(), sizeof( double ) * n );
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_bicgstab( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                   double* x, int precond, int exact_residue, orc_solver_state* st )
+{
+   // BICGStab.hpp:33-127
+   Csr A{ n, row_ptr, cols, vals };
+   Precond M;
+   M.kind = precond;
+   M.update( A );
+   std::vector< double > r( n ), r_ast( n ), p( n ), s( n ), Ap( n ), As( n ), tmp( n );
+   Iter S{ st };
+   double b_norm;
+   if( precond ) {
+      M.solve( b, tmp.data(), n );
+      b_norm = nrm2( tmp.data(), n );
+   }
+   else
+      b_norm = nrm2( b, n );
+   if( b_norm == 0 ) {
+      std::fill( x, x + n, 0.0 );
+      S.finish( true );
+      return;
+   }
+   pres( A, M, precond, x, b, r.data(), tmp );
+   p = r_ast = r;
+   std::fill( s.begin(), s.end(), 0.0 );
+   double rho = dot( r.data(), r_ast.data(), n ), r_ast_sqnorm = rho;
+   const double eps2 = std::numeric_limits< double >::epsilon() * std::numeric_limits< double >::epsilon();
+   S.res = std::sqrt( rho ) / b_norm;
+   while( S.next() ) {
+      pmv( A, M, precond, p.data(), Ap.data(), tmp );
+      const double alpha = rho / dot( Ap.data(), r_ast.data(), n );
+      for( int64_t i = 0; i < n; i++ )
+         s[ i ] = r[ i ] - alpha * Ap[ i ];
+      pmv( A, M, precond, s.data(), As.data(), tmp );
+      const double omega = dot( As.data(), s.data(), n ) / dot( As.data(), As.data(), n );
+      for( int64_t i = 0; i < n; i++ )
+         x[ i ] += alpha * p[ i ] + omega * s[ i ];
+      for( int64_t i = 0; i < n; i++ )
+         r[ i ] = s[ i ] - omega * As[ i ];
+      const double rho_old = rho;
+      rho = dot( r.data(), r_ast.data(), n );
+      if( std::abs( rho ) < eps2 * r_ast_sqnorm ) {
+         pres( A, M, precond, x, b, r.data(), tmp );
+         r_ast = r;
+         r_ast_sqnorm = rho = dot( r.data(), r_ast.data(), n );
+      }
+      const double beta = ( rho / rho_old ) * ( alpha / omega );
+      for( int64_t i = 0; i < n; i++ )
+         p[ i ] = r[ i ] + beta * p[ i ] - ( beta * omega ) * Ap[ i ];
+      if( exact_residue ) {
+         pres( A, M, precond, x, b, s.data(), tmp );
+         S.res = nrm2( s.data(), n ) / b_norm;
+      }
+      else
+         S.res = nrm2( r.data(), n ) / b_norm;
+   }
+   S.finish( S.converged() );
+}
+
+void orc_tfqmr( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                double* x, int precond, orc_solver_state* st )
+{
+   // TFQMR.hpp:12-121
+   Csr A{ n, row_ptr, cols, vals };
+   Precond M;
+   M.kind = precond;
+   M.update( A );
+   std::vector< double > r( n ), r_ast( n ), w( n ), u( n ), v( n ), d( n ), Au( n ), tmp( n );
+   Iter S{ st };
+   double b_norm;
+   if( precond ) {
+      M.solve( b, tmp.data(), n );
+      b_norm = nrm2( tmp.data(), n );
+   }
+   else
+      b_norm = nrm2( b, n );
+   pres( A, M, precond, x, b, r.data(), tmp );
+   if( b_norm == 0 ) {
+      std::fill( x, x + n, 0.0 );
+      S.finish( true );
+      return;
+   }
+   w = u = r;
+   pmv( A, M, precond, u.data(), Au.data(), tmp );
+   v = Au;
+   std::fill( d.begin(), d.end(), 0.0 );
+   double tau = nrm2( r.data(), n ), theta = 0.0, eta = 0.0, alpha = 0.0;
+   r_ast = r;
+   double rho = dot( r_ast.data(), r.data(), n );
+   S.res = tau / b_norm;
+   while( S.next() ) {
+      const int64_t iter = S.it;
+      if( iter % 2 == 1 )
+         alpha = rho / dot( v.data(), r_ast.data(), n );
+      else
+         pmv( A, M, precond, u.data(), Au.data(), tmp );
+      for( int64_t i = 0; i < n; i++ )
+         w[ i ] -= alpha * Au[ i ];
+      const double dc = theta * theta * eta / alpha;
+      for( int64_t i = 0; i < n; i++ )
+         d[ i ] = u[ i ] + dc * d[ i ];
+      const double w_norm = nrm2( w.data(), n );
+      theta = w_norm / tau;
+      const double c = 1.0 / std::sqrt( 1.0 + theta * theta );
+      tau = tau * theta * c;
+      eta = c * c * alpha;
+      for( int64_t i = 0; i < n; i++ )
+         x[ i ] += eta * d[ i ];
+      S.res = tau * std::sqrt( (double) ( iter + 1 ) ) / b_norm;
+      if( iter > st->min_iterations && S.res < st->convergence_residue )
+         break;
+      if( iter % 2 == 0 ) {
+         const double rho_new = dot( w.data(), r_ast.data(), n );
+         const double beta = rho_new / rho;
+         rho = rho_new;
+         for( int64_t i = 0; i < n; i++ ) {
+            u[ i ] = w[ i ] + beta * u[ i ];
+            v[ i ] = beta * Au[ i ] + ( beta * beta ) * v[ i ];
+         }
+         pmv( A, M, precond, u.data(), Au.data(), tmp );
+         for( int64_t i = 0; i < n; i++ )
+            v[ i ] += Au[ i ];
+      }
+      else
+         for( int64_t i = 0; i < n; i++ )
+            u[ i ] -= alpha * v[ i ];
+   }
+   S.finish( S.converged() );
+}
+
+}  // extern "C"

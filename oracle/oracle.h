@@ -134,6 +134,12 @@ void orc_cg( int64_t n, const int32_t* row_ptr, const int32_t* cols, const doubl
 void orc_gmres( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
                 double* x, int precond, int variant, int restart_min, int restart_max, int step_min,
                 int step_max, orc_solver_state* st );
+/* BICGStab.hpp:33-127 (exact_residue: the bicgstab-exact-residue option) */
+void orc_bicgstab( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                   double* x, int precond, int exact_residue, orc_solver_state* st );
+/* TFQMR.hpp:12-121 */
+void orc_tfqmr( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                double* x, int precond, orc_solver_state* st );
 /* Jacobi: Solvers/Linear/Jacobi.hpp:65-125. */
 void orc_jacobi( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
                  double* x, double omega, int64_t residue_period, orc_solver_state* st );

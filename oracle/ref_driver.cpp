@@ -19,6 +19,8 @@
 #include <TNL/Algorithms/Segments/SortedSegments.h>
 #include <TNL/Solvers/Linear/CG.h>
 #include <TNL/Solvers/Linear/GMRES.h>
+#include <TNL/Solvers/Linear/BICGStab.h>
+#include <TNL/Solvers/Linear/TFQMR.h>
 #include <TNL/Solvers/Linear/Jacobi.h>
 #include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
 #include <TNL/Solvers/Linear/Preconditioners/ILU0.h>
@@ -453,6 +455,27 @@ void ref_gmres( int64_t n, const int32_t* rp, const int32_t* c, const double* v,
    pc.addParameter< int >( "gmres-restarting-step-min", dmin );
    pc.addParameter< int >( "gmres-restarting-step-max", dmax );
    s.setup( pc );
+   run_solver( s, A, precond, b, x, n, st );
+}
+
+void ref_bicgstab( int64_t n, const int32_t* rp, const int32_t* c, const double* v, const double* b, double* x,
+                   int precond, int exact_residue, State* st )
+{
+   auto A = make_csr( n, rp, c, v );
+   Solvers::Linear::BICGStab< CsrMat > s;
+   Config::ConfigDescription cd;
+   Solvers::Linear::BICGStab< CsrMat >::configSetup( cd );
+   Config::ParameterContainer pc;
+   pc.addParameter< bool >( "bicgstab-exact-residue", exact_residue != 0 );
+   s.setup( pc );
+   run_solver( s, A, precond, b, x, n, st );
+}
+
+void ref_tfqmr( int64_t n, const int32_t* rp, const int32_t* c, const double* v, const double* b, double* x,
+                int precond, State* st )
+{
+   auto A = make_csr( n, rp, c, v );
+   Solvers::Linear::TFQMR< CsrMat > s;
    run_solver( s, A, precond, b, x, n, st );
 }
 

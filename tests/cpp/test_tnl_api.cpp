@@ -174,6 +174,10 @@ int main()
       std::cout << "GMRES(MGSR): " << g.getIterations() << " iterations\n";
    }
    solveExample< Solvers::Linear::Jacobi< Matrix > >( A, "Jacobi" );
+   solveExample< Solvers::Linear::BICGStab< Matrix > >( A, "BICGStab" );
+   solveExample< Solvers::Linear::BICGStab< Matrix > >( A, "BICGStab + Diagonal", true );
+   solveExample< Solvers::Linear::TFQMR< Matrix > >( A, "TFQMR" );
+   solveExample< Solvers::Linear::TFQMR< Matrix > >( A, "TFQMR + Diagonal", true );
 
    // copySparseMatrix into BiEllpack and ChunkedEllpack (column-major) and back to CSR
    {

@@ -168,6 +168,22 @@ class Oracle:
         self.L.orc_cg(I64(n), P(off), P(cols), P(vals), P(b), P(x), precond, ctypes.byref(st))
         return x, st
 
+    def bicgstab(self, off, cols, vals, b, x0=None, precond=0, exact=False, **kw):
+        off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
+        n = len(b)
+        x = np.zeros(n) if x0 is None else f64(x0).copy()
+        st = state(**kw)
+        self.L.orc_bicgstab(I64(n), P(off), P(cols), P(vals), P(b), P(x), precond, int(exact), ctypes.byref(st))
+        return x, st
+
+    def tfqmr(self, off, cols, vals, b, x0=None, precond=0, **kw):
+        off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
+        n = len(b)
+        x = np.zeros(n) if x0 is None else f64(x0).copy()
+        st = state(**kw)
+        self.L.orc_tfqmr(I64(n), P(off), P(cols), P(vals), P(b), P(x), precond, ctypes.byref(st))
+        return x, st
+
     def gmres(self, off, cols, vals, b, x0=None, precond=0, variant=3, rmin=10, rmax=10, dmin=3, dmax=3, **kw):
         off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
         n = len(b)
@@ -330,6 +346,12 @@ class Reference:
 
     def gmres(self, off, cols, vals, b, x0=None, precond=0, variant=3, rmin=10, rmax=10, dmin=3, dmax=3, **kw):
         return self._solve(self.L.ref_gmres, off, cols, vals, b, x0, precond, variant, rmin, rmax, dmin, dmax, **kw)
+
+    def bicgstab(self, off, cols, vals, b, x0=None, precond=0, exact=False, **kw):
+        return self._solve(self.L.ref_bicgstab, off, cols, vals, b, x0, precond, int(exact), **kw)
+
+    def tfqmr(self, off, cols, vals, b, x0=None, precond=0, **kw):
+        return self._solve(self.L.ref_tfqmr, off, cols, vals, b, x0, precond, **kw)
 
     def jacobi(self, off, cols, vals, b, x0=None, omega=1.0, period=4, **kw):
         return self._solve(self.L.ref_jacobi, off, cols, vals, b, x0, ctypes.c_double(omega), I64(period), **kw)

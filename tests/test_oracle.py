@@ -234,6 +234,57 @@ def test_gmres_vs_reference(orc, ref, variant):
         np.testing.assert_allclose(xo, xr, rtol=1e-7, atol=1e-9)
 
 
+def _dense_solve(off, cols, vals, b):
+    n = len(off) - 1
+    A = np.zeros((n, n))
+    for i in range(n):
+        for e in range(off[i], off[i + 1]):
+            if cols[e] >= 0:
+                A[i, cols[e]] += vals[e]
+    return np.linalg.solve(A, b)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_bicgstab_vs_reference(orc, ref, exact):
+    off, cols, vals = nonsym(300, np.random.default_rng(9))
+    b = np.random.default_rng(10).standard_normal(300)
+    xd = _dense_solve(off, cols, vals, b)
+    for pre in (0, 1, 2):
+        xo, so = orc.bicgstab(off, cols, vals, b, precond=pre, exact=exact, tol=1e-10)
+        xr, sr = ref.bicgstab(off, cols, vals, b, precond=pre, exact=exact, tol=1e-10)
+        assert so.converged == sr.converged == 1
+        assert abs(so.iterations - sr.iterations) <= 1, (pre, so.iterations, sr.iterations)
+        np.testing.assert_allclose(xo, xr, rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(xo, xd, rtol=1e-7, atol=1e-8)
+    # iteration limits as IterativeSolver counts them
+    for maxit in (3, 7):
+        xo, so = orc.bicgstab(off, cols, vals, b, tol=1e-14, maxit=maxit)
+        xr, sr = ref.bicgstab(off, cols, vals, b, tol=1e-14, maxit=maxit)
+        assert so.iterations == sr.iterations and so.converged == sr.converged == 0
+        np.testing.assert_allclose(xo, xr, rtol=1e-9, atol=1e-12)
+
+
+def test_tfqmr_vs_reference(orc, ref):
+    off, cols, vals = nonsym(300, np.random.default_rng(11))
+    b = np.random.default_rng(12).standard_normal(300)
+    xd = _dense_solve(off, cols, vals, b)
+    for pre in (0, 1, 2):
+        xo, so = orc.tfqmr(off, cols, vals, b, precond=pre, tol=1e-10)
+        xr, sr = ref.tfqmr(off, cols, vals, b, precond=pre, tol=1e-10)
+        assert so.converged == sr.converged == 1
+        assert abs(so.iterations - sr.iterations) <= 1, (pre, so.iterations, sr.iterations)
+        np.testing.assert_allclose(xo, xr, rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(xo, xd, rtol=1e-7, atol=1e-8)
+        if pre == 0:
+            # the quasi-residual bound ||b - A x_m|| <= sqrt(m + 1) tau_m (the reported residue)
+            assert orc.true_residual(off, cols, vals, xo, b) <= so.residue * (1 + 1e-6)
+    for maxit, minit in ((4, 0), (9, 0), (12, 10)):
+        xo, so = orc.tfqmr(off, cols, vals, b, tol=1e-3, maxit=maxit, minit=minit)
+        xr, sr = ref.tfqmr(off, cols, vals, b, tol=1e-3, maxit=maxit, minit=minit)
+        assert so.iterations == sr.iterations and so.converged == sr.converged, (maxit, minit)
+        np.testing.assert_allclose(xo, xr, rtol=1e-9, atol=1e-12)
+
+
 def test_jacobi_vs_reference(orc, ref):
     off, cols, vals = nonsym(200, np.random.default_rng(3))
     b = np.random.default_rng(4).standard_normal(200)

@@ -406,3 +406,73 @@ def test_fused_cg_ragged_rows(orc, fmt, param):
     x.zero_()
     assert m.solve(dev(b), x)
     assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-10
+
+
+# ------------------------------------------------------- BiCGStab / TFQMR
+@pytest.mark.parametrize("solver", ["bicgstab", "bicgstab_exact", "tfqmr"])
+@pytest.mark.parametrize("pre", [0, 1, 2], ids=["none", "diag", "ilu0"])
+@pytest.mark.parametrize("fmt", [tnl.CSR, tnl.ELLPACK], ids=["csr", "ell"])
+def test_bicgstab_tfqmr_match_oracle(orc, solver, pre, fmt):
+    # BICGStab.hpp:33-127 / TFQMR.hpp:12-121 on the GPU: the oracle's iterations
+    # (+-1: reduction order) and solution, and a true residual at the tolerance
+    off, cols, vals = nonsym(600, 21)
+    n = len(off) - 1
+    b = np.random.default_rng(22).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt)
+    if solver == "tfqmr":
+        s = S.TFQMR()
+        xo, so = orc.tfqmr(off, cols, vals, b, precond=pre, tol=1e-10)
+    else:
+        s = S.BICGStab()
+        s.setup(exact_residue=solver == "bicgstab_exact")
+        xo, so = orc.bicgstab(off, cols, vals, b, precond=pre, exact=solver == "bicgstab_exact", tol=1e-10)
+    s.setMatrix(A)
+    if pre:
+        s.setPreconditioner(S.Diagonal(A) if pre == 1 else S.ILU0(A))
+    s.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x) and so.converged
+    assert abs(s.getIterations() - so.iterations) <= 1, (s.getIterations(), so.iterations)
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-8)
+    assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) < 1e-8
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "tfqmr"])
+def test_bicgstab_tfqmr_limits_and_zero_rhs(orc, solver):
+    off, cols, vals = nonsym(300, 23)
+    n = len(off) - 1
+    b = np.random.default_rng(24).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    for maxit, minit in ((3, 0), (8, 0), (12, 10)):
+        s = S.TFQMR() if solver == "tfqmr" else S.BICGStab()
+        s.setMatrix(A)
+        s.setConvergenceResidue(1e-3 if minit else 1e-14)
+        s.setMaxIterations(maxit)
+        s.setMinIterations(minit)
+        fn = orc.tfqmr if solver == "tfqmr" else orc.bicgstab
+        xo, so = fn(off, cols, vals, b, tol=1e-3 if minit else 1e-14, maxit=maxit, minit=minit)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ok = s.solve(dev(b), x)
+        assert ok == bool(so.converged) and s.getIterations() == so.iterations, (maxit, minit)
+        np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-9, atol=1e-11)
+    s = S.TFQMR() if solver == "tfqmr" else S.BICGStab()
+    s.setMatrix(A)
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    assert s.solve(torch.zeros(n, dtype=torch.float64, device="cuda"), x)
+    assert float(x.abs().max()) == 0.0 and s.getIterations() == 0
+
+
+def test_bicgstab_tfqmr_convection_diffusion():
+    # the nonsymmetric alternatives to GMRES on the convection-diffusion class
+    off, cols, vals, _ = tnl.convdiff(64, seed=3)
+    n = len(off) - 1
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    b = torch.from_numpy(np.random.default_rng(25).standard_normal(n)).cuda()
+    for s in (S.BICGStab(), S.TFQMR()):
+        s.setMatrix(A)
+        s.setPreconditioner(S.ILU0(A))
+        s.setConvergenceResidue(1e-10)
+        s.setMaxIterations(5000)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        assert s.solve(b, x), type(s).__name__
+        assert S.true_residual(A, x, b) < 1e-8

@@ -869,6 +869,46 @@ private:
    int rmin_ = 10, rmax_ = 10, dmin_ = 3, dmax_ = 3;
 };
 
+//! BICGStab (BICGStab.hpp:12-127): bicgstab-exact-residue via setup()
+template< typename Matrix >
+class BICGStab : public LinearSolver< Matrix >
+{
+public:
+   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "bicgstab-exact-residue" ) )
+         exact_residue_ = p.getParameter< bool >( prefix + "bicgstab-exact-residue" );
+      return LinearSolver< Matrix >::setup( p, prefix );
+   }
+   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
+               typename LinearSolver< Matrix >::VectorType& x ) override
+   {
+      const tnlmi_matrix m = this->mat( "BICGStab" );
+      tnlmi_solver_state st = this->state();
+      detail::check( tnlmi_bicgstab_solve( &m, nullptr, this->pre(), exact_residue_ ? 1 : 0, b.getData(), x.getData(),
+                                           &st, nullptr ) );
+      return this->finish( st );
+   }
+
+private:
+   bool exact_residue_ = false;  // configSetup default, BICGStab.hpp:17-20
+};
+
+//! TFQMR (TFQMR.hpp:12-121)
+template< typename Matrix >
+class TFQMR : public LinearSolver< Matrix >
+{
+public:
+   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
+               typename LinearSolver< Matrix >::VectorType& x ) override
+   {
+      const tnlmi_matrix m = this->mat( "TFQMR" );
+      tnlmi_solver_state st = this->state();
+      detail::check( tnlmi_tfqmr_solve( &m, nullptr, this->pre(), b.getData(), x.getData(), &st, nullptr ) );
+      return this->finish( st );
+   }
+};
+
 //! Jacobi (Jacobi.hpp:65-125)
 template< typename Matrix >
 class Jacobi : public LinearSolver< Matrix >

@@ -1,4 +1,4 @@
-"""Solvers::Linear mirror: CG, GMRES, Jacobi and the Diagonal / ILU0
+"""Solvers::Linear mirror: CG, GMRES, BICGStab, TFQMR, Jacobi and the Diagonal / ILU0
 preconditioners of the reference (Solvers/Linear/*.h), running on the GPU
 through the C ABI.  Method names and defaults follow the reference
 (IterativeSolver.h:226-235, GMRES.hpp:18-69, Jacobi.h:121-123)."""
@@ -161,6 +161,39 @@ class GMRES(IterativeSolver):
                                                 self.restarting_max, self.restarting_step_min,
                                                 self.restarting_step_max, _f64(b), _f64(x), ctypes.byref(st),
                                                 _stream()))
+        return self._finish(st)
+
+
+class BICGStab(IterativeSolver):
+    """Solvers::Linear::BICGStab (BICGStab.hpp:33-127); setup(exact_residue=...)
+    is the bicgstab-exact-residue option (default false: the residue is
+    ||r|| of the BiCGStab recurrence)."""
+
+    def __init__(self):
+        super().__init__()
+        self.exact_residue = False
+
+    def setup(self, exact_residue=None):
+        if exact_residue is not None:
+            self.exact_residue = bool(exact_residue)
+        return True
+
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_bicgstab_solve(m, plan, pre, int(self.exact_residue), _f64(b), _f64(x),
+                                                   ctypes.byref(st), _stream()))
+        return self._finish(st)
+
+
+class TFQMR(IterativeSolver):
+    """Solvers::Linear::TFQMR (TFQMR.hpp:12-121): the residue is the quasi-residual
+    tau sqrt(it + 1) / ||M^-1 b||."""
+
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_tfqmr_solve(m, plan, pre, _f64(b), _f64(x), ctypes.byref(st), _stream()))
         return self._finish(st)
 
 
