@@ -30,9 +30,45 @@ template< typename V, typename X, typename C >
 __device__ __forceinline__ C fetch_nt( const int32_t* __restrict__ cols, const V* __restrict__ vals,
                                        const X* __restrict__ x, int64_t g )
 {
+   // the value is read only for a stored entry: a padding slot (column -1)
+   // costs its 4-byte column index, and a wavefront deep in a padded format's
+   // padding issues no value loads (the gather of x waits for the column
+   // anyway, so this adds no memory round trip)
    const int32_t c = nt_load( cols + g );
+   if( c < 0 )
+      return C( 0 );
    const C v = nt_load_val< V, C >( vals, g );
-   return c >= 0 ? v * (C) x[ c ] : C( 0 );
+   return v * (C) x[ c ];
+}
+
+// sum of fetch_nt over the slots base + stride * k, k < count, added in k
+// order: the column indexes of kSU slots are loaded together, then their
+// values and x entries, so a long strided walk (a padded slice or chunk)
+// keeps kSU loads in flight instead of one dependent pair per slot
+constexpr int kSU = 8;
+
+template< typename V, typename X, typename C >
+__device__ __forceinline__ C strided_sum( const int32_t* __restrict__ cols, const V* __restrict__ vals,
+                                          const X* __restrict__ x, int64_t base, int64_t stride, int32_t count )
+{
+   C acc = 0;
+   int32_t k = 0;
+   for( ; k + kSU <= count; k += kSU ) {
+      int32_t c[ kSU ];
+#pragma unroll
+      for( int u = 0; u < kSU; u++ )
+         c[ u ] = nt_load( cols + base + stride * ( k + u ) );
+      C t[ kSU ];
+#pragma unroll
+      for( int u = 0; u < kSU; u++ )
+         t[ u ] = c[ u ] >= 0 ? nt_load_val< V, C >( vals, base + stride * ( k + u ) ) * (C) x[ c[ u ] ] : C( 0 );
+#pragma unroll
+      for( int u = 0; u < kSU; u++ )
+         acc += t[ u ];
+   }
+   for( ; k < count; k++ )
+      acc += fetch_nt< V, X, C >( cols, vals, x, base + stride * k );
+   return acc;
 }
 
 template< typename V, typename X, typename C >
@@ -303,9 +339,7 @@ __global__ void __launch_bounds__( 256 ) ellpack_cm( int32_t width, int32_t alig
    const int64_t r = a.begin + (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    if( r >= a.end )
       return;
-   C acc = 0;
-   for( int32_t k = 0; k < width; k++ )
-      acc += fetch_nt< V, X, C >( a.cols, a.vals, a.x, r + (int64_t) aligned * k );
+   const C acc = strided_sum< V, X, C >( a.cols, a.vals, a.x, r, aligned, width );
    keep( a.y, r, acc, a.alpha, a.beta );
 }
 
@@ -328,8 +362,9 @@ __global__ void __launch_bounds__( 256 ) ellpack_rm( int32_t width, Args< V, X, 
 }
 
 // =========================================================== SlicedEllpack
-// Column-major: thread per row, slice width from sliceSegmentSizes
-// (ReducingKernels_SlicedEllpack.h:88-~320).
+// Column-major: thread per row over the slice width (every segment of a slice
+// has the slice's size, SlicedEllpackBase.hpp:79-92; padding slots cost only
+// their column index, see fetch_nt) (ReducingKernels_SlicedEllpack.h:88-~320).
 template< typename V, typename X, typename C >
 __global__ void __launch_bounds__( 256 ) sliced_cm( int32_t S, const int32_t* __restrict__ soff,
                                                     const int32_t* __restrict__ ssz, Args< V, X, C > a )
@@ -339,10 +374,7 @@ __global__ void __launch_bounds__( 256 ) sliced_cm( int32_t S, const int32_t* __
       return;
    const int64_t s = r / S;
    const int64_t base = soff[ s ] + r % S;
-   const int32_t w = ssz[ s ];
-   C acc = 0;
-   for( int32_t k = 0; k < w; k++ )
-      acc += fetch_nt< V, X, C >( a.cols, a.vals, a.x, base + (int64_t) S * k );
+   const C acc = strided_sum< V, X, C >( a.cols, a.vals, a.x, base, S, ssz[ s ] );
    keep( a.y, r, acc, a.alpha, a.beta );
 }
 
@@ -368,10 +400,11 @@ __global__ void __launch_bounds__( 256 ) sliced_rm( int32_t S, const int32_t* __
 }
 
 // =============================================================== BiEllpack
-// One 32-lane half-wavefront per strip of 32 rows: lane p accumulates the row
-// at sorted position p over the groups it belongs to (group g holds the first
-// 2^(5-g) positions), then the strip-local inverse permutation (built in LDS
-// from perm) routes the result to its row (ReducingKernels_BiEllpack.h:14-226).
+// One 32-lane half-wavefront per strip of 32 rows; lane p ends with the row at
+// sorted position p (group g holds the first 2^(5-g) positions), and the
+// strip-local inverse permutation (built in LDS from perm) routes the result
+// to its row (ReducingKernels_BiEllpack.h:14-226).  Column-major groups are
+// swept by the whole half-wave (below); row-major rows by their own lane.
 template< typename V, typename X, typename C >
 __global__ void __launch_bounds__( 256 ) biellpack( int64_t n, int64_t strips, int col_major,
                                                     const int32_t* __restrict__ perm,
@@ -389,24 +422,42 @@ __global__ void __launch_bounds__( 256 ) biellpack( int64_t n, int64_t strips, i
    if( ! live )
       return;
    const int64_t cnt = min( (int64_t) W, n - b );
-   if( p >= cnt )
-      return;
    const int32_t* g = gptr + strip * ( LOGW + 1 );
    C acc = 0;
-   int32_t height = W;
-   for( int gi = 0; gi <= LOGW && p < height; gi++, height >>= 1 ) {
-      const int32_t gs = g[ gi + 1 ] - g[ gi ];
-      if( gs == 0 )
-         continue;
-      const int32_t w = gs / height;
-      const int64_t base = g[ gi ];
-      if( col_major )
-         for( int32_t k = 0; k < w; k++ )
-            acc += fetch_nt< V, X, C >( a.cols, a.vals, a.x, base + p + (int64_t) k * height );
-      else
-         for( int32_t k = 0; k < w; k++ )
-            acc += fetch_nt< V, X, C >( a.cols, a.vals, a.x, base + (int64_t) p * w + k );
+   if( col_major ) {
+      // all 32 lanes sweep each group's height x width slots with stride 32:
+      // slot s holds position s % height, so lane p always lands on position
+      // p % height (height divides 32) and every load of the half-wave is
+      // contiguous; an xor butterfly over the lanes of one position then
+      // gives lane p < height its position's sum (a fixed order).  Lanes past
+      // the strip's last row only read padding slots (column -1).
+      int32_t height = W;
+      for( int gi = 0; gi <= LOGW; gi++, height >>= 1 ) {
+         const int32_t gs = g[ gi + 1 ] - g[ gi ];
+         if( gs == 0 )
+            continue;  // uniform over the half-wave
+         const int64_t base = g[ gi ];
+         C part = gs > p ? strided_sum< V, X, C >( a.cols, a.vals, a.x, base + p, W, ( gs - p + W - 1 ) / W ) : C( 0 );
+         for( int o = height; o < W; o <<= 1 )
+            part += __shfl_xor( part, o, W );
+         if( p < height )
+            acc += part;
+      }
    }
+   else {
+      if( p >= cnt )
+         return;
+      int32_t height = W;
+      for( int gi = 0; gi <= LOGW && p < height; gi++, height >>= 1 ) {
+         const int32_t gs = g[ gi + 1 ] - g[ gi ];
+         if( gs == 0 )
+            continue;
+         const int32_t w = gs / height;
+         acc += strided_sum< V, X, C >( a.cols, a.vals, a.x, g[ gi ] + (int64_t) p * w, 1, w );
+      }
+   }
+   if( p >= cnt )
+      return;
    const int64_t r = b + rowof[ half ][ p ];
    if( r >= a.begin && r < a.end )
       keep( a.y, r, acc, a.alpha, a.beta );
@@ -429,11 +480,9 @@ __global__ void __launch_bounds__( 256 ) chunked( int32_t chunks_in_slice, int c
    C acc = 0;
    if( q < chunks_in_slice ) {
       if( col_major )
-         for( int32_t i = 0; i < cs; i++ )
-            acc += fetch_nt< V, X, C >( a.cols, a.vals, a.x, ptr + (int64_t) i * chunks_in_slice + q );
+         acc = strided_sum< V, X, C >( a.cols, a.vals, a.x, ptr + q, chunks_in_slice, cs );
       else
-         for( int32_t i = 0; i < cs; i++ )
-            acc += fetch_nt< V, X, C >( a.cols, a.vals, a.x, ptr + (int64_t) q * cs + i );
+         acc = strided_sum< V, X, C >( a.cols, a.vals, a.x, ptr + (int64_t) q * cs, 1, cs );
    }
    csum[ q ] = acc;
    __syncthreads();
