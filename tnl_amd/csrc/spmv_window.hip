@@ -41,6 +41,8 @@ constexpr int kWinBits = 13;
 constexpr int kWin = 1 << kWinBits;  // x entries per window
 constexpr int kP1 = 1024;            // phase-1 threads
 constexpr int kP2 = 512;             // phase-2 threads
+constexpr int kRowsMax = 1024;       // rows per row block at most
+constexpr int kLongMax = 8;          // rows longer than kLongRow per row block at most (one per wave)
 constexpr int kLongRow = 256;        // longer rows are summed by a whole wave
 constexpr int kPad = 32;             // padding entries of prod and dest (phase-2 reads past a segment)
 
@@ -140,19 +142,21 @@ __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t*
       xw[ t ] = base + t < ncols ? x[ base + t ] : (C) 0;
    __syncthreads();
    const int64_t a = wstart[ k ], b = wstart[ k + 1 ];
+   // plain (cached) loads and stores: the nontemporal forms measured slower
+   // here, the stored products in particular (phase 2 reads them next)
    constexpr int U = 4;
    for( int64_t i0 = a + threadIdx.x; i0 < b; i0 += U * kP1 ) {
       C p[ U ];
 #pragma unroll
       for( int u = 0; u < U; u++ ) {
          const int64_t i = i0 + u * kP1;
-         p[ u ] = i < b ? (C) __builtin_nontemporal_load( pv + i ) * xw[ __builtin_nontemporal_load( pc + i ) ] : (C) 0;
+         p[ u ] = i < b ? (C) pv[ i ] * xw[ pc[ i ] ] : (C) 0;
       }
 #pragma unroll
       for( int u = 0; u < U; u++ ) {
          const int64_t i = i0 + u * kP1;
          if( i < b )
-            __builtin_nontemporal_store( p[ u ], prod + i );
+            prod[ i ] = p[ u ];
       }
    }
 }
@@ -163,12 +167,13 @@ __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t*
 // instruction covers kGrp * kPer terms of a segment, so the lanes of a group
 // touch consecutive cache lines (a lane per segment would touch a line of its
 // own in every load, and the L1 tag lookups, one per line, would bound the
-// kernel).  All kRounds rounds of a thread (kP2 / kGrp groups, kGrp segments
-// each) are issued before the first is stored.
+// kernel).  All rounds of a thread (kP2 / kGrp groups, R = kSegs / (kP2 /
+// kGrp) segments each) are issued before the first is stored.  (1024-thread
+// workgroups at 64 VGPRs, two per CU, measured no faster than 512.)
 constexpr int kGrp = 16;
 constexpr int kPer = 2;
 constexpr int kSpan = kGrp * kPer;  // terms of a segment per load round
-constexpr int kRounds = kGrp;       // segments per group: the group's own lanes hold their bounds
+constexpr int kSegs = 512;          // segments (windows) per sweep of a block
 
 // per-block header: 8 int32 -- rows [r0, r1), nonzeros [e0, e1), the long
 // row a piece belongs to (-1: a run of whole rows), long-row list [l0, l1)
@@ -227,7 +232,11 @@ __global__ void __launch_bounds__( kP2 )
              int64_t begin, int64_t end )
 {
    __shared__ C buf[ CAP ];
-   __shared__ C red[ kP2 / 64 ];
+   constexpr int P2 = kP2;
+   __shared__ C red[ P2 / 64 ];
+   // kSegs segments per sweep: P2 / kGrp groups of R segments each
+   constexpr int R = kSegs * kGrp / P2;
+   static_assert( R >= 1 && R <= kGrp, "a group's lanes hold its segments' bounds" );
    // consecutive row blocks on one XCD: their chunks of a window's products
    // (about 120 bytes each, 30 of slot indexes) share cache lines in its L2
    const int64_t b = xcd_contiguous( blockIdx.x, gridDim.x );
@@ -237,37 +246,40 @@ __global__ void __launch_bounds__( kP2 )
    const int t = threadIdx.x;
    const int lane = t & 63, wid = wave_uniform( (int32_t) ( t >> 6 ) );
    const int gl = lane % kGrp, gb = lane - gl;
-   // segment t (window t) of the block: its bounds, held by thread t
-   const int32_t kt = t < K ? t : K - 1;
+   // segment sx (window sx) of the block: its bounds, held by lane gb + i
+   // (i < R) of the group's lanes
+   const int32_t sx = wid * ( 64 / kGrp ) * R + ( lane / kGrp ) * R + std::min( gl, R - 1 );
+   const int32_t kt = sx < K ? sx : K - 1;
    const int32_t a = seg[ b * K + kt ];
-   const int32_t z = t < K ? seg[ ( b + 1 ) * K + kt ] : a;
-   // the rows of this thread (a block has at most 2 kP2) and of its wave's
+   const int32_t z = sx < K ? seg[ ( b + 1 ) * K + kt ] : a;
+   // the rows of this thread (a block has at most kRowsMax) and of its wave's
    // long row (at most one per wave); unconditional loads at clamped indexes
-   int32_t ob[ 2 ], oe[ 2 ];
+   constexpr int J = kRowsMax / P2;
+   int32_t ob[ J ], oe[ J ];
    const int32_t last = std::max( H.r0, H.r1 - 1 );
 #pragma unroll
-   for( int j = 0; j < 2; j++ ) {
-      const int32_t r = std::min( H.r0 + t + j * kP2, last );
+   for( int j = 0; j < J; j++ ) {
+      const int32_t r = std::min( H.r0 + t + j * P2, last );
       ob[ j ] = off[ r ];
       oe[ j ] = off[ r + 1 ];
    }
    const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
    const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
-   // round i: the group's segment gb + i (of this wave's 64), kSpan terms
-   C v[ kRounds ][ kPer ];
-   uint32_t d[ kRounds ];
+   // round i: the group's segment i, kSpan terms
+   C v[ R ][ kPer ];
+   uint32_t d[ R ];
 #pragma unroll
-   for( int i = 0; i < kRounds; i++ ) {
+   for( int i = 0; i < R; i++ ) {
       const int32_t ai = __shfl( a, gb + i, 64 );
       get_pair( prod, dest, ai + kPer * gl, v[ i ], d[ i ] );
    }
    // clear (padding slots, column -1, have no product: they read as zero)
    const int32_t len = H.e1 - H.e0;
-   for( int32_t q = t; q < len; q += kP2 )
+   for( int32_t q = t; q < len; q += P2 )
       buf[ q ] = (C) 0;
    __syncthreads();
 #pragma unroll
-   for( int i = 0; i < kRounds; i++ ) {
+   for( int i = 0; i < R; i++ ) {
       const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
       put_pair( buf, v[ i ], d[ i ], ai + kPer * gl, zi );
       // on demand: the rest of a segment longer than kSpan
@@ -278,11 +290,11 @@ __global__ void __launch_bounds__( kP2 )
          put_pair( buf, w, dw, i0, zi );
       }
    }
-   // on demand: windows past kP2 (segment k handled by the group of thread k % kP2)
-   for( int32_t k0 = kP2; k0 < K; k0 += kP2 ) {
-      const int32_t k = std::min( k0 + t, K - 1 );
-      const int32_t ak = seg[ b * K + k ], zk = k0 + t < K ? seg[ ( b + 1 ) * K + k ] : ak;
-      for( int i = 0; i < kRounds; i++ ) {
+   // on demand: windows past kSegs (segment k handled as segment k % kSegs)
+   for( int32_t k0 = kSegs; k0 < K; k0 += kSegs ) {
+      const int32_t k = std::min( k0 + sx, K - 1 );
+      const int32_t ak = seg[ b * K + k ], zk = k0 + sx < K ? seg[ ( b + 1 ) * K + k ] : ak;
+      for( int i = 0; i < R; i++ ) {
          const int32_t ai = __shfl( ak, gb + i, 64 ), zi = __shfl( zk, gb + i, 64 );
          for( int32_t i0 = ai + kPer * gl; i0 < zi; i0 += kSpan ) {
             C w[ kPer ];
@@ -302,9 +314,9 @@ __global__ void __launch_bounds__( kP2 )
       // a piece of a row longer than cap: its partial sum (strided parts,
       // then a fixed-order reduction), added to the other pieces' by w_fixup
       C p = 0;
-      for( int32_t q = t; q < len; q += kP2 )
+      for( int32_t q = t; q < len; q += P2 )
          p += buf[ q ];
-      const C sum = block_sum< kP2 >( p, red );
+      const C sum = block_sum< P2 >( p, red );
       if( t == 0 )
          part[ b ] = sum;
       return;
@@ -312,8 +324,8 @@ __global__ void __launch_bounds__( kP2 )
    // rows of at most kLongRow nonzeros: one thread each, summed from zero
    // left to right
 #pragma unroll
-   for( int j = 0; j < 2; j++ ) {
-      const int32_t r = H.r0 + t + j * kP2;
+   for( int j = 0; j < J; j++ ) {
+      const int32_t r = H.r0 + t + j * P2;
       if( r < H.r1 && oe[ j ] - ob[ j ] <= kLongRow )
          keep( r, sum_row( buf, ob[ j ] - e0, oe[ j ] - e0 ) );
    }
@@ -378,11 +390,11 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
          r++;
          continue;
       }
-      // at most cap nonzeros, 2 kP2 rows and kP2 / 64 rows longer than kLongRow
+      // at most cap nonzeros, kRowsMax rows and kLongMax rows longer than kLongRow
       int64_t e = r, longs = 0;
-      while( e < n && hoff[ e + 1 ] - start <= cap && e - r < 2 * kP2 ) {
+      while( e < n && hoff[ e + 1 ] - start <= cap && e - r < kRowsMax ) {
          const bool lg = hoff[ e + 1 ] - hoff[ e ] > kLongRow;
-         if( lg && longs == kP2 / 64 )
+         if( lg && longs == kLongMax )
             break;
          longs += lg;
          e++;
