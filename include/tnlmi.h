@@ -78,7 +78,9 @@ tnlmi_status tnlmi_segments_create( tnlmi_format format, tnlmi_organization orga
                                     tnlmi_segments** out );
 void tnlmi_segments_destroy( tnlmi_segments* segments );
 
-/* getSegmentCount() / getStorageSize() / getElementCount() */
+/* getSegmentCount() / getStorageSize() / getElementCount() (the sum of the
+ * segment sizes; for segments loaded from a file, the sizes as getSegmentSize
+ * derives them from the file's arrays) */
 int64_t tnlmi_segments_count( const tnlmi_segments* segments );
 int64_t tnlmi_segments_storage_size( const tnlmi_segments* segments );
 int64_t tnlmi_segments_element_count( const tnlmi_segments* segments );
@@ -179,6 +181,42 @@ tnlmi_status tnlmi_mtx_parse( const char* text, int64_t length, tnlmi_host_csr**
 tnlmi_status tnlmi_host_csr_get( const tnlmi_host_csr* m, int64_t* rows, int64_t* columns, int64_t* nnz,
                                  const int32_t** offsets, const int32_t** columns_out, const double** values );
 void tnlmi_host_csr_destroy( tnlmi_host_csr* m );
+
+/* Binary files in TNL's format (File / Object serialization).
+ * tnlmi_matrix_save replaces SparseMatrix::save (Matrices/SparseMatrix.hpp:
+ * 642-646 with operator<< of SparseMatrixBase.hpp:1149-1159 and the segments'
+ * save, e.g. Segments/CSR.hpp:116-121): the type string, rows, columns, the
+ * segments' fields and arrays, values, column indexes -- the bytes TNL writes
+ * for the same matrix (AdaptiveCSR: its kernel descriptors as built for
+ * wave64; ChunkedEllpack: slice entries past the last slice written as zero).
+ * FP64 and FP32 values (BF16: TNLMI_NOT_IMPLEMENTED).  Synchronous.
+ * tnlmi_matrix_serialization_type: SparseMatrixBase::getSerializationType
+ * (SparseMatrixBase.hpp:46-53), written to buffer (NUL-terminated, cut to
+ * capacity) with its full length in *length. */
+tnlmi_status tnlmi_matrix_save( const tnlmi_matrix* A, const char* path, tnlmi_stream stream );
+tnlmi_status tnlmi_matrix_serialization_type( const tnlmi_matrix* A, char* buffer, int64_t capacity,
+                                              int64_t* length );
+/* SparseMatrix::load (SparseMatrix.hpp:656-694): tnlmi_matrix_file_read parses
+ * and checks a file on the host (errors: TNLMI_RUNTIME_ERROR with the messages
+ * of Exceptions/FileDeserializationError.h; a type this library cannot hold:
+ * TNLMI_NOT_IMPLEMENTED).  Every index the file's segments define must lie in
+ * its storage and every column index in [-1, columns).  Then
+ * tnlmi_matrix_file_info gives the matrix type and storage size,
+ * tnlmi_matrix_file_segments builds the segments (the file's own arrays; the
+ * segment sizes as getSegmentSize derives them), tnlmi_matrix_file_upload
+ * copies values and column indexes into caller storage of that size.  The
+ * caller compares tnlmi_matrix_file_type with its own type where TNL would
+ * (object type does not match ...). */
+typedef struct tnlmi_matrix_file tnlmi_matrix_file;
+tnlmi_status tnlmi_matrix_file_read( const char* path, tnlmi_matrix_file** out );
+tnlmi_status tnlmi_matrix_file_info( const tnlmi_matrix_file* f, int64_t* rows, int64_t* columns,
+                                     tnlmi_format* format, tnlmi_organization* organization, int32_t* param,
+                                     tnlmi_dtype* value_type, int64_t* storage );
+const char* tnlmi_matrix_file_type( const tnlmi_matrix_file* f );
+tnlmi_status tnlmi_matrix_file_segments( const tnlmi_matrix_file* f, tnlmi_stream stream, tnlmi_segments** out );
+tnlmi_status tnlmi_matrix_file_upload( const tnlmi_matrix_file* f, void* d_values, int32_t* d_column_indexes,
+                                       tnlmi_stream stream );
+void tnlmi_matrix_file_destroy( tnlmi_matrix_file* f );
 
 /* Segments/LaunchConfiguration.h: ThreadsToSegmentsMapping (same values) plus
  * TNLMI_MAP_AUTO, the MI355X-tuned default (tiled nnz-balanced CSR). */

@@ -174,6 +174,37 @@ struct SpmvOp
    }
 };
 
+template< template< class, class, class > class S >
+struct SaveOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v,
+                       const char* path )
+   {
+      Mat< S > m;
+      build( m, n, ncols, rp, c, v );
+      m.save( path );
+      return 0;
+   }
+};
+template< template< class, class, class > class S >
+struct LoadOp
+{
+   static int64_t run( const char* path, int64_t* dims, int32_t* oc, double* ov, int64_t cap )
+   {
+      Mat< S > m;
+      m.load( path );
+      dims[ 0 ] = m.getRows();
+      dims[ 1 ] = m.getColumns();
+      const int64_t storage = m.getValues().getSize();
+      if( storage <= cap )
+         for( int64_t i = 0; i < storage; i++ ) {
+            oc[ i ] = m.getColumnIndexes()[ i ];
+            ov[ i ] = m.getValues()[ i ];
+         }
+      return storage;
+   }
+};
+
 struct State
 {
    int64_t max_iterations, min_iterations;
@@ -539,6 +570,33 @@ void ref_read_mtx_arrays( int32_t* off, int32_t* col, double* val )
 const char* ref_last_error()
 {
    return g_err.c_str();
+}
+
+// SparseMatrix::save / load (SparseMatrix.hpp:640-700) of a matrix of the given
+// format (numbering of ref_fill); -1 with the message in ref_last_error() when
+// TNL throws.  ref_load returns the storage size (arrays copied when <= cap).
+int64_t ref_save( int format, int col_major, int64_t n, int64_t ncols, const int32_t* row_ptr, const int32_t* cols,
+                  const double* vals, const char* path )
+{
+   try {
+      return dispatch< SaveOp >( format, col_major, n, ncols, row_ptr, cols, vals, path );
+   }
+   catch( const std::exception& e ) {
+      g_err = e.what();
+      return -1;
+   }
+}
+
+int64_t ref_load( int format, int col_major, const char* path, int64_t* dims, int32_t* out_cols, double* out_vals,
+                  int64_t cap )
+{
+   try {
+      return dispatch< LoadOp >( format, col_major, path, dims, out_cols, out_vals, cap );
+   }
+   catch( const std::exception& e ) {
+      g_err = e.what();
+      return -1;
+   }
 }
 
 // OpenMP threads of TNL's Host algorithms in the following calls

@@ -4,6 +4,7 @@
 // Documentation/Examples/Solvers/Linear/IterativeLinearSolverExample.cpp).
 // Prints "ALL OK" and exits 0 when every check passes.
 #include <sstream>
+#include <unistd.h>
 
 #include <TNL/MI355X.h>
 
@@ -228,6 +229,39 @@ int main()
          thrown = true;
       }
       EXPECT_EQ( thrown, true );
+   }
+
+   // SparseMatrix::save / load: BiEllpack written, read back into a new
+   // matrix, the same product; a file of another type is refused
+   {
+      using BMatrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Seg::BiEllpack >;
+      BMatrix Bm;
+      Matrices::copySparseMatrix( Bm, *A );
+      const std::string path = "/tmp/tnlmi_test_tnl_api_" + std::to_string( ::getpid() ) + ".tnl";
+      Bm.save( path );
+      BMatrix Lm;
+      Lm.load( path );
+      EXPECT_EQ( Lm.getRows(), 5 );
+      EXPECT_EQ( Lm.getElement( 2, 3 ), -1.0 );
+      EXPECT_EQ( BMatrix::getSerializationType().find( "BiEllpack< int, ColumnMajorOrder, 32 >" ) != std::string::npos,
+                 true );
+      Vector x( 5 ), y( 5 ), z( 5 );
+      for( int i = 0; i < 5; i++ )
+         x.setElement( i, 1.0 + i );
+      Bm.vectorProduct( x, y );
+      Lm.vectorProduct( x, z );
+      EXPECT_EQ( y == z, true );
+      bool thrown = false;
+      try {
+         Matrix Cm;
+         Cm.load( path );
+      }
+      catch( const std::runtime_error& e ) {
+         thrown = std::string( e.what() ).find( "object type does not match" ) != std::string::npos;
+      }
+      EXPECT_EQ( thrown, true );
+      std::remove( path.c_str() );
+      std::cout << "SparseMatrix save / load (BiEllpack) done\n";
    }
 
    // error behaviour of the reference

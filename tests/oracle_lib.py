@@ -235,8 +235,10 @@ class Reference:
             return None
         r = cls()
         r.L = ctypes.CDLL(REF_SO)
-        for name in ["ref_chunked_ellpack", "ref_adaptive_csr_blocks", "ref_fill", "ref_spmv"]:
+        for name in ["ref_chunked_ellpack", "ref_adaptive_csr_blocks", "ref_fill", "ref_spmv", "ref_save",
+                     "ref_load"]:
             getattr(r.L, name).restype = ctypes.c_int64
+        r.L.ref_last_error.restype = ctypes.c_char_p
         r.L.ref_spmv_csr_time.restype = ctypes.c_double
         r.L.ref_true_residual.restype = ctypes.c_double
         return r
@@ -309,6 +311,24 @@ class Reference:
         self.L.ref_fill(self.FORMATS[fmt], int(col_major), I64(n), I64(ncols), P(off), P(cols), P(vals), P(oc),
                         P(ov), I64(cap))
         return oc, ov
+
+    def save_file(self, fmt, off, cols, vals, ncols, path, col_major=True):
+        """SparseMatrix::save of the matrix TNL builds from these CSR arrays (as fill)."""
+        off, cols, vals = i32(off), i32(cols), f64(vals)
+        rc = self.L.ref_save(self.FORMATS[fmt], int(col_major), I64(len(off) - 1), I64(ncols), P(off), P(cols),
+                             P(vals), os.fsencode(path))
+        if rc < 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+
+    def load_file(self, fmt, path, col_major=True):
+        """SparseMatrix::load into TNL's matrix of this format: (rows, columns), column indexes, values."""
+        dims = np.zeros(2, np.int64)
+        st = self.L.ref_load(self.FORMATS[fmt], int(col_major), os.fsencode(path), P(dims), None, None, I64(0))
+        if st < 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+        oc, ov = np.zeros(st, np.int32), np.zeros(st)
+        self.L.ref_load(self.FORMATS[fmt], int(col_major), os.fsencode(path), P(dims), P(oc), P(ov), I64(st))
+        return (int(dims[0]), int(dims[1])), oc, ov
 
     def spmv(self, fmt, off, cols, vals, x, y=None, alpha=1.0, beta=0.0, begin=0, end=0, col_major=True):
         off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)

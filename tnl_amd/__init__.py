@@ -88,6 +88,16 @@ class Segments:
         self.format = fmt
         self.organization = organization
 
+    @classmethod
+    def _adopt(cls, handle, fmt, organization):
+        """A segments object around a handle the library built (it is destroyed with it)."""
+        obj = cls.__new__(cls)
+        obj._sizes = None
+        obj.handle = handle
+        obj.format = fmt
+        obj.organization = organization
+        return obj
+
     def __del__(self):
         h = getattr(self, "handle", None)
         try:
@@ -224,6 +234,53 @@ class SparseMatrix:
         rows, columns, off, cols, vals = read_mtx(path)
         return cls.from_csr(rows, columns, off, cols, vals, fmt=fmt, value_dtype=value_dtype,
                             organization=organization, param=param)
+
+    def getSerializationType(self):
+        """SparseMatrixBase::getSerializationType (Matrices/SparseMatrixBase.hpp:46-53)."""
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().tnlmi_matrix_serialization_type(ctypes.byref(self._m), None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _lib.check(_lib.lib().tnlmi_matrix_serialization_type(ctypes.byref(self._m), buf, n.value + 1,
+                                                              ctypes.byref(n)))
+        return buf.value.decode()
+
+    def save(self, path):
+        """SparseMatrix::save: TNL's binary format (tnlmi_matrix_save)."""
+        _lib.check(_lib.lib().tnlmi_matrix_save(ctypes.byref(self._m), os.fsencode(path), _stream()))
+
+    @classmethod
+    def load(cls, path, expected_type=None):
+        """SparseMatrix::load of a TNL binary file (tnlmi_matrix_file_*): the format,
+        organization and value type the file names.  expected_type (a
+        getSerializationType() string) is checked as TNL's load checks its own
+        type (SparseMatrix.hpp:676-682)."""
+        torch = _torch()
+        L = _lib.lib()
+        f = ctypes.c_void_p()
+        _lib.check(L.tnlmi_matrix_file_read(os.fsencode(path), ctypes.byref(f)))
+        try:
+            found = L.tnlmi_matrix_file_type(f).decode()
+            if expected_type is not None and found != expected_type:
+                raise _lib.TnlmiError(_lib.RUNTIME_ERROR, f"Failed to deserialize an object from the file '{path}': "
+                                                          f"object type does not match (expected {expected_type}, "
+                                                          f"found {found}).")
+            rows, cols, storage = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+            fmt, org, vt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            param = ctypes.c_int32()
+            _lib.check(L.tnlmi_matrix_file_info(f, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(fmt),
+                                                ctypes.byref(org), ctypes.byref(param), ctypes.byref(vt),
+                                                ctypes.byref(storage)))
+            h = ctypes.c_void_p()
+            _lib.check(L.tnlmi_matrix_file_segments(f, _stream(), ctypes.byref(h)))
+            seg = Segments._adopt(h, fmt.value, org.value)
+            n = max(1, storage.value)
+            values = torch.empty(n, dtype=torch_dtype(vt.value), device="cuda")
+            colidx = torch.empty(n, dtype=torch.int32, device="cuda")
+            _lib.check(L.tnlmi_matrix_file_upload(f, ctypes.c_void_p(values.data_ptr()),
+                                                  ctypes.c_void_p(colidx.data_ptr()), _stream()))
+        finally:
+            L.tnlmi_matrix_file_destroy(f)
+        return cls(rows.value, cols.value, seg, values, colidx)
 
     def getRows(self):
         return self.rows

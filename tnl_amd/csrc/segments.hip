@@ -613,6 +613,270 @@ void fill_typed( const tnlmi_segments& seg, const int32_t* off, const int32_t* c
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ file arrays
+namespace {
+
+[[noreturn]] void bad_file( const std::string& what )
+{
+   fail( TNLMI_RUNTIME_ERROR, "matrix file: " + what );
+}
+
+bool from_zero_nondecreasing( const std::vector< int32_t >& a )
+{
+   if( a.empty() || a[ 0 ] != 0 )
+      return false;
+   for( size_t i = 1; i < a.size(); i++ )
+      if( a[ i ] < a[ i - 1 ] )
+         return false;
+   return true;
+}
+
+}  // namespace
+
+void segments_export( const tnlmi_segments& seg, SegmentsFileData& f, hipStream_t s )
+{
+   const int64_t n = seg.n;
+   f = SegmentsFileData{};
+   f.elements = seg.file_size >= 0 ? seg.file_size : seg.elements;
+   f.storage = seg.storage;
+   switch( seg.format ) {
+      case TNLMI_CSR:
+      case TNLMI_ADAPTIVE_CSR:
+         f.offsets = download( seg.offsets.ptr, n + 1, s );
+         if( seg.format == TNLMI_ADAPTIVE_CSR )
+            for( int i = 0; i < 6; i++ )
+               f.blocks[ i ] = download( seg.blocks[ i ].ptr, seg.nblocks[ i ] * 6, s );
+         break;
+      case TNLMI_ELLPACK:
+         f.ell_width = seg.ell_width;
+         f.ell_aligned = seg.ell_aligned;
+         f.ell_count = (int32_t) n;
+         break;
+      case TNLMI_SLICED_ELLPACK:
+         f.slice_offsets = download( seg.slice_offsets.ptr, seg.slices + 1, s );
+         f.slice_seg_sizes = download( seg.slice_seg_sizes.ptr, seg.slices, s );
+         break;
+      case TNLMI_BIELLPACK:
+         f.perm = download( seg.perm.ptr, n, s );
+         f.group_ptrs = download( seg.group_ptrs.ptr, seg.group_ptrs.size, s );
+         break;
+      case TNLMI_CHUNKED_ELLPACK:
+         f.ch_nslices = (int32_t) seg.ch_nslices;
+         f.chunks_in_slice = seg.chunks_in_slice;
+         f.desired_chunk_size = seg.desired_chunk_size;
+         f.seg_to_chunk = download( seg.seg_to_chunk.ptr, n, s );
+         f.seg_to_slice = download( seg.seg_to_slice.ptr, n, s );
+         f.chunks_to_seg = download( seg.chunks_to_seg.ptr, seg.chunks_to_seg.size, s );
+         f.seg_ptrs = download( seg.seg_ptrs.ptr, n + 1, s );
+         f.ch_slices = download( seg.ch_slices.ptr, 4 * seg.ch_nslices, s );
+         // TNL sizes its slice array with one entry per segment
+         // (ChunkedEllpack.hpp:163); the entries past the last slice are unused
+         f.ch_slices.resize( 4 * std::max< int64_t >( n, seg.ch_nslices ), 0 );
+         break;
+      case TNLMI_SORTED_CSR:
+         f.offsets = download( seg.sorted_offsets.ptr, n + 1, s );
+         f.perm = download( seg.perm.ptr, n, s );
+         f.inv_perm = download( seg.inv_perm.ptr, n, s );
+         break;
+   }
+}
+
+void segments_import( tnlmi_segments& seg, const SegmentsFileData& f, hipStream_t s )
+{
+   const int64_t n = seg.n;
+   auto check = []( bool ok, const char* what ) {
+      if( ! ok )
+         bad_file( what );
+   };
+   check( f.storage >= 0 && f.storage <= INT32_MAX && f.elements >= 0, "storage size out of the int32 index range" );
+   std::vector< int32_t > sizes( n );
+   SegView v{};  // host view of the file's arrays, for the index check below
+   v.format = seg.format;
+   v.col_major = seg.org == TNLMI_COLUMN_MAJOR;
+   v.n = n;
+   switch( seg.format ) {
+      case TNLMI_CSR:
+      case TNLMI_ADAPTIVE_CSR:
+         check( (int64_t) f.offsets.size() == n + 1 && from_zero_nondecreasing( f.offsets ),
+                "CSR offsets are not rows + 1 nondecreasing values from 0" );
+         for( int64_t r = 0; r < n; r++ )
+            sizes[ r ] = f.offsets[ r + 1 ] - f.offsets[ r ];
+         break;
+      case TNLMI_ELLPACK: {
+         const int64_t aligned = seg.org == TNLMI_ROW_MAJOR ? n : ceil_div( n, 32 ) * 32;
+         check( f.ell_count == n, "Ellpack segment count differs from the matrix rows" );
+         check( f.ell_width >= 0 && f.ell_aligned == aligned, "Ellpack segment size or aligned size out of range" );
+         check( aligned * f.ell_width <= INT32_MAX, "Ellpack storage out of the int32 index range" );
+         std::fill( sizes.begin(), sizes.end(), f.ell_width );
+         v.ell_width = f.ell_width;
+         v.ell_aligned = f.ell_aligned;
+         break;
+      }
+      case TNLMI_SLICED_ELLPACK: {
+         const int64_t S = seg.param, ns = ceil_div( n, S );
+         check( (int64_t) f.slice_seg_sizes.size() == ns && (int64_t) f.slice_offsets.size() == ns + 1
+                   && from_zero_nondecreasing( f.slice_offsets ) && f.slice_offsets.back() == f.storage,
+                "SlicedEllpack slice arrays do not match the rows and the storage size" );
+         for( int64_t q = 0; q < ns; q++ )  // SlicedEllpack.hpp:137-141: slice q holds S rows of its width
+            check( f.slice_seg_sizes[ q ] >= 0
+                      && (int64_t) f.slice_offsets[ q + 1 ] - f.slice_offsets[ q ] == S * f.slice_seg_sizes[ q ],
+                   "SlicedEllpack slice offsets do not match the slice widths" );
+         for( int64_t r = 0; r < n; r++ )
+            sizes[ r ] = f.slice_seg_sizes[ r / S ];
+         v.slice_size = (int32_t) S;
+         v.slice_offsets = f.slice_offsets.data();
+         v.slice_seg_sizes = f.slice_seg_sizes.data();
+         break;
+      }
+      case TNLMI_BIELLPACK: {
+         // BiEllpack.h getSegmentSizeDirect: the widths of the row's active groups
+         const int32_t W = seg.param, logw = W == 32 ? 5 : 6;
+         const int64_t strips = ceil_div( n, W );
+         check( (int64_t) f.perm.size() == n && (int64_t) f.group_ptrs.size() == strips * ( logw + 1 ) + 1
+                   && from_zero_nondecreasing( f.group_ptrs ) && f.group_ptrs.back() == f.storage,
+                "BiEllpack arrays do not match the rows and the storage size" );
+         std::vector< char > seen( W );
+         for( int64_t st = 0; st < strips; st++ ) {
+            std::fill( seen.begin(), seen.end(), 0 );
+            for( int64_t r = st * W; r < std::min< int64_t >( n, ( st + 1 ) * W ); r++ ) {
+               const int64_t p = (int64_t) f.perm[ r ] - st * W;
+               check( p >= 0 && p < W && ! seen[ p ], "BiEllpack permutation is not a permutation of its strip" );
+               seen[ p ] = 1;
+            }
+         }
+         for( int64_t r = 0; r < n; r++ ) {
+            const int64_t st = r / W, g0 = st * ( logw + 1 );
+            const int32_t rsp = f.perm[ r ] - (int32_t) ( st * W );
+            int groups = 1;
+            for( int i = 0, bis = 1; i < logw + 1; i++, bis *= 2 )
+               if( rsp < bis ) {
+                  groups = logw + 1 - i;
+                  break;
+               }
+            int64_t size = 0;
+            for( int g = 0, height = W; g < groups; g++, height /= 2 )
+               size += ( f.group_ptrs[ g0 + g + 1 ] - f.group_ptrs[ g0 + g ] ) / height;
+            check( size <= INT32_MAX, "BiEllpack segment size out of range" );
+            sizes[ r ] = (int32_t) size;
+         }
+         v.perm = f.perm.data();
+         v.group_ptrs = f.group_ptrs.data();
+         v.warp = W;
+         v.logw = logw;
+         break;
+      }
+      case TNLMI_CHUNKED_ELLPACK: {
+         // ChunkedEllpack.h getSegmentSizeDirect: chunks of the row x the slice's chunk size
+         const int64_t ns = f.ch_nslices, C = f.chunks_in_slice;
+         check( ns >= 0 && C > 0 && f.desired_chunk_size > 0 && (int64_t) f.seg_to_chunk.size() == n
+                   && (int64_t) f.seg_to_slice.size() == n && (int64_t) f.seg_ptrs.size() == n + 1
+                   && (int64_t) f.chunks_to_seg.size() == ns * C && (int64_t) f.ch_slices.size() >= 4 * ns,
+                "ChunkedEllpack arrays do not match the rows and the slice count" );
+         // the SpMV kernel sums every chunk of a slice (one thread per chunk,
+         // one per row): TNL's 256 chunks per slice, slices covering the rows
+         // in order, each slice's chunk region inside the storage
+         check( C == 256, "ChunkedEllpack with other than 256 chunks per slice" );
+         int64_t next = 0;
+         for( int64_t si = 0; si < ns; si++ ) {
+            const int32_t* S = &f.ch_slices[ 4 * si ];
+            check( S[ 0 ] >= 0 && S[ 0 ] <= C && S[ 2 ] == next && S[ 1 ] >= 0 && S[ 3 ] >= 0
+                      && (int64_t) S[ 3 ] + C * (int64_t) S[ 1 ] <= f.storage,
+                   "ChunkedEllpack slice descriptor out of range" );
+            next += S[ 0 ];
+         }
+         check( next == n, "ChunkedEllpack slices do not cover the rows" );
+         for( int64_t r = 0; r < n; r++ ) {
+            const int64_t si = f.seg_to_slice[ r ];
+            check( si >= 0 && si < ns, "ChunkedEllpack segment-to-slice mapping out of range" );
+            const int32_t* S = &f.ch_slices[ 4 * si ];
+            check( S[ 2 ] <= r && r < S[ 2 ] + S[ 0 ], "ChunkedEllpack segment-to-slice mapping out of range" );
+            const int64_t first = r != S[ 2 ] ? f.seg_to_chunk[ r - 1 ] : 0, last = f.seg_to_chunk[ r ];
+            check( first >= 0 && first <= last && last <= C, "ChunkedEllpack segment-to-chunk mapping out of range" );
+            check( ( last - first ) * S[ 1 ] <= INT32_MAX, "ChunkedEllpack segment size out of range" );
+            sizes[ r ] = (int32_t) ( ( last - first ) * S[ 1 ] );
+         }
+         v.ch_slices = f.ch_slices.data();
+         v.seg_to_chunk = f.seg_to_chunk.data();
+         v.seg_to_slice = f.seg_to_slice.data();
+         v.chunks_in_slice = f.chunks_in_slice;
+         break;
+      }
+      case TNLMI_SORTED_CSR: {
+         check( (int64_t) f.offsets.size() == n + 1 && from_zero_nondecreasing( f.offsets )
+                   && (int64_t) f.perm.size() == n && (int64_t) f.inv_perm.size() == n,
+                "SortedSegments arrays do not match the rows" );
+         for( int64_t r = 0; r < n; r++ )
+            check( f.perm[ r ] >= 0 && f.perm[ r ] < n && f.inv_perm[ f.perm[ r ] ] == r,
+                   "SortedSegments permutations are not inverse permutations" );
+         for( int64_t r = 0; r < n; r++ )
+            sizes[ r ] = f.offsets[ f.perm[ r ] + 1 ] - f.offsets[ f.perm[ r ] ];
+         break;
+      }
+      default:
+         fail( TNLMI_INVALID_ARGUMENT, "segments: unknown format" );
+   }
+   // every element of every segment inside the storage (the kernels index
+   // with these arrays unchecked)
+   if( seg.format == TNLMI_CSR || seg.format == TNLMI_ADAPTIVE_CSR || seg.format == TNLMI_SORTED_CSR )
+      check( f.offsets.back() == f.storage, "CSR offsets do not end at the storage size" );
+   else
+      for( int64_t r = 0; r < n; r++ )
+         for( int32_t k = 0; k < sizes[ r ]; k++ ) {
+            const int64_t g = global_index( v, r, k );
+            check( g >= 0 && g < f.storage, "a segment element lies outside the storage" );
+         }
+   // the format built from these sizes, then the file's own arrays
+   DeviceArray< int32_t > d;
+   upload( d, sizes, s );
+   build( seg, d.ptr, s );
+   switch( seg.format ) {
+      case TNLMI_ELLPACK:
+         seg.ell_width = f.ell_width;
+         seg.ell_aligned = f.ell_aligned;
+         seg.storage = (int64_t) f.ell_aligned * f.ell_width;
+         break;
+      case TNLMI_SLICED_ELLPACK:
+         upload( seg.slice_offsets, f.slice_offsets, s );
+         upload( seg.slice_seg_sizes, f.slice_seg_sizes, s );
+         seg.storage = f.storage;
+         seg.file_size = f.elements;
+         break;
+      case TNLMI_BIELLPACK:
+         upload( seg.perm, f.perm, s );
+         upload( seg.group_ptrs, f.group_ptrs, s );
+         seg.storage = f.storage;
+         seg.file_size = f.elements;
+         break;
+      case TNLMI_CHUNKED_ELLPACK: {
+         seg.ch_nslices = f.ch_nslices;
+         seg.chunks_in_slice = f.chunks_in_slice;
+         seg.desired_chunk_size = f.desired_chunk_size;
+         upload( seg.seg_to_chunk, f.seg_to_chunk, s );
+         upload( seg.seg_to_slice, f.seg_to_slice, s );
+         upload( seg.chunks_to_seg, f.chunks_to_seg, s );
+         upload( seg.seg_ptrs, f.seg_ptrs, s );
+         upload( seg.ch_slices, std::vector< int32_t >( f.ch_slices.begin(), f.ch_slices.begin() + 4 * f.ch_nslices ), s );
+         seg.storage = f.storage;
+         seg.file_size = f.elements;
+         break;
+      }
+      case TNLMI_SORTED_CSR:
+         // the file's order (a TNL object sorted in pieces keeps its own)
+         upload( seg.perm, f.perm, s );
+         upload( seg.inv_perm, f.inv_perm, s );
+         upload( seg.sorted_offsets, f.offsets, s );
+         if( seg.tiles > 0 )
+            hipLaunchKernelGGL( csr_tile_rows, dim3( ceil_div( seg.tiles + 1, 256 ) ), dim3( 256 ), 0, s,
+                                seg.sorted_offsets.ptr, n, seg.tiles, seg.tile_nnz, seg.tile_rows.ptr );
+         break;
+      default:
+         break;
+   }
+   TNLMI_HIP( hipGetLastError() );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+}
+
 }  // namespace tnlmi
 
 using namespace tnlmi;

@@ -2,6 +2,8 @@
 // Segments) and their global-index functions.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 // The opaque handle of the C ABI.  Every format also keeps the CSR offsets of
@@ -54,6 +56,12 @@ struct tnlmi_segments
 
    // SortedSegments: CSR offsets of the sorted sizes (the embedded CSR)
    tnlmi::DeviceArray< int32_t > sorted_offsets;
+
+   // SlicedEllpack / BiEllpack / ChunkedEllpack loaded from a file: the `size`
+   // field it held (the sum of the sizes the segments were first built from;
+   // `elements` is the sum of the sizes as getSegmentSize derives them), written
+   // back by save.  -1 otherwise.
+   int64_t file_size = -1;
 };
 
 namespace tnlmi {
@@ -91,7 +99,7 @@ struct SegView
 SegView make_view( const tnlmi_segments& s );
 
 // getGlobalIndex( r, k ) of every format (reference file per branch).
-__device__ inline int64_t global_index( const SegView& v, int64_t r, int32_t k )
+__host__ __device__ inline int64_t global_index( const SegView& v, int64_t r, int32_t k )
 {
    switch( v.format ) {
       case TNLMI_CSR:
@@ -146,6 +154,34 @@ __device__ inline int64_t global_index( const SegView& v, int64_t r, int32_t k )
    }
    return -1;
 }
+
+// The index arrays and counters of a segments object as TNL's segments save
+// and load them (host copies; the fields of other formats stay empty):
+// CSR.hpp:116-130, Ellpack.hpp:105-120, SlicedEllpack.hpp:166-190,
+// BiEllpack.hpp:222-240, ChunkedEllpack.hpp:240-275, AdaptiveCSR.hpp:145-160,
+// SortedSegments.hpp:242-260.
+struct SegmentsFileData
+{
+   int64_t elements = 0, storage = 0;  // `size` (sum of the sizes) and storageSize
+   int32_t ell_width = 0, ell_aligned = 0, ell_count = 0;
+   std::vector< int32_t > offsets;  // CSR, AdaptiveCSR; the embedded CSR of SortedSegments
+   std::vector< int32_t > slice_offsets, slice_seg_sizes;
+   std::vector< int32_t > perm, group_ptrs, inv_perm;
+   int32_t ch_nslices = 0, chunks_in_slice = 0, desired_chunk_size = 0;
+   std::vector< int32_t > seg_to_chunk, seg_to_slice, chunks_to_seg, seg_ptrs;
+   std::vector< int32_t > ch_slices;  // 4 per slice (size, chunkSize, firstSegment, pointer), n slices as TNL sizes it
+   std::vector< int32_t > blocks[ 6 ];  // AdaptiveCSR descriptors, 6 int32 each (type, first, size, segments, warp, warps)
+};
+
+// Copies the arrays of `seg` to the host in that layout.
+void segments_export( const tnlmi_segments& seg, SegmentsFileData& f, hipStream_t s );
+
+// Builds `seg` (format, organization, param and n set) from arrays read from a
+// file: checks that every index they define lies inside the storage, derives
+// the segment sizes as TNL's getSegmentSize does, and keeps the file's arrays
+// (the AdaptiveCSR descriptors are rebuilt for this device).  Throws
+// TNLMI_RUNTIME_ERROR on arrays that no TNL segments object can hold.
+void segments_import( tnlmi_segments& seg, const SegmentsFileData& f, hipStream_t s );
 
 // Exclusive scan of int32 values into int32 (n+1 outputs, out[n] = total);
 // returns the total as int64 (checked by the caller for int32 overflow).

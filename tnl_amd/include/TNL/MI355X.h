@@ -541,6 +541,52 @@ public:
    int getRowCapacity( int row ) const { return hoff_[ row + 1 ] - hoff_[ row ]; }
    int64_t getStorageSize() const { return segments_ ? tnlmi_segments_storage_size( segments_ ) : 0; }
 
+   //! getSerializationType (SparseMatrixBase.hpp:46-53)
+   static std::string getSerializationType()
+   {
+      tnlmi_segments* empty = nullptr;
+      detail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, nullptr, 0, nullptr,
+                                            &empty ) );
+      std::unique_ptr< tnlmi_segments, void ( * )( tnlmi_segments* ) > own( empty, tnlmi_segments_destroy );
+      const tnlmi_matrix m{ 0, 0, empty, detail::dtype< Real >(), nullptr, nullptr };
+      int64_t n = 0;
+      detail::check( tnlmi_matrix_serialization_type( &m, nullptr, 0, &n ) );
+      std::string t( (size_t) n + 1, '\0' );
+      detail::check( tnlmi_matrix_serialization_type( &m, t.data(), n + 1, &n ) );
+      t.resize( (size_t) n );
+      return t;
+   }
+
+   //! save( fileName ) (SparseMatrix.hpp:640-646): TNL's binary format
+   void save( const std::string& fileName ) const
+   {
+      const tnlmi_matrix m = abi();
+      detail::check( tnlmi_matrix_save( &m, fileName.c_str(), nullptr ) );
+   }
+
+   //! load( fileName ) (SparseMatrix.hpp:656-694): a file of this matrix type
+   //! (other types: "object type does not match"), its segments arrays as saved
+   void load( const std::string& fileName )
+   {
+      tnlmi_matrix_file* f = nullptr;
+      detail::check( tnlmi_matrix_file_read( fileName.c_str(), &f ) );
+      std::unique_ptr< tnlmi_matrix_file, void ( * )( tnlmi_matrix_file* ) > own( f, tnlmi_matrix_file_destroy );
+      const std::string found = tnlmi_matrix_file_type( f ), expected = getSerializationType();
+      if( found != expected )
+         throw std::runtime_error( "Failed to deserialize an object from the file '" + fileName
+                                   + "': object type does not match (expected " + expected + ", found " + found + ")." );
+      int64_t rows = 0, columns = 0, storage = 0;
+      detail::check( tnlmi_matrix_file_info( f, &rows, &columns, nullptr, nullptr, nullptr, nullptr, &storage ) );
+      release();
+      rows_ = (int) rows;
+      columns_ = (int) columns;
+      detail::check( tnlmi_matrix_file_segments( f, nullptr, &segments_ ) );
+      detail::hip( hipMalloc( (void**) &values_, std::max< int64_t >( 1, storage ) * sizeof( Real ) ) );
+      detail::hip( hipMalloc( (void**) &columnIndexes_, std::max< int64_t >( 1, storage ) * sizeof( int ) ) );
+      detail::check( tnlmi_matrix_file_upload( f, values_, columnIndexes_, nullptr ) );
+      mirror();
+   }
+
    //! The C ABI description of this matrix.
    tnlmi_matrix abi() const
    {
@@ -548,6 +594,41 @@ public:
    }
 
 private:
+   // the host mirror in CSR order (row capacities, then each row's slots) of
+   // the device storage, through the segments' global indexes
+   void mirror()
+   {
+      const int32_t* d_sizes = nullptr;
+      int64_t n = 0;
+      detail::check( tnlmi_segments_get_array( segments_, TNLMI_ARR_SIZES, &d_sizes, &n ) );
+      std::vector< int > sizes( rows_ );
+      if( rows_ > 0 )
+         detail::hip( hipMemcpy( sizes.data(), d_sizes, rows_ * sizeof( int ), hipMemcpyDeviceToHost ) );
+      hoff_.assign( rows_ + 1, 0 );
+      for( int r = 0; r < rows_; r++ )
+         hoff_[ r + 1 ] = hoff_[ r ] + sizes[ r ];
+      const int64_t elements = hoff_[ rows_ ], storage = tnlmi_segments_storage_size( segments_ );
+      std::vector< int > gidx( elements ), cols( storage );
+      std::vector< Real > vals( storage );
+      int* d_gidx = nullptr;
+      detail::hip( hipMalloc( (void**) &d_gidx, std::max< int64_t >( 1, elements ) * sizeof( int ) ) );
+      std::unique_ptr< int, void ( * )( int* ) > own( d_gidx, []( int* p ) { (void) hipFree( p ); } );
+      detail::check( tnlmi_segments_global_indexes( segments_, d_gidx, nullptr ) );
+      detail::hip( hipDeviceSynchronize() );
+      if( elements > 0 )
+         detail::hip( hipMemcpy( gidx.data(), d_gidx, elements * sizeof( int ), hipMemcpyDeviceToHost ) );
+      if( storage > 0 ) {
+         detail::hip( hipMemcpy( cols.data(), columnIndexes_, storage * sizeof( int ), hipMemcpyDeviceToHost ) );
+         detail::hip( hipMemcpy( vals.data(), values_, storage * sizeof( Real ), hipMemcpyDeviceToHost ) );
+      }
+      hcols_.resize( elements );
+      hvals_.resize( elements );
+      for( int64_t e = 0; e < elements; e++ ) {
+         hcols_[ e ] = cols[ gidx[ e ] ];
+         hvals_[ e ] = vals[ gidx[ e ] ];
+      }
+   }
+
    void release()
    {
       if( segments_ )
