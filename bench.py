@@ -614,8 +614,55 @@ def wl_gmres_convdiff(args):
     return out
 
 
+def wl_spmv_symmetric(args):
+    """FP64 SpMV of the 3D Poisson matrix 256^3 stored as a SymmetricMatrix
+    (lower triangle and diagonal: 66.9 M of the 117 M entries) in CSR and
+    Ellpack, beside the full matrix in the same formats."""
+    import numpy as np
+    import torch
+    import tnl_amd
+
+    off, cols, vals = tnl_amd.poisson3d(256, 256, 256)
+    n = len(off) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    keep = cols <= rows
+    loff = np.zeros(n + 1, np.int32)
+    loff[1:] = np.cumsum(np.bincount(rows[keep], minlength=n))
+    lcols, lvals = cols[keep], vals[keep]
+    del rows
+    xd = torch.from_numpy(tnl_amd.uniform(n, -1.0, 1.0, seed=0)).cuda()
+    yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    steps = max(10, args.steps // 4)
+    res = {}
+    for name, fmt, sym, (o, c, v) in (("CSR full", tnl_amd.CSR, False, (off, cols, vals)),
+                                      ("Ellpack full", tnl_amd.ELLPACK, False, (off, cols, vals)),
+                                      ("CSR symmetric", tnl_amd.CSR, True, (loff, lcols, lvals)),
+                                      ("Ellpack symmetric", tnl_amd.ELLPACK, True, (loff, lcols, lvals))):
+        A = tnl_amd.SparseMatrix.from_csr(n, n, o, c, v, fmt=fmt, symmetric=sym)
+        ms = timed(lambda: A.vectorProduct(xd, yd), steps, 5)
+        res[name] = {"ms": round(ms, 6), "stored_entries": int(len(c)),
+                     "GBs_of_stored_bytes": round(spmv_bytes(n, len(c)) / (ms * 1e-3) / 1e9, 3)}
+        if name == "CSR full":
+            y_full = yd.cpu().numpy()
+        else:
+            res[name]["max_rel_diff_vs_full"] = float(np.abs(yd.cpu().numpy() - y_full).max() / np.abs(y_full).max())
+        log(f"Poisson 256^3 {name}: {ms * 1e3:.1f} us")
+        del A
+    best = res["Ellpack symmetric"]
+    nbytes = spmv_bytes(n, len(lcols))
+    gbs = nbytes / (best["ms"] * 1e-3) / 1e9
+    out = base_record(args, round(gbs, 3), "GB/s", best["ms"], "f64", "synthetic (3D 7-point Poisson 256^3, U[-1,1] x)",
+                      {"workload": "FP64 SpMV, 3D Poisson 256^3 as a SymmetricMatrix (lower triangle + diagonal), "
+                                   "Ellpack; atomic adds of the mirrored terms", "rows": n,
+                       "stored_entries": int(len(lcols)), "bytes_per_spmv": nbytes, "variants": res})
+    out["roofline"] = roofline(gbs)
+    out["cpu_baseline"] = None
+    return out
+
+
 WORKLOADS = {"spmv-powerlaw": wl_spmv_powerlaw, "spmv-laplace": wl_spmv_laplace, "spmv-formats": wl_spmv_formats,
-             "cg-poisson": wl_cg_poisson, "batched-cg": wl_batched_cg, "gmres-convdiff": wl_gmres_convdiff}
+             "spmv-symmetric": wl_spmv_symmetric, "cg-poisson": wl_cg_poisson, "batched-cg": wl_batched_cg,
+             "gmres-convdiff": wl_gmres_convdiff}
 
 
 def main():

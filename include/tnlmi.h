@@ -125,9 +125,14 @@ tnlmi_status tnlmi_segments_global_indexes( const tnlmi_segments* segments, int3
 
 /* -------------------------------------------------------------- matrix */
 
-/* A general sparse matrix: Matrices/SparseMatrix.h:41-56 with
- * MatrixType = GeneralMatrix.  values / column_indexes have the segments'
- * storage size; column index -1 marks a padding slot (MatrixBase.h:27). */
+/* Matrices/MatrixType.h: GeneralMatrix / SymmetricMatrix. */
+typedef enum { TNLMI_GENERAL = 0, TNLMI_SYMMETRIC = 1 } tnlmi_matrix_type;
+
+/* A sparse matrix: Matrices/SparseMatrix.h:41-56.  values / column_indexes
+ * have the segments' storage size; column index -1 marks a padding slot
+ * (MatrixBase.h:27).  matrix_type TNLMI_SYMMETRIC: the storage holds the lower
+ * triangle with the diagonal of a symmetric matrix (SymmetricMatrix); a
+ * zero-initialised trailing field means a general matrix. */
 typedef struct {
    int64_t rows;
    int64_t columns;
@@ -135,6 +140,7 @@ typedef struct {
    tnlmi_dtype value_type;
    const void* values;
    const int32_t* column_indexes;
+   int32_t matrix_type; /* tnlmi_matrix_type */
 } tnlmi_matrix;
 
 /* Writes a CSR-ordered element list into the format storage: padding slots get
@@ -211,7 +217,7 @@ typedef struct tnlmi_matrix_file tnlmi_matrix_file;
 tnlmi_status tnlmi_matrix_file_read( const char* path, tnlmi_matrix_file** out );
 tnlmi_status tnlmi_matrix_file_info( const tnlmi_matrix_file* f, int64_t* rows, int64_t* columns,
                                      tnlmi_format* format, tnlmi_organization* organization, int32_t* param,
-                                     tnlmi_dtype* value_type, int64_t* storage );
+                                     tnlmi_dtype* value_type, int64_t* storage, tnlmi_matrix_type* matrix_type );
 const char* tnlmi_matrix_file_type( const tnlmi_matrix_file* f );
 tnlmi_status tnlmi_matrix_file_segments( const tnlmi_matrix_file* f, tnlmi_stream stream, tnlmi_segments** out );
 tnlmi_status tnlmi_matrix_file_upload( const tnlmi_matrix_file* f, void* d_values, int32_t* d_column_indexes,
@@ -242,6 +248,23 @@ typedef struct {
 tnlmi_status tnlmi_spmv( const tnlmi_matrix* A, tnlmi_dtype x_type, const void* d_x, void* d_y,
                          tnlmi_dtype compute_type, double alpha, double beta, int64_t begin, int64_t end,
                          const tnlmi_launch* launch, tnlmi_stream stream );
+/* A symmetric matrix (matrix_type TNLMI_SYMMETRIC) in tnlmi_spmv: the
+ * reference's symmetric branch (SparseMatrixBase.hpp:298-329): y = beta*y over
+ * the whole vector when beta != 1 (beta == 0: y = 0, where the reference's
+ * multiplication would keep a NaN already in y), then for every row r in [begin, end)
+ * y[r] += alpha * sum_k v_k x[c_k] and, for each stored c_k < r,
+ * y[c_k] += alpha * v_k * x[r].  The adds are atomic, so the summation order
+ * of a y entry is not fixed (as in the reference's GPU build); compute type =
+ * vector type; launch is ignored. */
+
+/* transposedVectorProduct (SparseMatrixBase.hpp:541-588): y = beta*y (whole
+ * vector, when beta != 1), then y[c] += alpha * x[r] * v for every stored
+ * element (r, c, v) with c in [begin, end) (end == 0: all columns).  x has
+ * `rows` entries, y `columns`.  Atomic adds.  A symmetric matrix gives
+ * tnlmi_spmv's product (the reference calls vectorProduct).  Types as
+ * tnlmi_spmv with compute type = vector type. */
+tnlmi_status tnlmi_spmv_transposed( const tnlmi_matrix* A, tnlmi_dtype x_type, const void* d_x, void* d_y,
+                                    double alpha, double beta, int64_t begin, int64_t end, tnlmi_stream stream );
 
 /* ------------------------------------------- column-split / windowed plan */
 /* An execution plan of y = alpha*A*x + beta*y for a CSR matrix whose x

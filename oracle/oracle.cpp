@@ -526,6 +526,55 @@ void orc_spmv_f64_omp( int64_t rows, const int32_t* row_ptr, const int32_t* cols
       y[ r ] = row_sum< double, double, double >( row_ptr, cols, vals, x, r );
 }
 
+// SparseMatrixBase.hpp:298-329, a SymmetricMatrix whose CSR arrays hold the
+// lower triangle and the diagonal: y *= beta (whole vector, beta != 1), then
+// per row r of [begin, end), in element order, the mirrored term
+// alpha * v * x[r] into y[c] for c < r and v * x[c] into the row's sum, which
+// adds alpha * sum to y[r] (the host order of the reference with one thread).
+void orc_spmv_symmetric( int64_t rows, const int32_t* row_ptr, const int32_t* cols, const double* vals,
+                         const double* x, double* y, double alpha, double beta, int64_t begin, int64_t end )
+{
+   if( end == 0 )
+      end = rows;
+   if( beta != 1.0 )
+      for( int64_t i = 0; i < rows; i++ )
+         y[ i ] *= beta;
+   for( int64_t r = begin; r < end; r++ ) {
+      double sum = 0;
+      for( int64_t g = row_ptr[ r ]; g < row_ptr[ r + 1 ]; g++ ) {
+         const int32_t c = cols[ g ];
+         if( c == -1 ) {
+            sum += 0.0;  // the padding fetch returns 0
+            continue;
+         }
+         if( c < r )
+            y[ c ] += alpha * vals[ g ] * x[ r ];
+         sum += vals[ g ] * x[ c ];
+      }
+      y[ r ] += alpha * sum;
+   }
+}
+
+// SparseMatrixBase.hpp:541-588 (general matrix): y *= beta (whole vector,
+// beta != 1), then y[c] += alpha * x[r] * v for every element with c in
+// [begin, end), rows in order.
+void orc_spmv_transposed( int64_t rows, int64_t columns, const int32_t* row_ptr, const int32_t* cols,
+                          const double* vals, const double* x, double* y, double alpha, double beta, int64_t begin,
+                          int64_t end )
+{
+   if( end == 0 )
+      end = columns;
+   if( beta != 1.0 )
+      for( int64_t i = 0; i < columns; i++ )
+         y[ i ] *= beta;
+   for( int64_t r = 0; r < rows; r++ )
+      for( int64_t g = row_ptr[ r ]; g < row_ptr[ r + 1 ]; g++ ) {
+         const int32_t c = cols[ g ];
+         if( c >= begin && c < end && c != -1 )
+            y[ c ] += alpha * x[ r ] * vals[ g ];
+      }
+}
+
 }  // extern "C"
 
 /* -------------------------------------------------------------- solvers */

@@ -110,6 +110,15 @@ void orc_spmv_f32( int64_t rows, const int32_t* row_ptr, const int32_t* cols, co
  * per-row arithmetic, so the result is bit-identical. */
 void orc_spmv_f64_omp( int64_t rows, const int32_t* row_ptr, const int32_t* cols, const double* vals,
                        const double* x, double* y, int threads );
+/* SparseMatrixBase.hpp:298-329: the product of a SymmetricMatrix whose CSR
+ * arrays hold the lower triangle with the diagonal. */
+void orc_spmv_symmetric( int64_t rows, const int32_t* row_ptr, const int32_t* cols, const double* vals,
+                         const double* x, double* y, double alpha, double beta, int64_t begin, int64_t end );
+/* SparseMatrixBase.hpp:541-588: transposedVectorProduct of a general matrix
+ * (x has rows entries, y columns; [begin, end) ranges over columns). */
+void orc_spmv_transposed( int64_t rows, int64_t columns, const int32_t* row_ptr, const int32_t* cols,
+                          const double* vals, const double* x, double* y, double alpha, double beta, int64_t begin,
+                          int64_t end );
 
 /* ----------------------------------------------------------------- solvers */
 

@@ -175,6 +175,55 @@ struct SpmvOp
 };
 
 template< template< class, class, class > class S >
+using SymMat = Matrices::SparseMatrix< double, Host, int, Matrices::SymmetricMatrix, S >;
+
+// a SymmetricMatrix built from lower-triangle CSR arrays, vectorProduct
+template< template< class, class, class > class S >
+struct SymSpmvOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v, const double* x,
+                       double* y, double a, double b, int64_t be, int64_t en )
+   {
+      spmv_impl< SymMat< S > >( n, ncols, rp, c, v, x, y, a, b, be, en );
+      return 0;
+   }
+};
+
+// transposedVectorProduct of a general matrix
+template< template< class, class, class > class S >
+struct TransSpmvOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v, const double* x,
+                       double* y, double a, double b, int64_t be, int64_t en )
+   {
+      Mat< S > m;
+      build( m, n, ncols, rp, c, v );
+      DVec xv( n ), yv( ncols );
+      for( int64_t i = 0; i < n; i++ )
+         xv[ i ] = x[ i ];
+      for( int64_t i = 0; i < ncols; i++ )
+         yv[ i ] = y[ i ];
+      m.transposedVectorProduct( xv, yv, a, b, be, en );
+      for( int64_t i = 0; i < ncols; i++ )
+         y[ i ] = yv[ i ];
+      return 0;
+   }
+};
+
+template< template< class, class, class > class S >
+struct SymSaveOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v,
+                       const char* path )
+   {
+      SymMat< S > m;
+      build( m, n, ncols, rp, c, v );
+      m.save( path );
+      return 0;
+   }
+};
+
+template< template< class, class, class > class S >
 struct SaveOp
 {
    static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v,
@@ -592,6 +641,34 @@ int64_t ref_load( int format, int col_major, const char* path, int64_t* dims, in
 {
    try {
       return dispatch< LoadOp >( format, col_major, path, dims, out_cols, out_vals, cap );
+   }
+   catch( const std::exception& e ) {
+      g_err = e.what();
+      return -1;
+   }
+}
+
+// SparseMatrixBase.hpp:298-329 / :541-588 through TNL's own classes
+int64_t ref_spmv_symmetric( int format, int col_major, int64_t n, int64_t ncols, const int32_t* row_ptr,
+                            const int32_t* cols, const double* vals, const double* x, double* y, double alpha,
+                            double beta, int64_t begin, int64_t end )
+{
+   return dispatch< SymSpmvOp >( format, col_major, n, ncols, row_ptr, cols, vals, x, y, alpha, beta, begin, end );
+}
+
+int64_t ref_spmv_transposed( int format, int col_major, int64_t n, int64_t ncols, const int32_t* row_ptr,
+                             const int32_t* cols, const double* vals, const double* x, double* y, double alpha,
+                             double beta, int64_t begin, int64_t end )
+{
+   return dispatch< TransSpmvOp >( format, col_major, n, ncols, row_ptr, cols, vals, x, y, alpha, beta, begin, end );
+}
+
+// SparseMatrix::save of a SymmetricMatrix built from lower-triangle CSR arrays
+int64_t ref_save_symmetric( int format, int col_major, int64_t n, int64_t ncols, const int32_t* row_ptr,
+                            const int32_t* cols, const double* vals, const char* path )
+{
+   try {
+      return dispatch< SymSaveOp >( format, col_major, n, ncols, row_ptr, cols, vals, path );
    }
    catch( const std::exception& e ) {
       g_err = e.what();

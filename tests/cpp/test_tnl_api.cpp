@@ -264,6 +264,40 @@ int main()
       std::cout << "SparseMatrix save / load (BiEllpack) done\n";
    }
 
+   // SymmetricMatrix: the same tridiagonal matrix from its complete element
+   // list (the upper entries checked against the lower, then dropped), the
+   // same product and the CG example; transposedVectorProduct of A is A's
+   // product (A is symmetric)
+   {
+      using SymMatrix = Matrices::SparseMatrix< double, Device, int, Matrices::SymmetricMatrix >;
+      auto S = std::make_shared< SymMatrix >(
+         5, 5,
+         std::initializer_list< std::tuple< int, int, double > >{
+            { 0, 0, 2.5 }, { 0, 1, -1 }, { 1, 0, -1 }, { 1, 1, 2.5 }, { 1, 2, -1 }, { 2, 1, -1 }, { 2, 2, 2.5 },
+            { 2, 3, -1 }, { 3, 2, -1 }, { 3, 3, 2.5 }, { 3, 4, -1 }, { 4, 3, -1 }, { 4, 4, 2.5 } } );
+      EXPECT_EQ( S->getRowCapacity( 0 ), 1 );
+      EXPECT_EQ( S->getRowCapacity( 2 ), 2 );
+      EXPECT_EQ( S->getElement( 2, 3 ), -1.0 );
+      Vector x( 5 ), y( 5 ), z( 5 ), w( 5 );
+      for( int i = 0; i < 5; i++ )
+         x.setElement( i, 1.0 + i );
+      S->vectorProduct( x, y );
+      A->vectorProduct( x, z );
+      A->transposedVectorProduct( x, w );
+      EXPECT_EQ( y == z, true );
+      EXPECT_EQ( w == z, true );
+      solveExample< Solvers::Linear::CG< SymMatrix > >( S, "CG on SymmetricMatrix" );
+      bool thrown = false;
+      try {
+         SymMatrix bad( 2, 2, std::initializer_list< std::tuple< int, int, double > >{ { 0, 1, 1.0 }, { 1, 0, 2.0 } } );
+      }
+      catch( const std::logic_error& ) {
+         thrown = true;
+      }
+      EXPECT_EQ( thrown, true );
+      std::cout << "SymmetricMatrix product, CG and transposedVectorProduct done\n";
+   }
+
    // error behaviour of the reference
    {
       Vector in( 3, 1.0 ), out( 5 );

@@ -153,6 +153,22 @@ class Oracle:
                                 ctypes.c_float(beta), I64(begin), I64(end))
         return y
 
+    def spmv_symmetric(self, off, cols, vals, x, y=None, alpha=1.0, beta=0.0, begin=0, end=0):
+        off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
+        n = len(off) - 1
+        y = np.zeros(n) if y is None else f64(y).copy()
+        self.L.orc_spmv_symmetric(I64(n), P(off), P(cols), P(vals), P(x), P(y), ctypes.c_double(alpha),
+                                  ctypes.c_double(beta), I64(begin), I64(end))
+        return y
+
+    def spmv_transposed(self, off, cols, vals, ncols, x, y=None, alpha=1.0, beta=0.0, begin=0, end=0):
+        off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
+        n = len(off) - 1
+        y = np.zeros(ncols) if y is None else f64(y).copy()
+        self.L.orc_spmv_transposed(I64(n), I64(ncols), P(off), P(cols), P(vals), P(x), P(y), ctypes.c_double(alpha),
+                                   ctypes.c_double(beta), I64(begin), I64(end))
+        return y
+
     def spmv_omp(self, off, cols, vals, x, threads):
         off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
         n = len(off) - 1
@@ -236,7 +252,7 @@ class Reference:
         r = cls()
         r.L = ctypes.CDLL(REF_SO)
         for name in ["ref_chunked_ellpack", "ref_adaptive_csr_blocks", "ref_fill", "ref_spmv", "ref_save",
-                     "ref_load"]:
+                     "ref_load", "ref_spmv_symmetric", "ref_spmv_transposed", "ref_save_symmetric"]:
             getattr(r.L, name).restype = ctypes.c_int64
         r.L.ref_last_error.restype = ctypes.c_char_p
         r.L.ref_spmv_csr_time.restype = ctypes.c_double
@@ -337,6 +353,31 @@ class Reference:
         self.L.ref_spmv(self.FORMATS[fmt], int(col_major), I64(n), I64(len(x)), P(off), P(cols), P(vals), P(x),
                         P(y), ctypes.c_double(alpha), ctypes.c_double(beta), I64(begin), I64(end))
         return y
+
+    def spmv_symmetric(self, fmt, off, cols, vals, x, y=None, alpha=1.0, beta=0.0, begin=0, end=0, col_major=True):
+        """vectorProduct of TNL's SymmetricMatrix built from lower-triangle CSR arrays."""
+        off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
+        n = len(off) - 1
+        y = np.zeros(n) if y is None else f64(y).copy()
+        self.L.ref_spmv_symmetric(self.FORMATS[fmt], int(col_major), I64(n), I64(len(x)), P(off), P(cols), P(vals),
+                                  P(x), P(y), ctypes.c_double(alpha), ctypes.c_double(beta), I64(begin), I64(end))
+        return y
+
+    def spmv_transposed(self, fmt, off, cols, vals, ncols, x, y=None, alpha=1.0, beta=0.0, begin=0, end=0,
+                        col_major=True):
+        off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
+        n = len(off) - 1
+        y = np.zeros(ncols) if y is None else f64(y).copy()
+        self.L.ref_spmv_transposed(self.FORMATS[fmt], int(col_major), I64(n), I64(ncols), P(off), P(cols), P(vals),
+                                   P(x), P(y), ctypes.c_double(alpha), ctypes.c_double(beta), I64(begin), I64(end))
+        return y
+
+    def save_symmetric_file(self, fmt, off, cols, vals, ncols, path, col_major=True):
+        off, cols, vals = i32(off), i32(cols), f64(vals)
+        rc = self.L.ref_save_symmetric(self.FORMATS[fmt], int(col_major), I64(len(off) - 1), I64(ncols), P(off),
+                                       P(cols), P(vals), os.fsencode(path))
+        if rc < 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
 
     def spmv_f32(self, off, cols, vals, x):
         off, cols = i32(off), i32(cols)

@@ -66,11 +66,12 @@ class FileInfo:
         self.h = ctypes.c_void_p()
         _lib.check(L.tnlmi_matrix_file_read(os.fsencode(path), ctypes.byref(self.h)))
         rows, cols, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        fmt, org, vt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        fmt, org, vt, mt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         param = ctypes.c_int32()
         _lib.check(L.tnlmi_matrix_file_info(self.h, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(fmt),
                                             ctypes.byref(org), ctypes.byref(param), ctypes.byref(vt),
-                                            ctypes.byref(st)))
+                                            ctypes.byref(st), ctypes.byref(mt)))
+        self.symmetric = mt.value == 1
         self.type = L.tnlmi_matrix_file_type(self.h).decode()
         L.tnlmi_matrix_file_destroy(self.h)
         self.rows, self.columns, self.format, self.org = rows.value, cols.value, fmt.value, org.value
@@ -130,10 +131,10 @@ def test_read_errors(tmp_path):
     bad.write_bytes(bytes(arr))
     st, msg = read_error(bad)
     assert st == _lib.RUNTIME_ERROR and "column index" in msg
-    # a type this library cannot hold (a symmetric matrix)
+    # a type this library cannot hold (64-bit indexes)
     t0 = data.index(b"Matrices::SparseMatrix")
     tl = struct.unpack("<i", data[5:9])[0]
-    t = data[t0:t0 + tl].replace(b"General", b"Symmetric")
+    t = data[t0:t0 + tl].replace(b"[any_device], int,", b"[any_device], long int,")
     bad.write_bytes(data[:5] + struct.pack("<i", len(t)) + t + data[t0 + tl:])
     st, msg = read_error(bad)
     assert st == _lib.NOT_IMPLEMENTED and "unsupported object type" in msg

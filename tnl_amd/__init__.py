@@ -166,29 +166,42 @@ def _d2h(ptr, count, dtype):
 
 
 class SparseMatrix:
-    """Matrices::SparseMatrix<Real, GPU, int, GeneralMatrix, Segments> over device tensors."""
+    """Matrices::SparseMatrix<Real, GPU, int, MatrixType, Segments> over device tensors.
+    symmetric=True is SymmetricMatrix: the storage holds the lower triangle and
+    the diagonal."""
 
-    def __init__(self, rows, columns, segments, values, column_indexes):
+    def __init__(self, rows, columns, segments, values, column_indexes, symmetric=False):
         self.rows = int(rows)
         self.columns = int(columns)
         self.segments = segments
         self.values = values
         self.column_indexes = column_indexes
+        self.symmetric = bool(symmetric)
         self._m = _lib.Matrix(self.rows, self.columns, segments.handle.value, dtype_code(values.dtype),
-                              values.data_ptr(), column_indexes.data_ptr())
+                              values.data_ptr(), column_indexes.data_ptr(), int(self.symmetric))
 
     @classmethod
     def from_csr(cls, rows, columns, offsets, cols, vals, fmt=CSR, value_dtype=None, organization=COLUMN_MAJOR,
-                 param=0, capacities=None):
+                 param=0, capacities=None, symmetric=False):
         """Assemble from CSR arrays (any device): row r holds cols/vals[offsets[r]:offsets[r+1]].
 
         capacities (optional) are the row capacities (setRowCapacities);
-        by default the CSR row lengths.
+        by default the CSR row lengths.  symmetric=True: a SymmetricMatrix from
+        its lower triangle with the diagonal (setElements with the
+        SymmetricLower encoding, SparseMatrix.hpp:437-438: an entry above the
+        diagonal is an error).
         """
         torch = _torch()
         offsets = torch.as_tensor(offsets, dtype=torch.int32).cuda().contiguous()
         cols = torch.as_tensor(cols, dtype=torch.int32).cuda().contiguous()
         vals = torch.as_tensor(vals).cuda().contiguous()
+        if symmetric:
+            if rows != columns:
+                raise InvalidArgument(_lib.INVALID_ARGUMENT, "a symmetric matrix must be square")
+            lens = (offsets[1:] - offsets[:-1]).long()
+            row_of = torch.repeat_interleave(torch.arange(rows, device="cuda", dtype=torch.int32), lens)
+            if bool((cols > row_of).any()):
+                raise LogicError(_lib.LOGIC_ERROR, "Only lower part of the symmetric matrix is expected.")
         if capacities is None:
             capacities = offsets[1:] - offsets[:-1]
         seg = Segments(fmt, capacities, organization, param)
@@ -202,7 +215,7 @@ class SparseMatrix:
                                                 ctypes.c_void_p(cols.data_ptr()), ctypes.c_void_p(vals.data_ptr()),
                                                 dtype_code(vals.dtype), ctypes.c_void_p(values.data_ptr()),
                                                 dtype_code(vdt), ctypes.c_void_p(colidx.data_ptr()), _stream()))
-        return cls(rows, columns, seg, values, colidx)
+        return cls(rows, columns, seg, values, colidx, symmetric=symmetric)
 
     def rowLengths(self):
         """The leading entries of each row with a column below getColumns() (padding
@@ -223,7 +236,7 @@ class SparseMatrix:
         storage = max(1, seg.getStorageSize())
         values = torch.empty(storage, dtype=self.values.dtype, device="cuda")
         colidx = torch.empty(storage, dtype=torch.int32, device="cuda")
-        B = SparseMatrix(self.rows, self.columns, seg, values, colidx)
+        B = SparseMatrix(self.rows, self.columns, seg, values, colidx, symmetric=self.symmetric)
         _lib.check(_lib.lib().tnlmi_matrix_copy(ctypes.byref(self._m), ctypes.byref(B._m),
                                                 ctypes.c_void_p(lengths.data_ptr()), _stream()))
         return B
@@ -265,11 +278,11 @@ class SparseMatrix:
                                                           f"object type does not match (expected {expected_type}, "
                                                           f"found {found}).")
             rows, cols, storage = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-            fmt, org, vt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            fmt, org, vt, mt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
             param = ctypes.c_int32()
             _lib.check(L.tnlmi_matrix_file_info(f, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(fmt),
                                                 ctypes.byref(org), ctypes.byref(param), ctypes.byref(vt),
-                                                ctypes.byref(storage)))
+                                                ctypes.byref(storage), ctypes.byref(mt)))
             h = ctypes.c_void_p()
             _lib.check(L.tnlmi_matrix_file_segments(f, _stream(), ctypes.byref(h)))
             seg = Segments._adopt(h, fmt.value, org.value)
@@ -280,7 +293,22 @@ class SparseMatrix:
                                                   ctypes.c_void_p(colidx.data_ptr()), _stream()))
         finally:
             L.tnlmi_matrix_file_destroy(f)
-        return cls(rows.value, cols.value, seg, values, colidx)
+        return cls(rows.value, cols.value, seg, values, colidx, symmetric=mt.value == 1)
+
+    def transposedVectorProduct(self, x, y, alpha=1.0, beta=0.0, begin=0, end=0, stream=None):
+        """y = alpha*A^T*x + beta*y over columns [begin, end) (SparseMatrixBase::transposedVectorProduct)."""
+        if x.numel() != self.rows:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "transposedVectorProduct: size of the input vector does not "
+                                                         "match the number of matrix rows")
+        if y.numel() != self.columns:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "transposedVectorProduct: size of the output vector does not "
+                                                         "match the number of matrix columns")
+        if x.dtype != y.dtype:
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "transposedVectorProduct: x and y must have the same dtype")
+        _lib.check(_lib.lib().tnlmi_spmv_transposed(ctypes.byref(self._m), dtype_code(x.dtype),
+                                                    ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                                    float(alpha), float(beta), int(begin), int(end), _stream(stream)))
+        return y
 
     def getRows(self):
         return self.rows

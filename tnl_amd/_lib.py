@@ -90,6 +90,7 @@ class Matrix(ctypes.Structure):
         ("value_type", ctypes.c_int),
         ("values", ctypes.c_void_p),
         ("column_indexes", ctypes.c_void_p),
+        ("matrix_type", ctypes.c_int32),
     ]
 
 
@@ -176,13 +177,15 @@ _PROTOS = {
                                           ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     "tnlmi_host_csr_destroy": (None, [c_vp]),
     "tnlmi_matrix_save": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_char_p, c_vp]),
+    "tnlmi_spmv_transposed": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int, c_vp, c_vp, ctypes.c_double,
+                                             ctypes.c_double, c_i64, c_i64, c_vp]),
     "tnlmi_matrix_serialization_type": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_char_p, c_i64,
                                                        ctypes.POINTER(c_i64)]),
     "tnlmi_matrix_file_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
     "tnlmi_matrix_file_info": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int),
-                                              ctypes.POINTER(c_i64)]),
+                                              ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_int)]),
     "tnlmi_matrix_file_type": (ctypes.c_char_p, [c_vp]),
     "tnlmi_matrix_file_segments": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_vp)]),
     "tnlmi_matrix_file_upload": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),

@@ -329,6 +329,8 @@ tnlmi_status tnlmi_batched_cg_create( const tnlmi_matrix* A, const tnlmi_matrix*
       *out = nullptr;
       if( A->value_type != TNLMI_F64 )
          fail( TNLMI_INVALID_ARGUMENT, "batched_cg_create: the matrix must hold FP64 values" );
+      if( A->matrix_type == TNLMI_SYMMETRIC || ( A_lo && A_lo->matrix_type == TNLMI_SYMMETRIC ) )
+         fail( TNLMI_NOT_IMPLEMENTED, "batched_cg_create: symmetric storage is not supported by the batched kernel" );
       if( A_lo && ( A_lo->value_type != TNLMI_F32 || A_lo->segments != A->segments ) )
          fail( TNLMI_INVALID_ARGUMENT, "batched_cg_create: the inner matrix must be an FP32 copy on the same segments" );
       if( sys_rows[ 0 ] != 0 || sys_rows[ nsys ] != A->rows )

@@ -116,6 +116,8 @@ tnlmi_status tnlmi_matrix_copy( const tnlmi_matrix* src, tnlmi_matrix* dst, cons
          fail( TNLMI_INVALID_ARGUMENT, "matrix_copy: null argument" );
       if( src->value_type != dst->value_type )
          fail( TNLMI_INVALID_ARGUMENT, "matrix_copy: the matrices must have the same RealType" );
+      if( src->matrix_type != dst->matrix_type )
+         fail( TNLMI_NOT_IMPLEMENTED, "matrix_copy: copies between general and symmetric matrices are not supported" );
       if( src->rows != dst->rows || src->columns != dst->columns || src->rows != src->segments->n
           || dst->rows != dst->segments->n )
          fail( TNLMI_INVALID_ARGUMENT, "matrix_copy: the matrices' dimensions or segments differ" );

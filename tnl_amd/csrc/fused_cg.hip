@@ -749,7 +749,7 @@ Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_
    if( plan )
       return S;
    const int fmt = S.v.format;
-   if( ( fmt == TNLMI_ELLPACK || fmt == TNLMI_SLICED_ELLPACK ) && S.v.col_major ) {
+   if( ( fmt == TNLMI_ELLPACK || fmt == TNLMI_SLICED_ELLPACK ) && S.v.col_major && A->matrix_type != TNLMI_SYMMETRIC ) {
       S.kind = 1;
       // 16-byte loads of the column indexes for 4-byte values, of the values for FP64
       S.rows_per_thread = A->value_type == TNLMI_F64 ? 2 : 4;

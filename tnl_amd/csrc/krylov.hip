@@ -1034,8 +1034,11 @@ tnlmi_status tnlmi_preconditioner_create( tnlmi_precond_kind kind, const tnlmi_m
             TNLMI_HIP( hipGetLastError() );
             TNLMI_HIP( hipStreamSynchronize( s ) );
          }
-         else if( kind == TNLMI_PRECOND_ILU0 )
+         else if( kind == TNLMI_PRECOND_ILU0 ) {
+            if( A->matrix_type == TNLMI_SYMMETRIC )
+               fail( TNLMI_NOT_IMPLEMENTED, "ILU0: a general matrix is needed (the factorization reads both triangles)" );
             ilu0_build( *P, *A, s );
+         }
          else if( kind == TNLMI_PRECOND_EXPLICIT )
             P->M = *A;
          else if( kind != TNLMI_PRECOND_NONE )

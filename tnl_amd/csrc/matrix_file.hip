@@ -34,6 +34,7 @@ struct tnlmi_matrix_file
    tnlmi_organization org = TNLMI_COLUMN_MAJOR;
    int32_t param = 0;
    tnlmi_dtype value_type = TNLMI_F64;
+   tnlmi_matrix_type matrix_type = TNLMI_GENERAL;
    tnlmi::SegmentsFileData seg;
    std::vector< char > values;
    std::vector< int32_t > cols;
@@ -80,11 +81,12 @@ std::string value_name( tnlmi_dtype t )
    return t == TNLMI_F64 ? "double" : "float";
 }
 
-// SparseMatrixBase::getSerializationType (SparseMatrixBase.hpp:46-53), General matrices
-std::string matrix_type( tnlmi_dtype t, tnlmi_format f, tnlmi_organization o, int32_t param )
+// SparseMatrixBase::getSerializationType (SparseMatrixBase.hpp:46-53;
+// MatrixType.h: "General" / "Symmetric")
+std::string matrix_type( tnlmi_dtype t, tnlmi_format f, tnlmi_organization o, int32_t param, tnlmi_matrix_type mt )
 {
-   return "Matrices::SparseMatrix< " + value_name( t ) + ", " + segments_type( f, o, param )
-        + ", [any_device], int, General, [any_allocator], [any_allocator] >";
+   return "Matrices::SparseMatrix< " + value_name( t ) + ", " + segments_type( f, o, param ) + ", [any_device], int, "
+        + ( mt == TNLMI_SYMMETRIC ? "Symmetric" : "General" ) + ", [any_allocator], [any_allocator] >";
 }
 
 std::string array_type( const std::string& element )
@@ -205,6 +207,7 @@ struct Reader
 // string (every type this library can hold; others are refused)
 bool parse_matrix_type( const std::string& t, tnlmi_matrix_file& m )
 {
+   for( tnlmi_matrix_type mt : { TNLMI_GENERAL, TNLMI_SYMMETRIC } )
    for( tnlmi_dtype v : { TNLMI_F64, TNLMI_F32 } )
       for( int f = TNLMI_CSR; f <= TNLMI_SORTED_CSR; f++ )
          for( tnlmi_organization o : { TNLMI_COLUMN_MAJOR, TNLMI_ROW_MAJOR } ) {
@@ -217,7 +220,8 @@ bool parse_matrix_type( const std::string& t, tnlmi_matrix_file& m )
             else if( f == TNLMI_BIELLPACK )
                params = { 32, 64 };
             for( int32_t p : params )
-               if( t == matrix_type( v, (tnlmi_format) f, o, p ) ) {
+               if( t == matrix_type( v, (tnlmi_format) f, o, p, mt ) ) {
+                  m.matrix_type = mt;
                   m.value_type = v;
                   m.format = (tnlmi_format) f;
                   m.org = o;
@@ -372,7 +376,7 @@ tnlmi_status tnlmi_matrix_serialization_type( const tnlmi_matrix* A, char* buffe
       if( A->value_type == TNLMI_BF16 )
          fail( TNLMI_NOT_IMPLEMENTED, "matrix serialization: bf16 values have no TNL serialization type" );
       const tnlmi_segments& s = *A->segments;
-      const std::string t = matrix_type( A->value_type, s.format, s.org, s.param );
+      const std::string t = matrix_type( A->value_type, s.format, s.org, s.param, (tnlmi_matrix_type) A->matrix_type );
       if( length )
          *length = (int64_t) t.size();
       if( buffer && capacity > 0 ) {
@@ -408,7 +412,7 @@ tnlmi_status tnlmi_matrix_save( const tnlmi_matrix* A, const char* path, tnlmi_s
       }
       TNLMI_HIP( hipStreamSynchronize( s ) );
       Writer w( path );
-      w.object_type( matrix_type( A->value_type, seg.format, seg.org, seg.param ) );
+      w.object_type( matrix_type( A->value_type, seg.format, seg.org, seg.param, (tnlmi_matrix_type) A->matrix_type ) );
       w.scalar< uint64_t >( (uint64_t) A->rows );
       w.scalar< uint64_t >( (uint64_t) A->columns );
       save_segments( w, seg.format, seg.n, f );
@@ -455,7 +459,7 @@ tnlmi_status tnlmi_matrix_file_read( const char* path, tnlmi_matrix_file** out )
 
 tnlmi_status tnlmi_matrix_file_info( const tnlmi_matrix_file* m, int64_t* rows, int64_t* columns,
                                      tnlmi_format* format, tnlmi_organization* organization, int32_t* param,
-                                     tnlmi_dtype* value_type, int64_t* storage )
+                                     tnlmi_dtype* value_type, int64_t* storage, tnlmi_matrix_type* matrix_type )
 {
    return guarded( [ & ] {
       if( ! m )
@@ -474,6 +478,8 @@ tnlmi_status tnlmi_matrix_file_info( const tnlmi_matrix_file* m, int64_t* rows, 
          *value_type = m->value_type;
       if( storage )
          *storage = m->seg.storage;
+      if( matrix_type )
+         *matrix_type = m->matrix_type;
    } );
 }
 

@@ -570,6 +570,8 @@ void check_csr( const tnlmi_matrix* A )
       fail( TNLMI_INVALID_ARGUMENT, "spai: NULL matrix" );
    if( A->segments->format != TNLMI_CSR || A->value_type != TNLMI_F64 )
       fail( TNLMI_INVALID_ARGUMENT, "spai: the matrix must be CSR with FP64 values" );
+   if( A->matrix_type == TNLMI_SYMMETRIC )
+      fail( TNLMI_NOT_IMPLEMENTED, "spai: a general matrix is needed (the network reads both triangles' entries)" );
    if( A->rows != A->columns || A->rows <= 0 )
       fail( TNLMI_INVALID_ARGUMENT, "spai: the matrix must be square and not empty" );
 }

@@ -654,6 +654,10 @@ void spmv_typed( const tnlmi_matrix& A, const void* x, void* y, double alpha, do
 void spmv_dispatch( const tnlmi_matrix& A, tnlmi_dtype xt, const void* x, void* y, tnlmi_dtype ct, double alpha,
                     double beta, int64_t begin, int64_t end, const tnlmi_launch& cfg, hipStream_t s )
 {
+   if( A.matrix_type == TNLMI_SYMMETRIC ) {
+      spmv_symmetric_dispatch( A, xt, x, y, ct, alpha, beta, begin, end, s );
+      return;
+   }
    const tnlmi_dtype vt = A.value_type;
    if( vt == TNLMI_F64 && xt == TNLMI_F64 && ct == TNLMI_F64 )
       spmv_typed< double, double, double >( A, x, y, alpha, beta, begin, end, cfg, s );
@@ -686,9 +690,9 @@ extern "C" tnlmi_status tnlmi_spmv( const tnlmi_matrix* A, tnlmi_dtype x_type, c
          end = A->rows;  // SparseMatrixBase.hpp:297-298
       if( begin < 0 || end > A->rows || begin > end )
          fail( TNLMI_INVALID_ARGUMENT, "vectorProduct: row range [begin, end) is out of bounds" );
-      if( end == begin )
-         return;
-      if( ( A->columns && ! d_x ) || ! d_y )
+      if( end == begin && A->matrix_type != TNLMI_SYMMETRIC )
+         return;  // (a symmetric product still scales y by beta)
+      if( ( A->columns && ! d_x ) || ( A->rows && ! d_y ) )
          fail( TNLMI_INVALID_ARGUMENT, "vectorProduct: vector pointer is NULL" );
       if( A->segments->storage && ( ! A->values || ! A->column_indexes ) )
          fail( TNLMI_INVALID_ARGUMENT, "vectorProduct: matrix storage pointer is NULL" );

@@ -288,6 +288,8 @@ tnlmi_status tnlmi_spmv_plan_create( const tnlmi_matrix* A, int32_t parts, tnlmi
       *out = nullptr;
       if( A->segments->format != TNLMI_CSR )
          fail( TNLMI_INVALID_ARGUMENT, "spmv_plan_create: the column-split plan is built from a CSR matrix" );
+      if( A->matrix_type == TNLMI_SYMMETRIC )
+         fail( TNLMI_NOT_IMPLEMENTED, "spmv_plan_create: the plans serve general matrices" );
       if( parts != 0 && parts != 8 && parts != 16 )
          fail( TNLMI_INVALID_ARGUMENT, "spmv_plan_create: parts must be 0 (windowed plan), 8 or 16" );
       if( A->rows != A->segments->n )

@@ -378,6 +378,11 @@ struct GeneralMatrix
 {
    static constexpr bool isSymmetric() { return false; }
 };
+//! MatrixType.h: the storage holds the lower triangle and the diagonal
+struct SymmetricMatrix
+{
+   static constexpr bool isSymmetric() { return true; }
+};
 
 template< typename Real, typename Device, typename Index = int, typename MatrixType = GeneralMatrix,
           template< typename, typename, typename > class Segments = Algorithms::Segments::CSR,
@@ -474,13 +479,25 @@ public:
    }
    void setRowCapacities( const RowCapacitiesType& capacities ) { setRowCapacities( capacities.toHost() ); }
 
-   //! setElements( map ) (SparseMatrix.hpp:380-470): capacities = entries per row
+   //! setElements( map ) (SparseMatrix.hpp:380-470, Complete encoding):
+   //! capacities = entries per row; a symmetric matrix keeps the entries on
+   //! and below the diagonal and checks that the others mirror them
    void setElements( const std::map< std::pair< int, int >, Real >& m )
    {
       std::vector< int > caps( rows_, 0 );
       for( const auto& [ rc, v ] : m ) {
          if( rc.first < 0 || rc.first >= rows_ || rc.second < 0 || rc.second >= columns_ )
             throw std::invalid_argument( "setElements: element out of the matrix" );
+         if constexpr( MatrixType::isSymmetric() ) {
+            const auto q = m.find( { rc.second, rc.first } );
+            if( q != m.end() && q->second != v )
+               throw std::logic_error( "SparseMatrix is configured as symmetric, but the input data is not symmetric. "
+                                       "The matrix elements at position ("
+                                       + std::to_string( rc.first ) + ", " + std::to_string( rc.second )
+                                       + ") do not match." );
+            if( rc.first < rc.second )
+               continue;
+         }
          caps[ rc.first ]++;
       }
       std::vector< int > off( rows_ + 1, 0 );
@@ -489,6 +506,8 @@ public:
       std::vector< int > cols( off[ rows_ ] ), pos( off.begin(), off.end() - 1 );
       std::vector< Real > vals( off[ rows_ ] );
       for( const auto& [ rc, v ] : m ) {  // std::map is ordered: columns ascend within a row
+         if( MatrixType::isSymmetric() && rc.first < rc.second )
+            continue;
          cols[ pos[ rc.first ] ] = rc.second;
          vals[ pos[ rc.first ]++ ] = v;
       }
@@ -533,6 +552,8 @@ public:
    //! getElement( row, column ) (SparseMatrixBase.hpp:231-265), 0 if absent
    Real getElement( int row, int column ) const
    {
+      if( MatrixType::isSymmetric() && row < column )  // SparseMatrixBase.hpp:246-249
+         std::swap( row, column );
       for( int g = hoff_[ row ]; g < hoff_[ row + 1 ]; g++ )
          if( hcols_[ g ] == column )
             return hvals_[ g ];
@@ -548,7 +569,7 @@ public:
       detail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, nullptr, 0, nullptr,
                                             &empty ) );
       std::unique_ptr< tnlmi_segments, void ( * )( tnlmi_segments* ) > own( empty, tnlmi_segments_destroy );
-      const tnlmi_matrix m{ 0, 0, empty, detail::dtype< Real >(), nullptr, nullptr };
+      const tnlmi_matrix m{ 0, 0, empty, detail::dtype< Real >(), nullptr, nullptr, matrixType() };
       int64_t n = 0;
       detail::check( tnlmi_matrix_serialization_type( &m, nullptr, 0, &n ) );
       std::string t( (size_t) n + 1, '\0' );
@@ -576,7 +597,8 @@ public:
          throw std::runtime_error( "Failed to deserialize an object from the file '" + fileName
                                    + "': object type does not match (expected " + expected + ", found " + found + ")." );
       int64_t rows = 0, columns = 0, storage = 0;
-      detail::check( tnlmi_matrix_file_info( f, &rows, &columns, nullptr, nullptr, nullptr, nullptr, &storage ) );
+      detail::check( tnlmi_matrix_file_info( f, &rows, &columns, nullptr, nullptr, nullptr, nullptr, &storage,
+                                                     nullptr ) );
       release();
       rows_ = (int) rows;
       columns_ = (int) columns;
@@ -590,7 +612,27 @@ public:
    //! The C ABI description of this matrix.
    tnlmi_matrix abi() const
    {
-      return { rows_, columns_, segments_, detail::dtype< Real >(), values_, columnIndexes_ };
+      return { rows_, columns_, segments_, detail::dtype< Real >(), values_, columnIndexes_, matrixType() };
+   }
+   static constexpr bool isSymmetric() { return MatrixType::isSymmetric(); }
+   static constexpr int32_t matrixType() { return MatrixType::isSymmetric() ? TNLMI_SYMMETRIC : TNLMI_GENERAL; }
+
+   //! transposedVectorProduct (SparseMatrixBase.hpp:541-588): out = a*A^T*in + b*out on columns [begin, end)
+   template< typename InVector, typename OutVector >
+   void transposedVectorProduct( const InVector& in, OutVector& out, ComputeReal matrixMultiplicator = 1.0,
+                                 ComputeReal outVectorMultiplicator = 0.0, int begin = 0, int end = 0 ) const
+   {
+      if( getRows() != in.getSize() )
+         throw std::invalid_argument(
+            "transposedVectorProduct: size of the input vector does not match the number of matrix rows" );
+      if( getColumns() != out.getSize() )
+         throw std::invalid_argument(
+            "transposedVectorProduct: size of the output vector does not match the number of matrix columns" );
+      using OutReal = typename OutVector::RealType;
+      const tnlmi_matrix m = abi();
+      detail::check( tnlmi_spmv_transposed( &m, detail::dtype< OutReal >(), in.getData(), out.getData(),
+                                            matrixMultiplicator, outVectorMultiplicator, begin, end, nullptr ) );
+      detail::hip( hipDeviceSynchronize() );
    }
 
 private:
