@@ -349,6 +349,19 @@ tnlmi_status tnlmi_preconditioner_solve( const tnlmi_preconditioner* P, const do
 tnlmi_status tnlmi_bicgstab_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
                                    int32_t exact_residue, const double* d_b, double* d_x, tnlmi_solver_state* state,
                                    tnlmi_stream stream );
+/* BICGStabL::solve (Solvers/Linear/BICGStabL.hpp:37-257): BiCGStab(ell),
+ * ell in [1, 32] ("bicgstab-ell", default 1), exact_residue as
+ * "bicgstab-exact-residue"; one iteration per Bi-CG step.  A NaN initial
+ * residue is TNLMI_RUNTIME_ERROR (the reference throws std::runtime_error). */
+tnlmi_status tnlmi_bicgstabl_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan,
+                                    const tnlmi_preconditioner* M, int32_t ell, int32_t exact_residue,
+                                    const double* d_b, double* d_x, tnlmi_solver_state* state, tnlmi_stream stream );
+/* IDRs::solve (Solvers/Linear/IDRs.hpp:47-386): IDR(s), s in [1, 32] ("idr-s",
+ * default 4), smoothing as "idr-residual-smoothing"; the shadow space drawn as
+ * the reference draws it.  A breakdown returns with converged = 0. */
+tnlmi_status tnlmi_idrs_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
+                               int32_t s, int32_t smoothing, const double* d_b, double* d_x,
+                               tnlmi_solver_state* state, tnlmi_stream stream );
 tnlmi_status tnlmi_tfqmr_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
                                 const double* d_b, double* d_x, tnlmi_solver_state* state, tnlmi_stream stream );
 /* LinearSolver<Matrix>::solve(b, x) -> bool (synchronous; FP64 vectors; any

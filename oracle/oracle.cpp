@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <random>
 #include <numeric>
 #include <vector>
 #ifdef _OPENMP
@@ -1408,6 +1409,292 @@ void orc_tfqmr( int64_t n, const int32_t* row_ptr, const int32_t* cols, const do
          for( int64_t i = 0; i < n; i++ )
             u[ i ] -= alpha * v[ i ];
    }
+   S.finish( S.converged() );
+}
+
+
+// BICGStabL.hpp:37-257: BiCGStab(ell) -- ell Bi-CG steps on the columns of R
+// and U (column j is R_j, U_j; n entries each), then a minimal-residual step
+// over r_1..r_ell orthogonalised by modified Gram-Schmidt.  One iteration is
+// counted per Bi-CG step.
+void orc_bicgstabl( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                    double* x, int precond, int ell, int exact_residue, orc_solver_state* st )
+{
+   Csr A{ n, row_ptr, cols, vals };
+   Precond M;
+   M.kind = precond;
+   M.update( A );
+   Iter S{ st };
+   std::vector< double > R( ( ell + 1 ) * n ), U( ( ell + 1 ) * n ), r_ast( n ), tmp( n ), res( n );
+   std::vector< double > T( ell * ell ), sigma( ell + 1 ), g0( ell + 1 ), g1( ell + 1 ), g2( ell + 1 );
+   auto col = [ & ]( std::vector< double >& V, int j ) { return V.data() + (int64_t) j * n; };
+   double b_norm;
+   if( precond ) {
+      M.solve( b, tmp.data(), n );
+      b_norm = nrm2( tmp.data(), n );
+   }
+   else
+      b_norm = nrm2( b, n );
+   if( b_norm == 0 ) {
+      std::fill( x, x + n, 0.0 );
+      S.finish( true );
+      return;
+   }
+   pres( A, M, precond, x, b, col( R, 0 ), tmp );
+   sigma[ 0 ] = nrm2( col( R, 0 ), n );
+   if( std::isnan( sigma[ 0 ] ) ) {
+      S.res = sigma[ 0 ];
+      S.finish( false );  // the reference throws "initial residue is NAN"
+      return;
+   }
+   for( int64_t i = 0; i < n; i++ )
+      r_ast[ i ] = col( R, 0 )[ i ] / sigma[ 0 ];
+   double rho0 = 1.0, alpha = 0.0, omega = 1.0;
+   std::fill( U.begin(), U.begin() + n, 0.0 );
+   S.res = sigma[ 0 ] / b_norm;
+   while( S.check() ) {
+      rho0 = -omega * rho0;
+      for( int j = 0; j < ell; j++ ) {
+         S.next();
+         const double rho1 = dot( r_ast.data(), col( R, j ), n );
+         const double beta = alpha * rho1 / rho0;
+         rho0 = rho1;
+         for( int c = 0; c <= j; c++ )  // U_[0:j] = R_[0:j] - beta U_[0:j]
+            for( int64_t i = 0; i < n; i++ )
+               col( U, c )[ i ] = 1.0 * col( R, c )[ i ] + ( -beta ) * col( U, c )[ i ];
+         pmv( A, M, precond, col( U, j ), col( U, j + 1 ), tmp );
+         const double gamma = dot( r_ast.data(), col( U, j + 1 ), n );
+         alpha = rho0 / gamma;
+         for( int c = 0; c <= j; c++ )  // R_[0:j] = R_[0:j] - alpha U_[1:j+1]
+            for( int64_t i = 0; i < n; i++ )
+               col( R, c )[ i ] = 1.0 * col( R, c )[ i ] + ( -alpha ) * col( U, c + 1 )[ i ];
+         pmv( A, M, precond, col( R, j ), col( R, j + 1 ), tmp );
+         for( int64_t i = 0; i < n; i++ )
+            x[ i ] = x[ i ] + alpha * col( U, 0 )[ i ];
+      }
+      for( int j = 1; j <= ell; j++ ) {
+         for( int i = 1; i < j; i++ ) {
+            const int ij = ( i - 1 ) + ( j - 1 ) * ell;
+            T[ ij ] = dot( col( R, i ), col( R, j ), n ) / sigma[ i ];
+            for( int64_t q = 0; q < n; q++ )
+               col( R, j )[ q ] = col( R, j )[ q ] - T[ ij ] * col( R, i )[ q ];
+         }
+         sigma[ j ] = dot( col( R, j ), col( R, j ), n );
+         g1[ j ] = dot( col( R, 0 ), col( R, j ), n ) / sigma[ j ];
+      }
+      omega = g1[ ell ];
+      for( int j = ell; j >= 1; j-- ) {  // g0 = T^-1 g1
+         g0[ j ] = g1[ j ];
+         for( int i = j + 1; i <= ell; i++ )
+            g0[ j ] -= T[ ( j - 1 ) + ( i - 1 ) * ell ] * g0[ i ];
+      }
+      for( int j = 1; j < ell; j++ ) {  // g2 = T S g0
+         g2[ j ] = g0[ j + 1 ];
+         for( int i = j + 1; i < ell; i++ )
+            g2[ j ] += T[ ( j - 1 ) + ( i - 1 ) * ell ] * g0[ i + 1 ];
+      }
+      g2[ 0 ] = g0[ 1 ];
+      // x += R_[0:ell-1] g2; r_0 -= R_[1:ell] g1[1:ell]; u_0 -= U_[1:ell] g0[1:ell]
+      // (MatrixOperations::gemv: sum over the columns in order, then + y)
+      for( int64_t i = 0; i < n; i++ ) {
+         double ax = 0, ar = 0, au = 0;
+         for( int c = 0; c < ell; c++ ) {
+            ax += col( R, c )[ i ] * ( 1.0 * g2[ c ] );
+            ar += col( R, c + 1 )[ i ] * ( -1.0 * g1[ c + 1 ] );
+            au += col( U, c + 1 )[ i ] * ( -1.0 * g0[ c + 1 ] );
+         }
+         x[ i ] = ax + 1.0 * x[ i ];
+         col( R, 0 )[ i ] = ar + 1.0 * col( R, 0 )[ i ];
+         col( U, 0 )[ i ] = au + 1.0 * col( U, 0 )[ i ];
+      }
+      if( exact_residue ) {
+         pres( A, M, precond, x, b, res.data(), tmp );
+         sigma[ 0 ] = nrm2( res.data(), n );
+      }
+      else
+         sigma[ 0 ] = nrm2( col( R, 0 ), n );
+      S.res = sigma[ 0 ] / b_norm;
+   }
+   S.finish( S.converged() );
+}
+
+// IDRs.hpp:47-316 with setSize :334-386: IDR(s) with the shadow space P of s
+// columns drawn from std::mt19937 seeded 0 through
+// std::normal_distribution<double>(0, 1), each column having the scalar
+// (P_i, P_j) subtracted from every entry for j < i and then scaled to unit
+// norm, as the reference does; optional minimal-residual smoothing.  One
+// iteration per G-space step (s per cycle) and one for the omega step.
+void orc_idrs( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b, double* x,
+               int precond, int s, int smoothing, orc_solver_state* st )
+{
+   Csr A{ n, row_ptr, cols, vals };
+   Precond M;
+   M.kind = precond;
+   M.update( A );
+   Iter S{ st };
+   std::vector< double > P( (int64_t) s * n ), G( (int64_t) s * n, 0.0 ), Uv( (int64_t) s * n, 0.0 );
+   std::vector< double > r( n ), v( n ), t( n ), x_s, r_s, Mm( s * s, 0.0 ), f( s );
+   auto colp = [ & ]( std::vector< double >& V, int j ) { return V.data() + (int64_t) j * n; };
+   {
+      std::mt19937 rng;
+      rng.seed( 0 );
+      std::normal_distribution< double > dist( 0, 1 );
+      for( auto& p : P )
+         p = dist( rng );
+      for( int i = 0; i < s; i++ ) {
+         for( int j = 0; j < i; j++ ) {
+            const double d = dot( colp( P, i ), colp( P, j ), n );
+            for( int64_t q = 0; q < n; q++ )
+               colp( P, i )[ q ] = colp( P, i )[ q ] - d;
+         }
+         const double nr = nrm2( colp( P, i ), n );
+         for( int64_t q = 0; q < n; q++ )
+            colp( P, i )[ q ] /= nr;
+      }
+   }
+   auto Mat = [ & ]( int i, int j ) -> double& { return Mm[ i * s + j ]; };
+   auto psolve = [ & ]( const double* src, double* dst ) {
+      if( precond )
+         M.solve( src, dst, n );
+      else if( src != dst )
+         std::memcpy( dst, src, sizeof( double ) * n );
+   };
+   const double b_norm = nrm2( b, n );
+   if( b_norm == 0 ) {
+      std::fill( x, x + n, 0.0 );
+      S.finish( true );
+      return;
+   }
+   A.mv( x, r.data() );
+   for( int64_t i = 0; i < n; i++ )
+      r[ i ] = b[ i ] - r[ i ];
+   double r_norm = nrm2( r.data(), n );
+   if( std::isnan( r_norm ) ) {
+      S.res = r_norm;
+      S.finish( false );  // the reference throws "initial residue is NAN"
+      return;
+   }
+   const double angle = 0.7;
+   double omega = 1.0;
+   for( int i = 0; i < s; i++ )
+      Mat( i, i ) = 1;
+   if( smoothing ) {
+      x_s.assign( x, x + n );
+      r_s = r;
+   }
+   auto smooth = [ & ]() {
+      for( int64_t i = 0; i < n; i++ )
+         t[ i ] = r_s[ i ] - r[ i ];
+      const double gamma = dot( t.data(), r_s.data(), n ) / dot( t.data(), t.data(), n );
+      for( int64_t i = 0; i < n; i++ ) {
+         r_s[ i ] = r_s[ i ] - gamma * t[ i ];
+         x_s[ i ] = x_s[ i ] - gamma * ( x_s[ i ] - x[ i ] );
+      }
+      r_norm = nrm2( r_s.data(), n );
+   };
+   S.res = r_norm / b_norm;
+   while( S.check() ) {
+      for( int i = 0; i < s; i++ )
+         f[ i ] = dot( colp( P, i ), r.data(), n );
+      for( int k = 0; k < s; k++ ) {
+         // c = M[k:s, k:s]^-1 f[k:s] (LUsequential.h: no pivoting)
+         const int m = s - k;
+         std::vector< double > L( m * m ), c( m );
+         for( int i = 0; i < m; i++ )
+            for( int j = 0; j < m; j++ )
+               L[ i * m + j ] = Mat( i + k, j + k );
+         for( int q = 0; q < m; q++ )
+            for( int j = q + 1; j < m; j++ ) {
+               const double factor = L[ j * m + q ] / L[ q * m + q ];
+               L[ j * m + q ] = factor;
+               for( int i = q + 1; i < m; i++ )
+                  L[ j * m + i ] = L[ j * m + i ] - factor * L[ q * m + i ];
+            }
+         for( int i = 0; i < m; i++ )
+            c[ i ] = f[ k + i ];
+         for( int q = 1; q < m; q++ )
+            for( int j = 0; j < q; j++ )
+               c[ q ] -= L[ q * m + j ] * c[ j ];
+         for( int q = m - 1; q >= 0; q-- ) {
+            for( int j = m - 1; j > q; j-- )
+               c[ q ] -= L[ q * m + j ] * c[ j ];
+            c[ q ] /= L[ q * m + q ];
+         }
+         // v = r - G[:, k:s] c (gemv: column sum, then + y)
+         for( int64_t i = 0; i < n; i++ ) {
+            double a = 0;
+            for( int j = 0; j < m; j++ )
+               a += colp( G, k + j )[ i ] * ( -1.0 * c[ j ] );
+            v[ i ] = a + 1.0 * r[ i ];
+         }
+         psolve( v.data(), v.data() );
+         // U_k = U[:, k:s] c + omega v (gemv with beta 0: U_k is read before it is written, entry by entry)
+         double* Uk = colp( Uv, k );
+         for( int64_t i = 0; i < n; i++ ) {
+            double a = 0;
+            for( int j = 0; j < m; j++ )
+               a += colp( Uv, k + j )[ i ] * ( 1.0 * c[ j ] );
+            Uk[ i ] = a;
+         }
+         for( int64_t i = 0; i < n; i++ )
+            Uk[ i ] += omega * v[ i ];
+         double* Gk = colp( G, k );
+         A.mv( Uk, Gk );
+         for( int i = 0; i < k; i++ ) {
+            const double al = dot( colp( P, i ), Gk, n ) / Mat( i, i );
+            for( int64_t q = 0; q < n; q++ ) {
+               Uk[ q ] = Uk[ q ] - al * colp( Uv, i )[ q ];
+               Gk[ q ] = Gk[ q ] - al * colp( G, i )[ q ];
+            }
+         }
+         for( int i = k; i < s; i++ )
+            Mat( i, k ) = dot( colp( P, i ), Gk, n );
+         if( Mat( k, k ) == 0 ) {
+            S.finish( false );  // breakdown
+            return;
+         }
+         const double beta = f[ k ] / Mat( k, k );
+         for( int64_t i = 0; i < n; i++ ) {
+            x[ i ] = x[ i ] + beta * Uk[ i ];
+            r[ i ] = r[ i ] - beta * Gk[ i ];
+         }
+         r_norm = nrm2( r.data(), n );
+         if( smoothing )
+            smooth();
+         S.res = r_norm / b_norm;
+         if( ! S.next() ) {
+            if( smoothing )
+               std::memcpy( x, x_s.data(), sizeof( double ) * n );
+            S.finish( S.converged() );
+            return;
+         }
+         for( int i = k + 1; i < s; i++ )
+            f[ i ] = f[ i ] - beta * Mat( i, k );
+      }
+      psolve( r.data(), v.data() );
+      A.mv( v.data(), t.data() );
+      const double t_norm = nrm2( t.data(), n );
+      const double tr = dot( t.data(), r.data(), n );
+      const double rho = std::abs( tr / ( t_norm * r_norm ) );
+      omega = tr / ( t_norm * t_norm );
+      if( rho < angle )
+         omega = omega * angle / rho;
+      if( omega == 0 ) {
+         S.finish( false );  // breakdown
+         return;
+      }
+      for( int64_t i = 0; i < n; i++ ) {
+         x[ i ] = x[ i ] + omega * v[ i ];
+         r[ i ] = r[ i ] - omega * t[ i ];
+      }
+      r_norm = nrm2( r.data(), n );
+      if( smoothing )
+         smooth();
+      S.res = r_norm / b_norm;
+      S.next();
+   }
+   if( smoothing )
+      std::memcpy( x, x_s.data(), sizeof( double ) * n );
    S.finish( S.converged() );
 }
 

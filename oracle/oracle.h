@@ -149,6 +149,13 @@ void orc_bicgstab( int64_t n, const int32_t* row_ptr, const int32_t* cols, const
 /* TFQMR.hpp:12-121 */
 void orc_tfqmr( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
                 double* x, int precond, orc_solver_state* st );
+/* BICGStabL.hpp:37-257: BiCGStab(ell); an iteration per Bi-CG step. */
+void orc_bicgstabl( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                    double* x, int precond, int ell, int exact_residue, orc_solver_state* st );
+/* IDRs.hpp:47-386: IDR(s) with the reference's shadow space (mt19937 seed 0,
+ * normal_distribution) and optional residual smoothing. */
+void orc_idrs( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b, double* x,
+               int precond, int s, int smoothing, orc_solver_state* st );
 /* Jacobi: Solvers/Linear/Jacobi.hpp:65-125. */
 void orc_jacobi( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
                  double* x, double omega, int64_t residue_period, orc_solver_state* st );

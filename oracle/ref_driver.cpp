@@ -21,6 +21,8 @@
 #include <TNL/Solvers/Linear/GMRES.h>
 #include <TNL/Solvers/Linear/BICGStab.h>
 #include <TNL/Solvers/Linear/TFQMR.h>
+#include <TNL/Solvers/Linear/BICGStabL.h>
+#include <TNL/Solvers/Linear/IDRs.h>
 #include <TNL/Solvers/Linear/Jacobi.h>
 #include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
 #include <TNL/Solvers/Linear/Preconditioners/ILU0.h>
@@ -556,6 +558,30 @@ void ref_tfqmr( int64_t n, const int32_t* rp, const int32_t* c, const double* v,
 {
    auto A = make_csr( n, rp, c, v );
    Solvers::Linear::TFQMR< CsrMat > s;
+   run_solver( s, A, precond, b, x, n, st );
+}
+
+void ref_bicgstabl( int64_t n, const int32_t* rp, const int32_t* c, const double* v, const double* b, double* x,
+                    int precond, int ell, int exact_residue, State* st )
+{
+   auto A = make_csr( n, rp, c, v );
+   Solvers::Linear::BICGStabL< CsrMat > s;
+   Config::ConfigDescription cd;
+   Solvers::Linear::BICGStabL< CsrMat >::configSetup( cd );
+   Config::ParameterContainer pc;
+   pc.addParameter< int >( "bicgstab-ell", ell );
+   pc.addParameter< bool >( "bicgstab-exact-residue", exact_residue != 0 );
+   s.setup( pc );
+   run_solver( s, A, precond, b, x, n, st );
+}
+
+void ref_idrs( int64_t n, const int32_t* rp, const int32_t* c, const double* v, const double* b, double* x,
+               int precond, int s_dim, int smoothing, State* st )
+{
+   auto A = make_csr( n, rp, c, v );
+   Solvers::Linear::IDRs< CsrMat > s;
+   s.setShadowSpaceDimension( s_dim );
+   s.setResidualSmoothing( smoothing != 0 );
    run_solver( s, A, precond, b, x, n, st );
 }
 

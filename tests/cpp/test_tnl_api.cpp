@@ -179,6 +179,10 @@ int main()
    solveExample< Solvers::Linear::BICGStab< Matrix > >( A, "BICGStab + Diagonal", true );
    solveExample< Solvers::Linear::TFQMR< Matrix > >( A, "TFQMR" );
    solveExample< Solvers::Linear::TFQMR< Matrix > >( A, "TFQMR + Diagonal", true );
+   solveExample< Solvers::Linear::BICGStabL< Matrix > >( A, "BICGStabL" );
+   solveExample< Solvers::Linear::BICGStabL< Matrix > >( A, "BICGStabL + Diagonal", true );
+   solveExample< Solvers::Linear::IDRs< Matrix > >( A, "IDRs" );
+   solveExample< Solvers::Linear::IDRs< Matrix > >( A, "IDRs + Diagonal", true );
 
    // copySparseMatrix into BiEllpack and ChunkedEllpack (column-major) and back to CSR
    {

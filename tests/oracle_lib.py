@@ -200,6 +200,24 @@ class Oracle:
         self.L.orc_tfqmr(I64(n), P(off), P(cols), P(vals), P(b), P(x), precond, ctypes.byref(st))
         return x, st
 
+    def bicgstabl(self, off, cols, vals, b, x0=None, precond=0, ell=1, exact=False, **kw):
+        off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
+        n = len(b)
+        x = np.zeros(n) if x0 is None else f64(x0).copy()
+        st = state(**kw)
+        self.L.orc_bicgstabl(I64(n), P(off), P(cols), P(vals), P(b), P(x), precond, int(ell), int(exact),
+                             ctypes.byref(st))
+        return x, st
+
+    def idrs(self, off, cols, vals, b, x0=None, precond=0, s=4, smoothing=False, **kw):
+        off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
+        n = len(b)
+        x = np.zeros(n) if x0 is None else f64(x0).copy()
+        st = state(**kw)
+        self.L.orc_idrs(I64(n), P(off), P(cols), P(vals), P(b), P(x), precond, int(s), int(smoothing),
+                        ctypes.byref(st))
+        return x, st
+
     def gmres(self, off, cols, vals, b, x0=None, precond=0, variant=3, rmin=10, rmax=10, dmin=3, dmax=3, **kw):
         off, cols, vals, b = i32(off), i32(cols), f64(vals), f64(b)
         n = len(b)
@@ -413,6 +431,12 @@ class Reference:
 
     def tfqmr(self, off, cols, vals, b, x0=None, precond=0, **kw):
         return self._solve(self.L.ref_tfqmr, off, cols, vals, b, x0, precond, **kw)
+
+    def bicgstabl(self, off, cols, vals, b, x0=None, precond=0, ell=1, exact=False, **kw):
+        return self._solve(self.L.ref_bicgstabl, off, cols, vals, b, x0, precond, int(ell), int(exact), **kw)
+
+    def idrs(self, off, cols, vals, b, x0=None, precond=0, s=4, smoothing=False, **kw):
+        return self._solve(self.L.ref_idrs, off, cols, vals, b, x0, precond, int(s), int(smoothing), **kw)
 
     def jacobi(self, off, cols, vals, b, x0=None, omega=1.0, period=4, **kw):
         return self._solve(self.L.ref_jacobi, off, cols, vals, b, x0, ctypes.c_double(omega), I64(period), **kw)

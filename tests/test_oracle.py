@@ -14,6 +14,8 @@ Pins, per oracle function:
   (iteration counts and solutions), plus closed forms (ILU(0) of a tridiagonal
   matrix is its exact LU factorisation; the 5x5 solver example solves to ones).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -283,6 +285,64 @@ def test_tfqmr_vs_reference(orc, ref):
         xr, sr = ref.tfqmr(off, cols, vals, b, tol=1e-3, maxit=maxit, minit=minit)
         assert so.iterations == sr.iterations and so.converged == sr.converged, (maxit, minit)
         np.testing.assert_allclose(xo, xr, rtol=1e-9, atol=1e-12)
+
+
+def _scaled_nonsym(n, seed):
+    # rows scaled by 1..5 so that the Diagonal preconditioner is not a uniform scaling
+    off, cols, vals = nonsym(n, np.random.default_rng(seed))
+    scale = 1.0 + (np.arange(n) % 5)
+    return off, cols, vals * np.repeat(scale, np.diff(off))
+
+
+@pytest.mark.parametrize("ell,exact", [(1, False), (2, False), (4, True)])
+def test_bicgstabl_vs_reference(orc, ref, ell, exact):
+    # BICGStabL.hpp:37-257; the reference runs single-threaded (its reductions
+    # then sum in one order)
+    off, cols, vals = _scaled_nonsym(300, 13)
+    b = np.random.default_rng(14).standard_normal(300)
+    xd = _dense_solve(off, cols, vals, b)
+    ref.set_threads(1)
+    try:
+        for pre in (0, 1, 2):
+            xo, so = orc.bicgstabl(off, cols, vals, b, precond=pre, ell=ell, exact=exact, tol=1e-10)
+            xr, sr = ref.bicgstabl(off, cols, vals, b, precond=pre, ell=ell, exact=exact, tol=1e-10)
+            assert so.converged == sr.converged == 1
+            assert so.iterations == sr.iterations, (pre, so.iterations, sr.iterations)
+            np.testing.assert_allclose(xo, xr, rtol=1e-9, atol=1e-11)
+            np.testing.assert_allclose(xo, xd, rtol=1e-7, atol=1e-8)
+        # iteration limits: the loop checks once per ell Bi-CG steps
+        for maxit in (3, 7):
+            xo, so = orc.bicgstabl(off, cols, vals, b, ell=ell, tol=1e-14, maxit=maxit)
+            xr, sr = ref.bicgstabl(off, cols, vals, b, ell=ell, tol=1e-14, maxit=maxit)
+            assert so.iterations == sr.iterations and so.converged == sr.converged == 0
+            np.testing.assert_allclose(xo, xr, rtol=1e-9, atol=1e-12)
+    finally:
+        ref.set_threads(os.cpu_count() or 1)
+
+
+@pytest.mark.parametrize("s,smoothing", [(1, False), (4, False), (4, True), (8, True)])
+def test_idrs_vs_reference(orc, ref, s, smoothing):
+    # IDRs.hpp:47-386 with the reference's shadow space (std::mt19937 seed 0)
+    off, cols, vals = _scaled_nonsym(300, 15)
+    b = np.random.default_rng(16).standard_normal(300)
+    xd = _dense_solve(off, cols, vals, b)
+    ref.set_threads(1)
+    try:
+        for pre in (0, 1, 2):
+            xo, so = orc.idrs(off, cols, vals, b, precond=pre, s=s, smoothing=smoothing, tol=1e-10)
+            xr, sr = ref.idrs(off, cols, vals, b, precond=pre, s=s, smoothing=smoothing, tol=1e-10)
+            assert so.converged == sr.converged == 1
+            assert so.iterations == sr.iterations, (pre, so.iterations, sr.iterations)
+            np.testing.assert_allclose(xo, xr, rtol=1e-9, atol=1e-11)
+            np.testing.assert_allclose(xo, xd, rtol=1e-7, atol=1e-8)
+        for maxit in (3, 10):
+            xo, so = orc.idrs(off, cols, vals, b, s=s, smoothing=smoothing, tol=1e-14, maxit=maxit)
+            xr, sr = ref.idrs(off, cols, vals, b, s=s, smoothing=smoothing, tol=1e-14, maxit=maxit)
+            assert so.iterations == sr.iterations and so.converged == sr.converged == 0
+            # (inner products summed in another order: s = 8 amplifies it to 1e-11)
+            np.testing.assert_allclose(xo, xr, rtol=1e-7, atol=1e-10)
+    finally:
+        ref.set_threads(os.cpu_count() or 1)
 
 
 def test_jacobi_vs_reference(orc, ref):

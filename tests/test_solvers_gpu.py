@@ -476,3 +476,77 @@ def test_bicgstab_tfqmr_convection_diffusion():
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         assert s.solve(b, x), type(s).__name__
         assert S.true_residual(A, x, b) < 1e-8
+
+
+def scaled_nonsym(n, seed):
+    off, cols, vals = nonsym(n, seed)
+    scale = 1.0 + (np.arange(n) % 5)
+    return off, cols, vals * np.repeat(scale, np.diff(off))
+
+
+@pytest.mark.parametrize("fmt", [tnl.CSR, tnl.ELLPACK], ids=["csr", "ellpack"])
+@pytest.mark.parametrize("pre", [0, 1, 2], ids=["none", "diag", "ilu0"])
+@pytest.mark.parametrize("variant", ["bicgstabl1", "bicgstabl2", "bicgstabl4exact", "idr1", "idr4", "idr4smooth",
+                                     "idr8smooth"])
+def test_bicgstabl_idrs_match_oracle(orc, variant, pre, fmt):
+    # BICGStabL.hpp:37-257 / IDRs.hpp:47-386 on the GPU: the oracle's iterations
+    # (+-1: the GPU sums inner products in another order) and solution
+    off, cols, vals = scaled_nonsym(700, 31)
+    n = len(off) - 1
+    b = np.random.default_rng(32).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt)
+    if variant.startswith("bicgstabl"):
+        ell, exact = int(variant[9]), variant.endswith("exact")
+        s = S.BICGStabL()
+        s.setup(ell=ell, exact_residue=exact)
+        xo, so = orc.bicgstabl(off, cols, vals, b, precond=pre, ell=ell, exact=exact, tol=1e-10)
+    else:
+        sd, sm = int(variant[3]), variant.endswith("smooth")
+        s = S.IDRs()
+        s.setShadowSpaceDimension(sd)
+        s.setResidualSmoothing(sm)
+        xo, so = orc.idrs(off, cols, vals, b, precond=pre, s=sd, smoothing=sm, tol=1e-10)
+    s.setMatrix(A)
+    if pre:
+        s.setPreconditioner(S.Diagonal(A) if pre == 1 else S.ILU0(A))
+    s.setConvergenceResidue(1e-10)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x) and so.converged
+    tol_it = 1 if variant.startswith("idr") else int(variant[9])  # BiCGStab(ell) stops every ell steps
+    assert abs(s.getIterations() - so.iterations) <= tol_it, (s.getIterations(), so.iterations)
+    np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-8)
+    assert orc.true_residual(off, cols, vals, x.cpu().numpy(), b) < 1e-8
+
+
+@pytest.mark.parametrize("solver", ["bicgstabl", "idrs"])
+def test_bicgstabl_idrs_limits_zero_rhs_nan(orc, solver):
+    off, cols, vals = scaled_nonsym(300, 33)
+    n = len(off) - 1
+    b = np.random.default_rng(34).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+
+    def make():
+        s = S.BICGStabL() if solver == "bicgstabl" else S.IDRs()
+        if solver == "bicgstabl":
+            s.setup(ell=2)
+        s.setMatrix(A)
+        return s
+    fn = (lambda **kw: orc.bicgstabl(off, cols, vals, b, ell=2, **kw)) if solver == "bicgstabl" else \
+        (lambda **kw: orc.idrs(off, cols, vals, b, **kw))
+    for maxit in (3, 9):
+        s = make()
+        s.setConvergenceResidue(1e-14)
+        s.setMaxIterations(maxit)
+        xo, so = fn(tol=1e-14, maxit=maxit)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ok = s.solve(dev(b), x)
+        assert ok == bool(so.converged) and s.getIterations() == so.iterations, maxit
+        np.testing.assert_allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-10)
+    s = make()
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    assert s.solve(torch.zeros(n, dtype=torch.float64, device="cuda"), x)
+    assert float(x.abs().max()) == 0.0 and s.getIterations() == 0
+    bn = b.copy()
+    bn[7] = np.nan
+    with pytest.raises(tnl.TnlmiError, match="initial residue is NAN"):
+        make().solve(dev(bn), torch.zeros(n, dtype=torch.float64, device="cuda"))

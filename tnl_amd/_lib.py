@@ -144,6 +144,10 @@ _PROTOS = {
                                          ctypes.POINTER(SolverState), c_vp]),
     "tnlmi_bicgstab_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int32, c_vp, c_vp,
                                             ctypes.POINTER(SolverState), c_vp]),
+    "tnlmi_bicgstabl_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int32, ctypes.c_int32, c_vp,
+                                             c_vp, ctypes.POINTER(SolverState), c_vp]),
+    "tnlmi_idrs_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int32, ctypes.c_int32, c_vp, c_vp,
+                                        ctypes.POINTER(SolverState), c_vp]),
     "tnlmi_tfqmr_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(SolverState),
                                          c_vp]),
     "tnlmi_jacobi_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, ctypes.c_double, c_i64, c_vp, c_vp,

@@ -1032,6 +1032,64 @@ public:
    }
 };
 
+//! BICGStabL (BICGStabL.hpp:12-257): bicgstab-ell and bicgstab-exact-residue via setup()
+template< typename Matrix >
+class BICGStabL : public LinearSolver< Matrix >
+{
+public:
+   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "bicgstab-ell" ) )
+         ell_ = p.getParameter< int >( prefix + "bicgstab-ell" );
+      if( p.checkParameter( prefix + "bicgstab-exact-residue" ) )
+         exact_residue_ = p.getParameter< bool >( prefix + "bicgstab-exact-residue" );
+      return LinearSolver< Matrix >::setup( p, prefix );
+   }
+   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
+               typename LinearSolver< Matrix >::VectorType& x ) override
+   {
+      const tnlmi_matrix m = this->mat( "BICGStabL" );
+      tnlmi_solver_state st = this->state();
+      detail::check( tnlmi_bicgstabl_solve( &m, nullptr, this->pre(), ell_, exact_residue_ ? 1 : 0, b.getData(),
+                                            x.getData(), &st, nullptr ) );
+      return this->finish( st );
+   }
+
+private:
+   int ell_ = 1;                 // configSetup default, BICGStabL.hpp:17
+   bool exact_residue_ = false;  // BICGStabL.hpp:18-21
+};
+
+//! IDRs (IDRs.hpp:12-386): idr-s and idr-residual-smoothing via setup() or the setters
+template< typename Matrix >
+class IDRs : public LinearSolver< Matrix >
+{
+public:
+   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "idr-s" ) )
+         s_ = p.getParameter< int >( prefix + "idr-s" );
+      if( p.checkParameter( prefix + "idr-residual-smoothing" ) )
+         smoothing_ = p.getParameter< bool >( prefix + "idr-residual-smoothing" );
+      return LinearSolver< Matrix >::setup( p, prefix );
+   }
+   void setShadowSpaceDimension( int s ) { s_ = s; }
+   void setResidualSmoothing( bool smoothing ) { smoothing_ = smoothing; }
+   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
+               typename LinearSolver< Matrix >::VectorType& x ) override
+   {
+      const tnlmi_matrix m = this->mat( "IDRs" );
+      tnlmi_solver_state st = this->state();
+      detail::check(
+         tnlmi_idrs_solve( &m, nullptr, this->pre(), s_, smoothing_ ? 1 : 0, b.getData(), x.getData(), &st, nullptr ) );
+      return this->finish( st );
+   }
+
+private:
+   int s_ = 4;               // IDRs.h default shadow space dimension
+   bool smoothing_ = false;  // IDRs.h default
+};
+
 //! Jacobi (Jacobi.hpp:65-125)
 template< typename Matrix >
 class Jacobi : public LinearSolver< Matrix >

@@ -197,6 +197,56 @@ class TFQMR(IterativeSolver):
         return self._finish(st)
 
 
+class BICGStabL(IterativeSolver):
+    """Solvers::Linear::BICGStabL (BICGStabL.hpp:37-257): BiCGStab(ell);
+    setup(ell=..., exact_residue=...) are "bicgstab-ell" (default 1) and
+    "bicgstab-exact-residue" (default false).  One iteration per Bi-CG step."""
+
+    def __init__(self):
+        super().__init__()
+        self.ell = 1
+        self.exact_residue = False
+
+    def setup(self, ell=None, exact_residue=None):
+        if ell is not None:
+            self.ell = int(ell)
+        if exact_residue is not None:
+            self.exact_residue = bool(exact_residue)
+        return True
+
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_bicgstabl_solve(m, plan, pre, self.ell, int(self.exact_residue), _f64(b),
+                                                    _f64(x), ctypes.byref(st), _stream()))
+        return self._finish(st)
+
+
+class IDRs(IterativeSolver):
+    """Solvers::Linear::IDRs (IDRs.hpp:47-386): IDR(s) with the reference's
+    shadow space; setShadowSpaceDimension (idr-s, default 4),
+    setResidualSmoothing (idr-residual-smoothing, default false).  A breakdown
+    returns False."""
+
+    def __init__(self):
+        super().__init__()
+        self.s = 4
+        self.smoothing = False
+
+    def setShadowSpaceDimension(self, s):
+        self.s = int(s)
+
+    def setResidualSmoothing(self, smoothing):
+        self.smoothing = bool(smoothing)
+
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_idrs_solve(m, plan, pre, self.s, int(self.smoothing), _f64(b), _f64(x),
+                                               ctypes.byref(st), _stream()))
+        return self._finish(st)
+
+
 class Jacobi(IterativeSolver):
     def __init__(self):
         super().__init__()
