@@ -270,8 +270,10 @@ __global__ void __launch_bounds__( kP2 )
    uint32_t d[ R ];
 #pragma unroll
    for( int i = 0; i < R; i++ ) {
-      const int32_t ai = __shfl( a, gb + i, 64 );
-      get_pair( prod, dest, ai + kPer * gl, v[ i ], d[ i ] );
+      // lanes past the segment's end read its last term again (no extra
+      // cache line per request); what they read is not stored
+      const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
+      get_pair( prod, dest, std::min( ai + kPer * gl, std::max( zi - 1, ai ) ), v[ i ], d[ i ] );
    }
    // clear (padding slots, column -1, have no product: they read as zero)
    const int32_t len = H.e1 - H.e0;
