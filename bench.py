@@ -586,6 +586,44 @@ def wl_gmres_convdiff(args):
         return solvers.Explicit(net.inverse(A)), {}
     run("learned_spai", learned, args.max_iterations)
     run("ilu0", lambda: (solvers.ILU0(A), {}), args.ilu_max_iterations)
+
+    # the nonsymmetric alternative on the same system: BiCGStab (BICGStab.hpp),
+    # restarted from its own x with a tighter target until the true residual
+    # meets the tolerance (its stopping residue is the preconditioned one)
+    def true_residual(x):
+        y = torch.zeros(n, dtype=torch.float64, device="cuda")
+        A.vectorProduct(x, y)
+        return float(torch.linalg.norm(b - y) / torch.linalg.norm(b))
+
+    def run_bicgstab(label, make_pre):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pre, _ = make_pre()
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        s = solvers.BICGStab()
+        s.setMatrix(A)
+        s.setPreconditioner(pre)
+        s.setMaxIterations(args.max_iterations)
+        target, its, ok, r = tol, 0, False, float("nan")
+        for _ in range(6):
+            s.setConvergenceResidue(target)
+            ok = s.solve(b, x)
+            its += s.getIterations()
+            r = true_residual(x)
+            if r <= tol or not ok:
+                break
+            target *= 0.5 * tol / r
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        rec = {"seconds": round(t, 4), "setup_seconds": round(t_setup, 4), "converged": bool(ok and r <= tol),
+               "iterations": its, "true_residual": r}
+        log(f"{label}: {rec}")
+        res[label] = rec
+
+    run_bicgstab("bicgstab_jacobi", lambda: (solvers.Diagonal(A), {}))
+    run_bicgstab("bicgstab_learned_spai", learned)
     # probe loss of the two inverses on the solve matrix
     Z = torch.from_numpy(np.random.default_rng(5).integers(0, 2, (4, n)) * 2.0 - 1.0).cuda()
     L_learned, _ = net.loss_grad(A, Z)
@@ -606,7 +644,9 @@ def wl_gmres_convdiff(args):
                        "probe_loss_on_solve_matrix": {"jacobi": L_jacobi, "learned": L_learned},
                        "gmres_variant": args.gmres_variant,
                        "note": "left-preconditioned GMRES(30); the solve time of the learned preconditioner includes "
-                               "the network's inference on the solve matrix; ILU(0) capped at --ilu-max-iterations"})
+                               "the network's inference on the solve matrix; ILU(0) capped at --ilu-max-iterations; "
+                               "bicgstab_* are BiCGStab on the same system to the same true residual (beside the "
+                               "GMRES comparison the value is quoted on)"})
     its_l = res["learned_spai"]["iterations"]
     out["roofline"] = roofline(2 * nb * its_l / (t_l - res["learned_spai"]["setup_seconds"]) / 1e9)
     out["roofline"]["kernel"] = "SpMV with A and with M per GMRES iteration (orthogonalisation not counted)"
