@@ -682,10 +682,11 @@ def wl_spmv_symmetric(args):
         ms = timed(lambda: A.vectorProduct(xd, yd), steps, 5)
         res[name] = {"ms": round(ms, 6), "stored_entries": int(len(c)),
                      "GBs_of_stored_bytes": round(spmv_bytes(n, len(c)) / (ms * 1e-3) / 1e9, 3)}
+        y_now = yd.cpu().numpy()
         if name == "CSR full":
-            y_full = yd.cpu().numpy()
+            y_full = y_now
         else:
-            res[name]["max_rel_diff_vs_full"] = float(np.abs(yd.cpu().numpy() - y_full).max() / np.abs(y_full).max())
+            res[name]["max_rel_diff_vs_full"] = float(np.abs(y_now - y_full).max() / np.abs(y_full).max())
         log(f"Poisson 256^3 {name}: {ms * 1e3:.1f} us")
         del A
     best = res["Ellpack symmetric"]
@@ -697,6 +698,20 @@ def wl_spmv_symmetric(args):
                        "stored_entries": int(len(lcols)), "bytes_per_spmv": nbytes, "variants": res})
     out["roofline"] = roofline(gbs)
     out["cpu_baseline"] = None
+    ref = _oracle_lib().Reference.load() if not args.no_cpu else None
+    if ref is not None:
+        # TNL's Host SymmetricMatrix<CSR> product of the same lower triangle, all host cores
+        cores = cpu_cores()
+        x = xd.cpu().numpy()
+        t1, _ = ref.spmv_symmetric_time(loff, lcols, lvals, x, 1, cores)
+        reps = int(max(3, min(50, args.cpu_budget / max(t1, 1e-6))))
+        t, y_cpu = ref.spmv_symmetric_time(loff, lcols, lvals, x, reps, cores)
+        out["cpu_baseline"] = {"value": round(nbytes / t / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "reference",
+                               "sample": f"median of {reps} products of the same lower triangle (TNL Host "
+                                         f"SymmetricMatrix<CSR> vectorProduct, OpenMP atomics, {cores} threads)"}
+        out["parity_max_rel_err_vs_cpu"] = float(np.abs(y_now - y_cpu).max() / np.abs(y_cpu).max())
+        log(f"CPU baseline: {out['cpu_baseline']['value']} GB/s ({t * 1e3:.1f} ms/SpMV, {cores} threads), "
+            f"GPU vs CPU max rel err {out['parity_max_rel_err_vs_cpu']:.2e}")
     return out
 
 

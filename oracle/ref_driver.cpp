@@ -514,6 +514,35 @@ double ref_spmv_csr_time( int64_t n, int64_t ncols, const int32_t* row_ptr, cons
    return t.empty() ? 0.0 : t[ t.size() / 2 ];
 }
 
+// the SymmetricMatrix product (CSR storage of the lower triangle) timed as ref_spmv_csr_time
+double ref_spmv_symmetric_time( int64_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals,
+                                const double* x, double* y, int reps, int threads )
+{
+   Host::setMaxThreadsCount( threads );
+   SymMat< Seg::CSR > m;
+   m.setDimensions( n, n );
+   IVec caps( n );
+   for( int64_t r = 0; r < n; r++ )
+      caps[ r ] = row_ptr[ r + 1 ] - row_ptr[ r ];
+   m.setRowCapacities( caps );
+   const int64_t nnz = row_ptr[ n ];
+   std::copy( cols, cols + nnz, m.getColumnIndexes().getData() );
+   std::copy( vals, vals + nnz, m.getValues().getData() );
+   DVec xv( n ), yv( n );
+   std::copy( x, x + n, xv.getData() );
+   m.vectorProduct( xv, yv );  // warm-up
+   std::vector< double > t;
+   for( int i = 0; i < reps; i++ ) {
+      const auto t0 = std::chrono::steady_clock::now();
+      m.vectorProduct( xv, yv );
+      const auto t1 = std::chrono::steady_clock::now();
+      t.push_back( std::chrono::duration< double >( t1 - t0 ).count() );
+   }
+   std::copy( yv.getData(), yv.getData() + n, y );
+   std::sort( t.begin(), t.end() );
+   return t.empty() ? 0.0 : t[ t.size() / 2 ];
+}
+
 void ref_cg( int64_t n, const int32_t* rp, const int32_t* c, const double* v, const double* b, double* x,
              int precond, State* st )
 {

@@ -390,6 +390,16 @@ class Reference:
                                    P(x), P(y), ctypes.c_double(alpha), ctypes.c_double(beta), I64(begin), I64(end))
         return y
 
+    def spmv_symmetric_time(self, off, cols, vals, x, reps, threads):
+        """Median seconds of TNL's Host SymmetricMatrix<CSR> vectorProduct (lower-triangle
+        CSR arrays, the matrix built once) and its last result."""
+        off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
+        n = len(off) - 1
+        y = np.zeros(n)
+        self.L.ref_spmv_symmetric_time.restype = ctypes.c_double
+        t = self.L.ref_spmv_symmetric_time(I64(n), P(off), P(cols), P(vals), P(x), P(y), int(reps), int(threads))
+        return t, y
+
     def save_symmetric_file(self, fmt, off, cols, vals, ncols, path, col_major=True):
         off, cols, vals = i32(off), i32(cols), f64(vals)
         rc = self.L.ref_save_symmetric(self.FORMATS[fmt], int(col_major), I64(len(off) - 1), I64(ncols), P(off),
