@@ -1,10 +1,12 @@
 """Benchmarks of the hot path; each workload prints one JSON line (rank 0).
 
 Default workload (`--workload spmv-powerlaw`): FP64 CSR SpMV y = A x on the
-power-law matrix through the CSR matrix's XCD column-split plan
-(tnlmi_spmv_plan, three kernels per product: split_main, split_fixup,
-split_reduce); the plain tiled CSR kernel over the CSR arrays is timed beside
-it and reported in `config`.  4,194,304 x 4,194,304, row lengths ~ Zipf(2) on
+power-law matrix through the CSR matrix's windowed plan (tnlmi_spmv_plan with
+parts 0: w_phase1 forms the products of each 8192-column window with x in LDS,
+w_phase2 sums them per row block in LDS, w_fixup adds the pieces of rows
+longer than a block); the XCD column-split plan and the plain tiled CSR
+kernel are timed beside it and reported in `config`.  4,194,304 x 4,194,304,
+row lengths ~ Zipf(2) on
 [1, 4096] scaled to mean 16 (67,108,864 nonzeros), uniform distinct sorted
 columns, values ~ N(0,1), x ~ U[-1,1], int32 indices, FP64 values/vectors, all
 resident in HBM before the timed region; value = algorithmic bytes / time with

@@ -182,6 +182,46 @@ def test_errors():
         A.vectorProduct(x, x.clone(), begin=10, end=5)
 
 
+def test_csr_short_rows_one_launch(orc):
+    # no row longer than 32 elements: the tiled kernel's owner finishes its last
+    # row past the tile end (no carries, no fixup launch).  Tiles of >= 64 rows
+    # sum every row left to right from zero: bit-exact with TNL's host loop.
+    off, cols, vals = tnl.laplace2d(512, 512)
+    n = len(off) - 1
+    x = tnl.uniform(n, -1.0, 1.0, seed=0)
+    want = orc.spmv(off, cols, vals, x)
+    for fmt in (tnl.CSR, tnl.SORTED_CSR):
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt)
+        assert np.array_equal(run(A, x), want), fmt
+    ref = OL.Reference.load()
+    if ref is not None:
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+        assert np.array_equal(run(A, x), ref.spmv("CSR", off, cols, vals, x))
+    rng = np.random.default_rng(31)
+    # longest row 32 (one launch) and 33 (carries): empty rows, padding slots,
+    # rows of exactly 32 (16 rows per tile: 4 lanes per row), row ranges
+    for n, m, maxlen, full in [(5000, 7000, 32, False), (4000, 9000, 32, True), (5000, 7000, 33, False),
+                               (700, 40, 12, False)]:
+        off, cols, vals = random_csr(n, m, maxlen, rng)
+        if full:
+            off, cols, vals = random_csr(n, m, maxlen, rng, pad=0.0, long_rows=[(r, 32) for r in range(n)])
+        for fmt in (tnl.CSR, tnl.SORTED_CSR):
+            A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, fmt=fmt)
+            xs = rng.standard_normal(m)
+            y0 = rng.standard_normal(n)
+            for alpha, beta, b, e in [(1, 0, 0, 0), (-2.0, 0.5, 0, 0), (0.75, -1.25, n // 3, n - n // 5)]:
+                want = orc.spmv(off, cols, vals, xs, y=y0, alpha=alpha, beta=beta, begin=b, end=e)
+                got = run(A, xs, y0, alpha=alpha, beta=beta, begin=b, end=e)
+                assert relerr(got, want) <= 1e-13, (n, maxlen, fmt, alpha, beta, b, e)
+                if b or e:
+                    assert np.array_equal(got[:b], y0[:b]) and np.array_equal(got[e:], y0[e:])
+        A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, value_dtype=torch.float32)
+        xs = rng.standard_normal(m).astype(np.float32)
+        want = orc.spmv(off, cols, vals.astype(np.float32).astype(np.float64), xs.astype(np.float64))
+        got = run(A, xs, dtype=torch.float32)
+        assert relerr(got, want) <= 1e-5, (n, maxlen)
+
+
 @pytest.mark.parametrize("parts", [8, 16, 0], ids=["split8", "split16", "windowed"])
 def test_column_split_plan(orc, parts):
     rng = np.random.default_rng(parts)

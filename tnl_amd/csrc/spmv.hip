@@ -212,14 +212,17 @@ __global__ void __launch_bounds__( 256 ) csr_dynamic( const int32_t* __restrict_
 // wavefront therefore does the same amount of streaming work, whatever the
 // row-length distribution.  Workgroups are renumbered so that each XCD walks
 // one contiguous range of tiles (x entries shared by neighbouring rows stay in
-// one L2).
-template< typename V, typename X, typename C, bool MAPPED >
+// one L2).  WHOLE (no row longer than kWholeRowMax, e.g. stencil matrices):
+// the owner also reads the few elements of its last row past the tile end, so
+// every row is summed complete by one owner, no carries exist and the product
+// is one launch.
+template< typename V, typename X, typename C, bool MAPPED, bool WHOLE >
 __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ off,
                                                     const int32_t* __restrict__ tile_rows, int64_t tiles, int64_t nnz,
                                                     const int32_t* __restrict__ row_map, int32_t* __restrict__ carry_row,
-                                                    C* __restrict__ carry_val, Args< V, X, C > a )
+                                                    C* __restrict__ carry_val, Args< V, X, C > a, int32_t tail_max )
 {
-   __shared__ C lds[ 4 ][ kTileNnz ];
+   __shared__ C lds[ 4 ][ kTileNnz + ( WHOLE ? kWholeRowMax : 0 ) ];
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int64_t t = xcd_contiguous( blockIdx.x, gridDim.x ) * 4 + wave;
    if( t >= tiles )
@@ -229,7 +232,7 @@ __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ 
       const int64_t o = MAPPED ? (int64_t) row_map[ r ] : r;
       return o >= a.begin && o < a.end;
    };
-   csr_tile< V, X, C >(
+   csr_tile< V, X, C, WHOLE >(
       off, a.n_rows, a.cols, a.vals, a.x, t, tile_rows[ t ], tile_rows[ t + 1 ], nnz, lds[ wave ], lane,
       [ & ]( int64_t r, C v ) {
          if( in_range( r ) )
@@ -238,7 +241,8 @@ __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ 
       [ & ]( int64_t rc, C v ) {
          carry_row[ t ] = rc >= 0 && in_range( rc ) ? (int32_t) rc : -1;
          carry_val[ t ] = v;
-      } );
+      },
+      tail_max );
 }
 
 // Adds the carried partial sums of rows that span several tiles, in tile
@@ -540,15 +544,24 @@ void csr_tiled_launch( const tnlmi_segments& seg, const int32_t* off, const int3
    const dim3 grid( (unsigned) std::max< int64_t >( 1, ceil_div( seg.tiles, 4 ) ) );
    const dim3 fgrid( (unsigned) std::max< int64_t >( 1, ceil_div( seg.tiles, 256 ) ) );
    C* cv = reinterpret_cast< C* >( seg.carry_val.ptr );
-   if( row_map ) {
-      hipLaunchKernelGGL( ( csr_tiled< V, X, C, true > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr, seg.tiles,
-                          seg.elements, row_map, seg.carry_row.ptr, cv, a );
+   const int32_t tail_max = std::max( 0, seg.max_size - 1 );
+   if( seg.max_size <= kWholeRowMax ) {  // one launch, no carries
+      if( row_map )
+         hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, true > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
+                             seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, tail_max );
+      else
+         hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, true > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
+                             seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, tail_max );
+   }
+   else if( row_map ) {
+      hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, false > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
+                          seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, 0 );
       hipLaunchKernelGGL( ( csr_carry_fixup< X, C, true > ), fgrid, dim3( 256 ), 0, s, seg.tiles, seg.carry_row.ptr,
                           cv, row_map, a.y, a.alpha );
    }
    else {
-      hipLaunchKernelGGL( ( csr_tiled< V, X, C, false > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr, seg.tiles,
-                          seg.elements, row_map, seg.carry_row.ptr, cv, a );
+      hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, false > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
+                          seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, 0 );
       hipLaunchKernelGGL( ( csr_carry_fixup< X, C, false > ), fgrid, dim3( 256 ), 0, s, seg.tiles, seg.carry_row.ptr,
                           cv, row_map, a.y, a.alpha );
    }

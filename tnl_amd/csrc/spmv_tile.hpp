@@ -26,6 +26,8 @@ __device__ __forceinline__ void keep( X* __restrict__ y, int64_t r, C v, C alpha
 }
 
 constexpr int kTileUnroll = kTileNnz / 64;
+// longest row for which the tiled kernel completes rows in their owner's tile
+constexpr int32_t kWholeRowMax = 32;
 
 // Tile t of a CSR element stream (offsets `off`, n_rows rows, nnz elements):
 // the elements [t*kTileNnz, (t+1)*kTileNnz) and the rows [rs, re) whose first
@@ -38,15 +40,44 @@ constexpr int kTileUnroll = kTileNnz / 64;
 // the sum of each owned row's part in this tile -- 64/pow2(rows) lanes per
 // row, or one lane per row summing left to right from 0 (the reference's host
 // order) when the tile holds 64 rows or more.
-template< typename V, typename X, typename C, typename Store, typename Carry >
+// WHOLE (every row at most kWholeRowMax elements long): the owner of the row
+// that continues past the tile end also loads that row's remaining elements
+// (at most tail_max = longest row - 1, one per lane, beside the tile's own
+// loads) and stages their products after the tile's in s, so every owned row
+// is stored complete and carry is never called.  s then holds kTileNnz +
+// kWholeRowMax entries.
+template< typename V, typename X, typename C, bool WHOLE = false, typename Store, typename Carry >
 __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64_t n_rows,
                                           const int32_t* __restrict__ cols, const V* __restrict__ vals,
                                           const X* __restrict__ x, int64_t t, int64_t rs, int64_t re, int64_t nnz,
-                                          C* __restrict__ s, int lane, Store&& store, Carry&& carry )
+                                          C* __restrict__ s, int lane, Store&& store, Carry&& carry,
+                                          int32_t tail_max = 0 )
 {
    const int64_t e0 = t * kTileNnz;
    const int64_t e1 = min( e0 + kTileNnz, nnz );
    const int32_t nel = (int32_t) max( (int64_t) 0, e1 - e0 );
+   // WHOLE: elements [e1, off[re]) finish the last owned row; off[re] - e1 < the
+   // row's length <= tail_max + 1 because that row starts before e1
+   int32_t ct = -1;
+   C vt = 0;
+   if( WHOLE && lane < tail_max && e1 + lane < nnz ) {
+      ct = cols[ e1 + lane ];
+      vt = load_val< V, C >( vals, e1 + lane );
+   }
+   // row bounds of this lane's first row in phase 2, loaded beside the tile
+   // (phase 2 then waits on no further memory for rows of >= 64-row tiles'
+   // first pass, nor for any row of a smaller tile)
+   const int32_t nrows = (int32_t) ( re - rs );
+   int G = 64;  // lanes per row: largest power of two with nrows * G <= 64 (1 for >= 64 rows)
+   while( G > 1 && nrows * G > 64 )
+      G >>= 1;
+   const int64_t r_first = rs + lane / G;
+   int64_t fb = 0, fe = 0;
+   if( r_first < re ) {
+      fb = off[ r_first ];
+      fe = off[ r_first + 1 ];
+   }
+   const int64_t tail = WHOLE && re > rs ? (int64_t) off[ re ] - e1 : 0;
    {
       const int32_t* __restrict__ cp = cols + e0;
       const V* __restrict__ vp = vals + e0;
@@ -68,9 +99,11 @@ __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64
          if( i < nel )
             s[ i ] = c[ u ] >= 0 ? v[ u ] * (C) x[ c[ u ] ] : C( 0 );
       }
+      if( WHOLE && lane < tail )
+         s[ nel + lane ] = ct >= 0 ? vt * (C) x[ ct ] : C( 0 );
    }
    wave_sync();
-   {
+   if( ! WHOLE ) {
       // the row containing element e0 started earlier iff no owned row starts at e0
       int64_t rc = -1;
       int32_t cend = 0;
@@ -86,10 +119,13 @@ __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64
       if( lane == 0 )
          carry( rc, acc );
    }
-   const int32_t nrows = (int32_t) ( re - rs );
    if( nrows >= 64 ) {
       for( int64_t r = rs + lane; r < re; r += 64 ) {
-         const int32_t b = (int32_t) ( off[ r ] - e0 ), e = (int32_t) ( min( (int64_t) off[ r + 1 ], e1 ) - e0 );
+         if( r != r_first ) {
+            fb = off[ r ];
+            fe = off[ r + 1 ];
+         }
+         const int32_t b = (int32_t) ( fb - e0 ), e = (int32_t) ( ( WHOLE ? fe : min( fe, e1 ) ) - e0 );
          C acc = 0;
          for( int32_t i = b; i < e; i++ )
             acc = acc + s[ i ];
@@ -97,14 +133,11 @@ __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64
       }
    }
    else if( nrows > 0 ) {
-      int G = 64;  // lanes per row: largest power of two with nrows * G <= 64
-      while( G > 1 && nrows * G > 64 )
-         G >>= 1;
       const int grp = lane / G, gl = lane % G;
       C acc = 0;
-      const int64_t r = rs + grp;
+      const int64_t r = r_first;
       if( grp < nrows ) {
-         const int32_t b = (int32_t) ( off[ r ] - e0 ), e = (int32_t) ( min( (int64_t) off[ r + 1 ], e1 ) - e0 );
+         const int32_t b = (int32_t) ( fb - e0 ), e = (int32_t) ( ( WHOLE ? fe : min( fe, e1 ) ) - e0 );
          for( int32_t i = b + gl; i < e; i += G )
             acc += s[ i ];
       }
