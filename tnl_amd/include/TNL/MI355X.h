@@ -29,6 +29,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <variant>
 #include <vector>
@@ -387,7 +388,12 @@ struct SymmetricMatrix
 template< typename Real, typename Device, typename Index = int, typename MatrixType = GeneralMatrix,
           template< typename, typename, typename > class Segments = Algorithms::Segments::CSR,
           typename ComputeReal = Real >
-class SparseMatrix;
+class SparseMatrix
+{
+   // the device kernels take int32 offsets and column indexes (TNL's default
+   // Index = int); 64-bit indices are not provided
+   static_assert( std::is_same_v< Index, int >, "SparseMatrix on Devices::MI355X supports Index = int only" );
+};
 
 //! Matrices::SparseMatrix on the MI355X (SparseMatrix.h:41-56), general
 //! matrix, int32 indices, FP64 or FP32 values.
