@@ -184,22 +184,24 @@ def test_errors():
 
 def test_csr_short_rows_one_launch(orc):
     # no row longer than 32 elements: the tiled kernel's owner finishes its last
-    # row past the tile end (no carries, no fixup launch).  Tiles of >= 64 rows
-    # sum every row left to right from zero: bit-exact with TNL's host loop.
-    off, cols, vals = tnl.laplace2d(512, 512)
-    n = len(off) - 1
-    x = tnl.uniform(n, -1.0, 1.0, seed=0)
-    want = orc.spmv(off, cols, vals, x)
-    for fmt in (tnl.CSR, tnl.SORTED_CSR):
-        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt)
-        assert np.array_equal(run(A, x), want), fmt
+    # row past the tile end (no carries, no fixup launch) and one lane sums each
+    # row left to right from zero: bit-exact with TNL's host loop.  Matrices of
+    # at most 2^22 elements get 256-element tiles, larger ones 512.
+    for side in (1100, 512):  # 6,046,000 elements (512-element tiles), then 1,308,672 (256)
+        off, cols, vals = tnl.laplace2d(side, side)
+        n = len(off) - 1
+        x = tnl.uniform(n, -1.0, 1.0, seed=0)
+        want = orc.spmv(off, cols, vals, x)
+        for fmt in (tnl.CSR, tnl.SORTED_CSR):
+            A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt)
+            assert np.array_equal(run(A, x), want), (side, fmt)
     ref = OL.Reference.load()
     if ref is not None:
         A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
         assert np.array_equal(run(A, x), ref.spmv("CSR", off, cols, vals, x))
     rng = np.random.default_rng(31)
     # longest row 32 (one launch) and 33 (carries): empty rows, padding slots,
-    # rows of exactly 32 (16 rows per tile: 4 lanes per row), row ranges
+    # rows of exactly 32 (8 rows per tile), row ranges
     for n, m, maxlen, full in [(5000, 7000, 32, False), (4000, 9000, 32, True), (5000, 7000, 33, False),
                                (700, 40, 12, False)]:
         off, cols, vals = random_csr(n, m, maxlen, rng)
@@ -213,6 +215,8 @@ def test_csr_short_rows_one_launch(orc):
                 want = orc.spmv(off, cols, vals, xs, y=y0, alpha=alpha, beta=beta, begin=b, end=e)
                 got = run(A, xs, y0, alpha=alpha, beta=beta, begin=b, end=e)
                 assert relerr(got, want) <= 1e-13, (n, maxlen, fmt, alpha, beta, b, e)
+                if maxlen <= 32 and (alpha, beta) == (1, 0):
+                    assert np.array_equal(got, want), (n, maxlen, fmt)
                 if b or e:
                     assert np.array_equal(got[:b], y0[:b]) and np.array_equal(got[e:], y0[e:])
         A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, value_dtype=torch.float32)

@@ -415,8 +415,8 @@ void build( tnlmi_segments& seg, const int32_t* d_sizes, hipStream_t s )
    switch( seg.format ) {
       case TNLMI_CSR:
       case TNLMI_ADAPTIVE_CSR: {
-         seg.tile_nnz = kTileNnz;
-         seg.tiles = std::max< int64_t >( 1, ceil_div( total, kTileNnz ) );
+         seg.tile_nnz = csr_tile_size( seg.max_size, total );
+         seg.tiles = std::max< int64_t >( 1, ceil_div( total, seg.tile_nnz ) );
          seg.tile_rows.allocate( seg.tiles + 1 );
          hipLaunchKernelGGL( csr_tile_rows, dim3( ceil_div( seg.tiles + 1, 256 ) ), dim3( 256 ), 0, s,
                              seg.offsets.ptr, n, seg.tiles, seg.tile_nnz, seg.tile_rows.ptr );
@@ -512,8 +512,8 @@ void build( tnlmi_segments& seg, const int32_t* d_sizes, hipStream_t s )
          }
          exclusive_scan_i32( keys_out.ptr, seg.sorted_offsets.ptr, n, s );
          // the tiled CSR plan over the embedded (sorted) CSR
-         seg.tile_nnz = kTileNnz;
-         seg.tiles = std::max< int64_t >( 1, ceil_div( total, kTileNnz ) );
+         seg.tile_nnz = csr_tile_size( seg.max_size, total );
+         seg.tiles = std::max< int64_t >( 1, ceil_div( total, seg.tile_nnz ) );
          seg.tile_rows.allocate( seg.tiles + 1 );
          hipLaunchKernelGGL( csr_tile_rows, dim3( ceil_div( seg.tiles + 1, 256 ) ), dim3( 256 ), 0, s,
                              seg.sorted_offsets.ptr, n, seg.tiles, seg.tile_nnz, seg.tile_rows.ptr );

@@ -22,7 +22,7 @@ struct tnlmi_segments
    tnlmi::DeviceArray< int32_t > sizes;
    tnlmi::DeviceArray< int32_t > offsets;  // exclusive scan of sizes, n+1
 
-   // CSR tiled kernel plan: tile t covers elements [t*kTileNnz, (t+1)*kTileNnz)
+   // CSR tiled kernel plan: tile t covers elements [t*tile_nnz, (t+1)*tile_nnz)
    // and owns rows [tile_rows[t], tile_rows[t+1]), the rows whose first
    // element lies there; carry_row / carry_val hold the partial sums of rows
    // continued from an earlier tile (scratch of the SpMV: concurrent products
@@ -68,6 +68,17 @@ namespace tnlmi {
 
 // elements per wavefront tile of the tiled CSR kernel
 constexpr int32_t kTileNnz = 512;
+// longest row for which the tiled kernel completes rows in their owner's tile
+// (one launch, no carries)
+constexpr int32_t kWholeRowMax = 32;
+// such a matrix of at most kSmallTileElems elements gets tiles of
+// kTileNnzSmall: twice the wavefronts to hide a small product's memory latency
+constexpr int32_t kTileNnzSmall = 256;
+constexpr int64_t kSmallTileElems = int64_t( 1 ) << 22;
+inline int32_t csr_tile_size( int32_t max_size, int64_t total )
+{
+   return max_size <= kWholeRowMax && total <= kSmallTileElems ? kTileNnzSmall : kTileNnz;
+}
 
 // Plain-value view of a segments object for kernels.
 struct SegView

@@ -216,13 +216,13 @@ __global__ void __launch_bounds__( 256 ) csr_dynamic( const int32_t* __restrict_
 // the owner also reads the few elements of its last row past the tile end, so
 // every row is summed complete by one owner, no carries exist and the product
 // is one launch.
-template< typename V, typename X, typename C, bool MAPPED, bool WHOLE >
+template< typename V, typename X, typename C, bool MAPPED, bool WHOLE, int TILE >
 __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ off,
                                                     const int32_t* __restrict__ tile_rows, int64_t tiles, int64_t nnz,
                                                     const int32_t* __restrict__ row_map, int32_t* __restrict__ carry_row,
                                                     C* __restrict__ carry_val, Args< V, X, C > a, int32_t tail_max )
 {
-   __shared__ C lds[ 4 ][ kTileNnz + ( WHOLE ? kWholeRowMax : 0 ) ];
+   __shared__ C lds[ 4 ][ TILE + ( WHOLE ? kWholeRowMax : 0 ) ];
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int64_t t = xcd_contiguous( blockIdx.x, gridDim.x ) * 4 + wave;
    if( t >= tiles )
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__( 256 ) csr_tiled( const int32_t* __restrict__ 
       const int64_t o = MAPPED ? (int64_t) row_map[ r ] : r;
       return o >= a.begin && o < a.end;
    };
-   csr_tile< V, X, C, WHOLE >(
+   csr_tile< V, X, C, WHOLE, TILE >(
       off, a.n_rows, a.cols, a.vals, a.x, t, tile_rows[ t ], tile_rows[ t + 1 ], nnz, lds[ wave ], lane,
       [ & ]( int64_t r, C v ) {
          if( in_range( r ) )
@@ -545,23 +545,35 @@ void csr_tiled_launch( const tnlmi_segments& seg, const int32_t* off, const int3
    const dim3 fgrid( (unsigned) std::max< int64_t >( 1, ceil_div( seg.tiles, 256 ) ) );
    C* cv = reinterpret_cast< C* >( seg.carry_val.ptr );
    const int32_t tail_max = std::max( 0, seg.max_size - 1 );
+   if( seg.tile_nnz != kTileNnz && ! ( seg.tile_nnz == kTileNnzSmall && seg.max_size <= kWholeRowMax ) )
+      fail( TNLMI_RUNTIME_ERROR, "vectorProduct: inconsistent CSR tile table" );
    if( seg.max_size <= kWholeRowMax ) {  // one launch, no carries
-      if( row_map )
-         hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, true > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
-                             seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, tail_max );
+      if( seg.tile_nnz == kTileNnzSmall ) {
+         if( row_map )
+            hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, true, kTileNnzSmall > ), grid, dim3( 256 ), 0, s, off,
+                                seg.tile_rows.ptr, seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a,
+                                tail_max );
+         else
+            hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, true, kTileNnzSmall > ), grid, dim3( 256 ), 0, s, off,
+                                seg.tile_rows.ptr, seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a,
+                                tail_max );
+      }
+      else if( row_map )
+         hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, true, kTileNnz > ), grid, dim3( 256 ), 0, s, off,
+                             seg.tile_rows.ptr, seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, tail_max );
       else
-         hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, true > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
-                             seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, tail_max );
+         hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, true, kTileNnz > ), grid, dim3( 256 ), 0, s, off,
+                             seg.tile_rows.ptr, seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, tail_max );
    }
    else if( row_map ) {
-      hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, false > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
-                          seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, 0 );
+      hipLaunchKernelGGL( ( csr_tiled< V, X, C, true, false, kTileNnz > ), grid, dim3( 256 ), 0, s, off,
+                          seg.tile_rows.ptr, seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, 0 );
       hipLaunchKernelGGL( ( csr_carry_fixup< X, C, true > ), fgrid, dim3( 256 ), 0, s, seg.tiles, seg.carry_row.ptr,
                           cv, row_map, a.y, a.alpha );
    }
    else {
-      hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, false > ), grid, dim3( 256 ), 0, s, off, seg.tile_rows.ptr,
-                          seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, 0 );
+      hipLaunchKernelGGL( ( csr_tiled< V, X, C, false, false, kTileNnz > ), grid, dim3( 256 ), 0, s, off,
+                          seg.tile_rows.ptr, seg.tiles, seg.elements, row_map, seg.carry_row.ptr, cv, a, 0 );
       hipLaunchKernelGGL( ( csr_carry_fixup< X, C, false > ), fgrid, dim3( 256 ), 0, s, seg.tiles, seg.carry_row.ptr,
                           cv, row_map, a.y, a.alpha );
    }

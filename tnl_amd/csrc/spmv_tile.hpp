@@ -25,16 +25,13 @@ __device__ __forceinline__ void keep( X* __restrict__ y, int64_t r, C v, C alpha
       y[ r ] = alpha == C( 1 ) ? X( beta * (C) y[ r ] + v ) : X( beta * (C) y[ r ] + alpha * v );
 }
 
-constexpr int kTileUnroll = kTileNnz / 64;
-// longest row for which the tiled kernel completes rows in their owner's tile
-constexpr int32_t kWholeRowMax = 32;
 
 // Tile t of a CSR element stream (offsets `off`, n_rows rows, nnz elements):
-// the elements [t*kTileNnz, (t+1)*kTileNnz) and the rows [rs, re) whose first
+// the elements [t*TILE, (t+1)*TILE) and the rows [rs, re) whose first
 // element lies there.
 // Phase 1 streams the tile's columns/values once with non-temporal, fully
-// coalesced loads (kTileUnroll independent loads per lane in flight), gathers
-// x and stages the products in LDS (s, kTileNnz entries of this wavefront).
+// coalesced loads (TILE/64 independent loads per lane in flight), gathers
+// x and stages the products in LDS (s, TILE entries of this wavefront).
 // Phase 2: carry(rc, v) receives the partial sum of the row rc that started
 // in an earlier tile (rc = -1 if none) -- called by lane 0 only; store(r, v)
 // the sum of each owned row's part in this tile -- 64/pow2(rows) lanes per
@@ -44,17 +41,19 @@ constexpr int32_t kWholeRowMax = 32;
 // that continues past the tile end also loads that row's remaining elements
 // (at most tail_max = longest row - 1, one per lane, beside the tile's own
 // loads) and stages their products after the tile's in s, so every owned row
-// is stored complete and carry is never called.  s then holds kTileNnz +
-// kWholeRowMax entries.
-template< typename V, typename X, typename C, bool WHOLE = false, typename Store, typename Carry >
+// is stored complete and carry is never called; every row is then summed by
+// one lane, left to right from 0.  s then holds TILE + kWholeRowMax entries.
+template< typename V, typename X, typename C, bool WHOLE = false, int TILE = kTileNnz, typename Store,
+          typename Carry >
 __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64_t n_rows,
                                           const int32_t* __restrict__ cols, const V* __restrict__ vals,
                                           const X* __restrict__ x, int64_t t, int64_t rs, int64_t re, int64_t nnz,
                                           C* __restrict__ s, int lane, Store&& store, Carry&& carry,
                                           int32_t tail_max = 0 )
 {
-   const int64_t e0 = t * kTileNnz;
-   const int64_t e1 = min( e0 + kTileNnz, nnz );
+   constexpr int kTileUnroll = TILE / 64;
+   const int64_t e0 = t * TILE;
+   const int64_t e1 = min( e0 + TILE, nnz );
    const int32_t nel = (int32_t) max( (int64_t) 0, e1 - e0 );
    // WHOLE: elements [e1, off[re]) finish the last owned row; off[re] - e1 < the
    // row's length <= tail_max + 1 because that row starts before e1
@@ -68,8 +67,8 @@ __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64
    // (phase 2 then waits on no further memory for rows of >= 64-row tiles'
    // first pass, nor for any row of a smaller tile)
    const int32_t nrows = (int32_t) ( re - rs );
-   int G = 64;  // lanes per row: largest power of two with nrows * G <= 64 (1 for >= 64 rows)
-   while( G > 1 && nrows * G > 64 )
+   int G = 64;  // lanes per row: largest power of two with nrows * G <= 64 (1 for >= 64 rows or WHOLE)
+   while( G > 1 && ( WHOLE || nrows * G > 64 ) )
       G >>= 1;
    const int64_t r_first = rs + lane / G;
    int64_t fb = 0, fe = 0;
@@ -119,7 +118,7 @@ __device__ __forceinline__ void csr_tile( const int32_t* __restrict__ off, int64
       if( lane == 0 )
          carry( rc, acc );
    }
-   if( nrows >= 64 ) {
+   if( WHOLE || nrows >= 64 ) {
       for( int64_t r = rs + lane; r < re; r += 64 ) {
          if( r != r_first ) {
             fb = off[ r ];
