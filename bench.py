@@ -90,10 +90,33 @@ def cpu_spmv_baseline(off, cols, vals, x, y_gpu, nbytes, budget_s):
             ts.append(time.perf_counter() - t0)
         t = sorted(ts)[len(ts) // 2]
     err = float(np.abs(y_gpu - y).max() / max(np.abs(y).max(), 1e-300))
+    # the same product on one thread (a few repetitions within a quarter of the budget)
+    reps1 = int(max(1, min(5, budget_s / 4 / max(t * cores, 1e-6))))
+    if ref is not None:
+        t_one, _ = ref.spmv_time(off, cols, vals, x, reps1, 1)
+    else:
+        ts = []
+        for _ in range(reps1):
+            t0 = time.perf_counter()
+            orc.spmv_omp(off, cols, vals, x, 1)
+            ts.append(time.perf_counter() - t0)
+        t_one = sorted(ts)[len(ts) // 2]
     return dict(value=round(nbytes / t / 1e9, 3), unit="GB/s", cores=cores, kind=kind,
                 sample=f"median of {reps} full products of the same {len(y)}-row matrix (TNL Host CSR "
                        f"vectorProduct, OpenMP schedule(dynamic,100), {cores} threads)",
+                one_thread_value=round(nbytes / t_one / 1e9, 3), cpu_model=cpu_model(),
                 seconds_per_spmv=t), err
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def timed(op, steps, warmup, flush=None):
@@ -203,7 +226,8 @@ def wl_spmv_powerlaw(args):
     if not args.no_cpu:
         y_gpu = yd.cpu().numpy()
         cb, err = cpu_spmv_baseline(off, cols, vals, x, y_gpu, nbytes, args.cpu_budget)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "one_thread_value",
+                                                  "cpu_model")}
         out["parity_max_rel_err_vs_cpu"] = err
         log(f"CPU baseline: {cb['value']} GB/s ({cb['seconds_per_spmv'] * 1e3:.1f} ms/SpMV, {cb['cores']} threads, "
             f"{cb['kind']}), GPU vs CPU max rel err {err:.2e}")
@@ -248,7 +272,8 @@ def wl_spmv_laplace(args):
     out["roofline"] = roofline(gbs)
     if not args.no_cpu:
         cb, err = cpu_spmv_baseline(off, cols, vals, x, yd.cpu().numpy(), nbytes, args.cpu_budget)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "one_thread_value",
+                                                  "cpu_model")}
         out["parity_max_rel_err_vs_cpu"] = err
     else:
         out["cpu_baseline"] = None
