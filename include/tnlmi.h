@@ -170,6 +170,27 @@ tnlmi_status tnlmi_matrix_row_lengths( const tnlmi_matrix* A, int32_t* d_lengths
 tnlmi_status tnlmi_matrix_copy( const tnlmi_matrix* src, tnlmi_matrix* dst, const int32_t* d_lengths,
                                 tnlmi_stream stream );
 
+/* ---------------------------------------------------- element access */
+/* SparseMatrixBase::addElement( row, column, value, thisElementMultiplicator )
+ * (Matrices/SparseMatrixBase.hpp:166-231; setElement, :152-161, is
+ * this_element_multiplicator = 0): writes A's storage.  The row's
+ * getSegmentSize slots (Ellpack: the full width; SlicedEllpack: the slice's
+ * width) are walked in getGlobalIndex order: a stored `column` becomes
+ * multiplicator * old + value (rounded once per operation); otherwise the
+ * element takes the first padding slot, or the slot of the first larger
+ * column with the slots after it shifted right (a full row loses its last
+ * slot, as in the reference).  A symmetric matrix stores (column, row) for
+ * row < column.  TNLMI_LOGIC_ERROR "The capacity of the sparse matrix row
+ * number r was exceeded." when the row has neither; TNLMI_INVALID_ARGUMENT for
+ * a position outside the matrix.  Synchronous. */
+tnlmi_status tnlmi_matrix_add_element( tnlmi_matrix* A, int64_t row, int64_t column, double value,
+                                       double this_element_multiplicator, tnlmi_stream stream );
+/* SparseMatrixBase::getElement( row, column ) (SparseMatrixBase.hpp:231-265):
+ * *value = the stored element (FP32 / BF16 widened), 0 if the row holds no
+ * such column.  Synchronous. */
+tnlmi_status tnlmi_matrix_get_element( const tnlmi_matrix* A, int64_t row, int64_t column, double* value,
+                                       tnlmi_stream stream );
+
 /* ---------------------------------------------------- MatrixMarket input */
 /* A MatrixMarket coordinate file (real / integer / pattern, general /
  * symmetric) read into a host CSR matrix: rows ascending, columns ascending

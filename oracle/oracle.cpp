@@ -559,6 +559,49 @@ void orc_spmv_symmetric( int64_t rows, const int32_t* row_ptr, const int32_t* co
 // SparseMatrixBase.hpp:541-588 (general matrix): y *= beta (whole vector,
 // beta != 1), then y[c] += alpha * x[r] * v for every element with c in
 // [begin, end), rows in order.
+// SparseMatrixBase.hpp:166-231: the row's slots in order; a stored column is
+// updated (multiplicator * old + value, two roundings), else the first padding
+// slot or the first larger column takes the element -- for the latter the
+// slots from it to the row end move one place right first, the last slot's
+// content falling off (the reference does not check for a full row there)
+int64_t orc_add_elements( int64_t n, const int32_t* off, int32_t* cols, double* vals, int symmetric,
+                          const int64_t* op_pos, const double* op_val, int64_t nops )
+{
+   (void) n;
+   for( int64_t i = 0; i < nops; i++ ) {
+      int64_t r = op_pos[ 2 * i ], c = op_pos[ 2 * i + 1 ];
+      const double value = op_val[ 2 * i ], mult = op_val[ 2 * i + 1 ];
+      if( symmetric && r < c )
+         std::swap( r, c );
+      const int64_t b = off[ r ], size = off[ r + 1 ] - b;
+      int64_t k = 0;
+      bool done = false;
+      for( ; k < size; k++ ) {
+         const int32_t col = cols[ b + k ];
+         if( col == c ) {
+            const double scaled = mult * vals[ b + k ];
+            vals[ b + k ] = scaled + value;
+            done = true;
+            break;
+         }
+         if( col == -1 || col > c )
+            break;
+      }
+      if( done )
+         continue;
+      if( k == size )
+         return i;
+      if( cols[ b + k ] != -1 )
+         for( int64_t j = size - 1; j > k; j-- ) {
+            cols[ b + j ] = cols[ b + j - 1 ];
+            vals[ b + j ] = vals[ b + j - 1 ];
+         }
+      cols[ b + k ] = (int32_t) c;
+      vals[ b + k ] = value;
+   }
+   return -1;
+}
+
 void orc_spmv_transposed( int64_t rows, int64_t columns, const int32_t* row_ptr, const int32_t* cols,
                           const double* vals, const double* x, double* y, double alpha, double beta, int64_t begin,
                           int64_t end )

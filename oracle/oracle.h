@@ -116,6 +116,13 @@ void orc_spmv_symmetric( int64_t rows, const int32_t* row_ptr, const int32_t* co
                          const double* x, double* y, double alpha, double beta, int64_t begin, int64_t end );
 /* SparseMatrixBase.hpp:541-588: transposedVectorProduct of a general matrix
  * (x has rows entries, y columns; [begin, end) ranges over columns). */
+/* SparseMatrixBase::addElement / setElement (SparseMatrixBase.hpp:152-231) on
+ * CSR-ordered slots (row r's slots [off[r], off[r+1]) in getGlobalIndex order),
+ * ops (row, column) in op_pos and (value, multiplicator) in op_val; symmetric
+ * stores (column, row) for row < column.  Returns the index of the first op
+ * whose row has no room (the reference throws there), -1 if none. */
+int64_t orc_add_elements( int64_t n, const int32_t* off, int32_t* cols, double* vals, int symmetric,
+                          const int64_t* op_pos, const double* op_val, int64_t nops );
 void orc_spmv_transposed( int64_t rows, int64_t columns, const int32_t* row_ptr, const int32_t* cols,
                           const double* vals, const double* x, double* y, double alpha, double beta, int64_t begin,
                           int64_t end );

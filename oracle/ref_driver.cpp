@@ -113,6 +113,40 @@ int64_t fill_impl( int64_t n, int64_t ncols, const int32_t* row_ptr, const int32
    return storage;
 }
 
+// message of the last exception a driver call caught
+static std::string g_err;
+
+// setRowCapacities from the CSR arrays (columns -1: empty rows of that
+// capacity), then addElement( row, column, value, multiplicator ) per op until
+// one throws (its index in *failed, its message in g_err; -1 if none); the
+// storage arrays as fill_impl exports them
+template< class M >
+int64_t elements_impl( int64_t n, int64_t ncols, const int32_t* row_ptr, const int32_t* cols, const double* vals,
+                       const int64_t* op_pos, const double* op_val, int64_t nops, int64_t* failed, int32_t* out_cols,
+                       double* out_vals, int64_t cap )
+{
+   M m;
+   build( m, n, ncols, row_ptr, cols, vals );
+   *failed = -1;
+   for( int64_t i = 0; i < nops; i++ ) {
+      try {
+         m.addElement( (int) op_pos[ 2 * i ], (int) op_pos[ 2 * i + 1 ], op_val[ 2 * i ], op_val[ 2 * i + 1 ] );
+      }
+      catch( const std::logic_error& e ) {
+         g_err = e.what();
+         *failed = i;
+         break;
+      }
+   }
+   const int64_t storage = m.getValues().getSize();
+   if( storage <= cap )
+      for( int64_t i = 0; i < storage; i++ ) {
+         out_cols[ i ] = m.getColumnIndexes()[ i ];
+         out_vals[ i ] = m.getValues()[ i ];
+      }
+   return storage;
+}
+
 template< class M >
 void spmv_impl( int64_t n, int64_t ncols, const int32_t* row_ptr, const int32_t* cols, const double* vals,
                 const double* x, double* y, double alpha, double beta, int64_t begin, int64_t end )
@@ -209,6 +243,27 @@ struct TransSpmvOp
       for( int64_t i = 0; i < ncols; i++ )
          y[ i ] = yv[ i ];
       return 0;
+   }
+};
+
+template< template< class, class, class > class S >
+struct ElementsOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v,
+                       const int64_t* op, const double* ov, int64_t nops, int64_t* failed, int32_t* oc, double* os,
+                       int64_t cap )
+   {
+      return elements_impl< Mat< S > >( n, ncols, rp, c, v, op, ov, nops, failed, oc, os, cap );
+   }
+};
+template< template< class, class, class > class S >
+struct SymElementsOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v,
+                       const int64_t* op, const double* ov, int64_t nops, int64_t* failed, int32_t* oc, double* os,
+                       int64_t cap )
+   {
+      return elements_impl< SymMat< S > >( n, ncols, rp, c, v, op, ov, nops, failed, oc, os, cap );
    }
 };
 
@@ -640,7 +695,6 @@ void ref_ilu0_apply( int64_t n, const int32_t* rp, const int32_t* c, const doubl
 // ref_last_error().
 static std::vector< int > g_off, g_col;
 static std::vector< double > g_val;
-static std::string g_err;
 
 int ref_read_mtx( const char* path, int64_t* rows, int64_t* cols, int64_t* nnz )
 {
@@ -729,6 +783,19 @@ int64_t ref_save_symmetric( int format, int col_major, int64_t n, int64_t ncols,
       g_err = e.what();
       return -1;
    }
+}
+
+// SparseMatrixBase::addElement / setElement (SparseMatrixBase.hpp:152-231) on a
+// matrix of the given row capacities, general or symmetric
+int64_t ref_add_elements( int format, int col_major, int symmetric, int64_t n, int64_t ncols, const int32_t* row_ptr,
+                          const int32_t* cols, const double* vals, const int64_t* op_pos, const double* op_val,
+                          int64_t nops, int64_t* failed, int32_t* out_cols, double* out_vals, int64_t cap )
+{
+   if( symmetric )
+      return dispatch< SymElementsOp >( format, col_major, n, ncols, row_ptr, cols, vals, op_pos, op_val, nops, failed,
+                                        out_cols, out_vals, cap );
+   return dispatch< ElementsOp >( format, col_major, n, ncols, row_ptr, cols, vals, op_pos, op_val, nops, failed,
+                                  out_cols, out_vals, cap );
 }
 
 // OpenMP threads of TNL's Host algorithms in the following calls

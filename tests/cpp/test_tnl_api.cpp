@@ -149,6 +149,53 @@ int main()
    vectorProductKATs< Seg::ChunkedEllpack >( "ChunkedEllpack" );
    vectorProductKATs< Seg::SortedCSR >( "SortedCSR" );
 
+   // SparseMatrixExample_addElement.cpp: diagonal i, then addElement( i, j, 1, 5 )
+   // everywhere: 5 i + 1 on the diagonal, 1 elsewhere (inserted in column order)
+   {
+      Matrices::SparseMatrix< double, Device > M( { 5, 5, 5, 5, 5 }, 5 );
+      for( int i = 0; i < 5; i++ )
+         M.setElement( i, i, i );
+      for( int i = 0; i < 5; i++ )
+         for( int j = 0; j < 5; j++ )
+            M.addElement( i, j, 1.0, 5.0 );
+      for( int i = 0; i < 5; i++ )
+         for( int j = 0; j < 5; j++ )
+            EXPECT_EQ( M.getElement( i, j ), i == j ? 5.0 * i + 1.0 : 1.0 );
+      Vector x( 5, 1.0 ), y( 5 );
+      M.vectorProduct( x, y );
+      for( int i = 0; i < 5; i++ )
+         EXPECT_EQ( y.getElement( i ), 5.0 * i + 5.0 );
+      bool thrown = false;
+      try {
+         Matrices::SparseMatrix< double, Device > F( { 1, 1 }, 3 );
+         F.setElement( 0, 2, 1.0 );
+         F.setElement( 0, 1, 2.0 );  // full row: the reference shifts, the last slot falls off
+         EXPECT_EQ( F.getElement( 0, 1 ), 2.0 );
+         EXPECT_EQ( F.getElement( 0, 2 ), 0.0 );
+         F.setElement( 0, 2, 3.0 );  // full row, no larger column
+      }
+      catch( const std::logic_error& e ) {
+         thrown = std::string( e.what() ) == "The capacity of the sparse matrix row number 0 was exceeded.";
+      }
+      EXPECT_EQ( thrown, true );
+      // a symmetric matrix stores (j, i) for i < j and reads it back from either side
+      Matrices::SparseMatrix< double, Device, int, Matrices::SymmetricMatrix > S( { 1, 2, 2 }, 3 );
+      S.setElement( 0, 0, 4.0 );
+      S.setElement( 0, 1, -1.0 );
+      S.setElement( 1, 1, 4.0 );
+      S.setElement( 1, 2, -1.0 );
+      S.setElement( 2, 2, 4.0 );
+      EXPECT_EQ( S.getElement( 1, 0 ), -1.0 );
+      EXPECT_EQ( S.getElement( 1, 2 ), -1.0 );
+      EXPECT_EQ( S.getElement( 0, 2 ), 0.0 );
+      Vector s1( 3, 1.0 ), s2( 3 );
+      S.vectorProduct( s1, s2 );
+      EXPECT_EQ( s2.getElement( 0 ), 3.0 );
+      EXPECT_EQ( s2.getElement( 1 ), 2.0 );
+      EXPECT_EQ( s2.getElement( 2 ), 3.0 );
+      std::cout << "setElement / addElement / getElement: ok\n";
+   }
+
    // IterativeLinearSolverExample.cpp: tridiag(-1, 2.5, -1), b = A * 1
    using Matrix = Matrices::SparseMatrix< double, Device, int >;
    auto A = std::make_shared< Matrix >( 5, 5,

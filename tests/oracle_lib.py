@@ -41,7 +41,7 @@ class Oracle:
         if not os.path.exists(ORACLE_SO):
             raise RuntimeError("oracle/liboracle.so is not built: run __graft_entry__.build()")
         self.L = ctypes.CDLL(ORACLE_SO)
-        for name in ["orc_chunked_ellpack", "orc_adaptive_csr_blocks"]:
+        for name in ["orc_chunked_ellpack", "orc_adaptive_csr_blocks", "orc_add_elements"]:
             getattr(self.L, name).restype = ctypes.c_int64
         self.L.orc_true_residual.restype = ctypes.c_double
 
@@ -169,6 +169,17 @@ class Oracle:
                                    ctypes.c_double(beta), I64(begin), I64(end))
         return y
 
+    def add_elements(self, off, ops, symmetric=False):
+        """Empty rows of capacities diff(off), then addElement per op (row, column, value,
+        multiplicator) until a row is full.  Returns (CSR-ordered columns, values, failing op or -1)."""
+        off = i32(off)
+        cols, vals = np.full(int(off[-1]), -1, np.int32), np.zeros(int(off[-1]))
+        pos = np.ascontiguousarray([(o[0], o[1]) for o in ops], np.int64).reshape(-1)
+        val = np.ascontiguousarray([(o[2], o[3]) for o in ops], np.float64).reshape(-1)
+        failed = self.L.orc_add_elements(I64(len(off) - 1), P(off), P(cols), P(vals), int(symmetric), P(pos), P(val),
+                                         I64(len(ops)))
+        return cols, vals, int(failed)
+
     def spmv_omp(self, off, cols, vals, x, threads):
         off, cols, vals, x = i32(off), i32(cols), f64(vals), f64(x)
         n = len(off) - 1
@@ -270,7 +281,8 @@ class Reference:
         r = cls()
         r.L = ctypes.CDLL(REF_SO)
         for name in ["ref_chunked_ellpack", "ref_adaptive_csr_blocks", "ref_fill", "ref_spmv", "ref_save",
-                     "ref_load", "ref_spmv_symmetric", "ref_spmv_transposed", "ref_save_symmetric"]:
+                     "ref_load", "ref_spmv_symmetric", "ref_spmv_transposed", "ref_save_symmetric",
+                     "ref_add_elements"]:
             getattr(r.L, name).restype = ctypes.c_int64
         r.L.ref_last_error.restype = ctypes.c_char_p
         r.L.ref_spmv_csr_time.restype = ctypes.c_double
@@ -345,6 +357,24 @@ class Reference:
         self.L.ref_fill(self.FORMATS[fmt], int(col_major), I64(n), I64(ncols), P(off), P(cols), P(vals), P(oc),
                         P(ov), I64(cap))
         return oc, ov
+
+    def add_elements(self, fmt, off, ncols, ops, symmetric=False, col_major=True):
+        """TNL's SparseMatrix (general or symmetric) with row capacities diff(off), then
+        addElement(row, column, value, multiplicator) for each op until one throws.
+        Returns (storage columns, storage values, index of the failing op or -1, its message)."""
+        off = i32(off)
+        n = len(off) - 1
+        cols, vals = np.full(int(off[-1]), -1, np.int32), np.zeros(int(off[-1]))
+        pos = np.ascontiguousarray([(o[0], o[1]) for o in ops], np.int64).reshape(-1)
+        val = np.ascontiguousarray([(o[2], o[3]) for o in ops], np.float64).reshape(-1)
+        failed = np.zeros(1, np.int64)
+        args = (self.FORMATS[fmt], int(col_major), int(symmetric), I64(n), I64(ncols), P(off), P(cols), P(vals),
+                P(pos), P(val), I64(len(ops)), P(failed))
+        cap = int(self.L.ref_add_elements(*args, None, None, I64(0)))
+        oc, ov = np.zeros(cap, np.int32), np.zeros(cap)
+        self.L.ref_add_elements(*args, P(oc), P(ov), I64(cap))
+        msg = self.L.ref_last_error().decode() if failed[0] >= 0 else ""
+        return oc, ov, int(failed[0]), msg
 
     def save_file(self, fmt, off, cols, vals, ncols, path, col_major=True):
         """SparseMatrix::save of the matrix TNL builds from these CSR arrays (as fill)."""

@@ -1,0 +1,123 @@
+"""setElement / addElement / getElement on the GPU through the C ABI, every
+format and organisation, general and symmetric: the storage after each
+operation list equals the oracle's (bit-exact, mapped through the format's
+global indexes) and TNL's own Host build's storage arrays; the capacity error
+comes at the same operation with the reference's message."""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+tnl = pytest.importorskip("tnl_amd")
+
+FORMATS = [(tnl.CSR, 0, "CSR"), (tnl.ADAPTIVE_CSR, 0, "AdaptiveCSR"), (tnl.ELLPACK, 0, "Ellpack"),
+           (tnl.SLICED_ELLPACK, 32, "SlicedEllpack32"), (tnl.SLICED_ELLPACK, 64, "SlicedEllpack64"),
+           (tnl.BIELLPACK, 32, "BiEllpack"), (tnl.CHUNKED_ELLPACK, 0, "ChunkedEllpack"), (tnl.SORTED_CSR, 0, "SortedCSR")]
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return OL.Oracle()
+
+
+def apply(A, ops, failed):
+    """Applies ops up to the failing one, which must raise the reference's error."""
+    for i, (r, c, v, mult) in enumerate(ops):
+        if i == failed:
+            with pytest.raises(tnl.LogicError, match="The capacity of the sparse matrix row number"):
+                A.addElement(r, c, v, mult)
+            return
+        if mult == 0.0:
+            A.setElement(r, c, v)
+        else:
+            A.addElement(r, c, v, mult)
+
+
+CSR_FAMILY = (tnl.CSR, tnl.ADAPTIVE_CSR, tnl.SORTED_CSR)
+
+
+@pytest.mark.parametrize("fmt,param,name", FORMATS, ids=[f[2] for f in FORMATS])
+@pytest.mark.parametrize("org", [tnl.COLUMN_MAJOR, tnl.ROW_MAJOR], ids=["cm", "rm"])
+@pytest.mark.parametrize("symmetric", [False, True], ids=["general", "symmetric"])
+def test_elements_match_oracle_and_tnl(orc, fmt, param, name, org, symmetric):
+    # a row has getSegmentSize slots: its capacity in the CSR formats (checked
+    # against the oracle in CSR order), the padded width in the Ellpack
+    # formats (checked against TNL's own Host build, storage arrays bit-exact)
+    if org == tnl.ROW_MAJOR and fmt in CSR_FAMILY:
+        pytest.skip("CSR formats have one organisation")
+    ref = OL.Reference.load()
+    if ref is None and fmt not in CSR_FAMILY:
+        pytest.skip("the padded formats are checked against TNL's Host build (oracle/_ref), not built")
+    rng = np.random.default_rng(fmt * 4 + org * 2 + symmetric)
+    n = m = 70
+    mults = [0.0, 1.0, 0.5, -2.0]
+    for trial in range(2):
+        caps = rng.integers(0, 6, n)
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum(caps)
+        ops = [(int(rng.integers(n)), int(rng.integers(m)), float(rng.standard_normal()),
+                mults[rng.integers(len(mults))]) for _ in range(150)]
+        want_c, want_v, failed = orc.add_elements(off, ops, symmetric=symmetric)
+        if ref is not None:
+            rc, rv, rfailed, _ = ref.add_elements("CSR" if fmt == tnl.ADAPTIVE_CSR else name, off, m, ops,
+                                                  symmetric=symmetric, col_major=org == tnl.COLUMN_MAJOR)
+            if fmt in CSR_FAMILY:
+                assert rfailed == failed
+            failed = rfailed
+        A = tnl.SparseMatrix.from_csr(n, m, off, np.full(off[-1], -1, np.int32), np.zeros(off[-1]), fmt=fmt,
+                                      param=param, organization=org, symmetric=symmetric)
+        apply(A, ops, failed)
+        if ref is not None:
+            assert np.array_equal(A.column_indexes.cpu().numpy()[: len(rc)], rc), (name, trial)
+            assert np.array_equal(A.values.cpu().numpy()[: len(rv)], rv), (name, trial)
+        if fmt in CSR_FAMILY:
+            g = A.segments.global_indexes().long()
+            got_c = A.column_indexes[g].cpu().numpy()
+            got_v = A.values[g].cpu().numpy()
+            assert np.array_equal(got_c, want_c) and np.array_equal(got_v, want_v), (name, trial, failed)
+            # getElement: stored elements, and absent positions read 0
+            for r in range(0, n, 7):
+                for c in range(0, m, 5):
+                    rr, cc = (max(r, c), min(r, c)) if symmetric else (r, c)
+                    seg = slice(off[rr], off[rr + 1])
+                    hit = np.nonzero(want_c[seg] == cc)[0]
+                    expect = float(want_v[seg][hit[0]]) if len(hit) else 0.0
+                    assert A.getElement(r, c) == expect, (name, r, c)
+
+
+def test_elements_ellpack_rows_have_the_full_width():
+    # Ellpack: every row has width = the largest capacity (EllpackBase.hpp), so a
+    # row of capacity 1 takes three elements; SlicedEllpack: its slice's width
+    for fmt, param in ((tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32)):
+        off = np.array([0, 1, 4], np.int32)
+        A = tnl.SparseMatrix.from_csr(2, 6, off, np.full(4, -1, np.int32), np.zeros(4), fmt=fmt, param=param)
+        A.setElement(0, 4, 1.0)  # [4, -, -]
+        A.setElement(0, 0, 2.0)  # before 4: [0, 4, -]
+        A.setElement(0, 2, 3.0)  # before 4: [0, 2, 4]
+        assert [A.getElement(0, c) for c in range(6)] == [2.0, 0.0, 3.0, 0.0, 1.0, 0.0]
+        with pytest.raises(tnl.LogicError, match="The capacity of the sparse matrix row number 0 was exceeded."):
+            A.setElement(0, 5, 9.0)  # all three slots hold smaller columns
+
+
+def test_elements_fp32_and_errors(orc):
+    off = np.array([0, 2, 4], np.int32)
+    A = tnl.SparseMatrix.from_csr(2, 6, off, np.full(4, -1, np.int32), np.zeros(4), value_dtype=torch.float32)
+    A.setElement(0, 3, 1.5)
+    A.addElement(0, 3, 0.25, 0.5)  # 0.5 * 1.5 + 0.25 = 1.0
+    A.setElement(1, 5, -2.0)
+    A.addElement(1, 0, 3.0)
+    assert A.getElement(0, 3) == 1.0 and A.getElement(1, 0) == 3.0 and A.getElement(1, 5) == -2.0
+    assert A.getElement(1, 2) == 0.0
+    with pytest.raises(tnl.TnlmiError):
+        A.setElement(2, 0, 1.0)
+    with pytest.raises(tnl.TnlmiError):
+        A.getElement(0, 6)
+    A.setElement(0, 1, 7.0)  # row 0 full: [1, 3]
+    with pytest.raises(tnl.LogicError, match="row number 0 was exceeded"):
+        A.setElement(0, 4, 1.0)
+    y = torch.zeros(2, dtype=torch.float32, device="cuda")
+    A.vectorProduct(torch.ones(6, dtype=torch.float32, device="cuda"), y)
+    assert y.cpu().tolist() == [8.0, 1.0]

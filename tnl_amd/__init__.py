@@ -310,6 +310,24 @@ class SparseMatrix:
                                                     float(alpha), float(beta), int(begin), int(end), _stream(stream)))
         return y
 
+    def setElement(self, row, column, value, stream=None):
+        """SparseMatrixBase::setElement (SparseMatrixBase.hpp:152-161): addElement with multiplicator 0."""
+        self.addElement(row, column, value, 0.0, stream)
+
+    def addElement(self, row, column, value, thisElementMultiplicator=1.0, stream=None):
+        """SparseMatrixBase::addElement (SparseMatrixBase.hpp:166-231): a stored element becomes
+        thisElementMultiplicator * old + value, a new one takes the row's first padding slot or the
+        slot of the first larger column (later slots shift right); LogicError when the row is full."""
+        _lib.check(_lib.lib().tnlmi_matrix_add_element(ctypes.byref(self._m), int(row), int(column), float(value),
+                                                       float(thisElementMultiplicator), _stream(stream)))
+
+    def getElement(self, row, column, stream=None):
+        """SparseMatrixBase::getElement (SparseMatrixBase.hpp:231-265): 0.0 if not stored."""
+        out = ctypes.c_double(0.0)
+        _lib.check(_lib.lib().tnlmi_matrix_get_element(ctypes.byref(self._m), int(row), int(column),
+                                                       ctypes.byref(out), _stream(stream)))
+        return out.value
+
     def getRows(self):
         return self.rows
 

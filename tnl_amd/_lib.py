@@ -175,6 +175,10 @@ _PROTOS = {
                                               ctypes.c_double, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "tnlmi_matrix_row_lengths": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp]),
     "tnlmi_matrix_copy": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp]),
+    "tnlmi_matrix_add_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_double, ctypes.c_double, c_vp]),
+    "tnlmi_matrix_get_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_double), c_vp]),
     "tnlmi_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
     "tnlmi_mtx_parse": (ctypes.c_int, [ctypes.c_char_p, c_i64, ctypes.POINTER(c_vp)]),
     "tnlmi_host_csr_get": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),

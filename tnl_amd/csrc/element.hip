@@ -1,0 +1,201 @@
+// element.hip -- SparseMatrixBase::addElement / setElement / getElement on the
+// device (Matrices/SparseMatrixBase.hpp:152-230 and :231-265).  One thread walks
+// the row's getSegmentSize slots (an Ellpack row has the full width, however
+// small its requested capacity) through the format's getGlobalIndex, as the
+// reference's loop does with element-wise device reads and writes: a stored
+// column is updated in place (value = thisElementMultiplicator * value + new value), else the new
+// element goes into the first padding slot or, before the first larger column,
+// with the slots after it shifted one place right (the last slot's content is
+// dropped if the row is full, as in the reference).  A row with neither raises
+// the reference's std::logic_error text.  The update rounds the product and the
+// sum separately (no fused multiply-add), as the reference's host build does.
+#include "segments.hpp"
+
+namespace tnlmi {
+namespace {
+
+__device__ __forceinline__ double update( double old, double mult, double value )
+{
+   return __dadd_rn( __dmul_rn( mult, old ), value );
+}
+__device__ __forceinline__ float update( float old, double mult, double value )
+{
+   return __fadd_rn( __fmul_rn( (float) mult, old ), (float) value );
+}
+__device__ __forceinline__ bf16 update( bf16 old, double mult, double value )
+{
+   bf16 r;
+   r.bits = float_to_bf16( __fadd_rn( __fmul_rn( (float) mult, bf16_to_float( old.bits ) ), (float) value ) );
+   return r;
+}
+template< typename V >
+__device__ __forceinline__ V from_double( double v )
+{
+   return (V) v;
+}
+template<>
+__device__ __forceinline__ bf16 from_double< bf16 >( double v )
+{
+   bf16 r;
+   r.bits = float_to_bf16( (float) v );
+   return r;
+}
+__device__ __forceinline__ double to_double( double v ) { return v; }
+__device__ __forceinline__ double to_double( float v ) { return v; }
+__device__ __forceinline__ double to_double( bf16 v ) { return bf16_to_float( v.bits ); }
+
+// status: 0 done, 1 the row's capacity is exceeded
+template< typename V >
+__global__ void add_element( SegView v, int32_t* __restrict__ cols, V* __restrict__ vals, int64_t row, int32_t column,
+                             double value, double mult, int32_t* __restrict__ status )
+{
+   const int32_t size = segment_size( v, row );  // SparseMatrixBase.hpp:184, 252
+   int32_t col = -1, i = 0;
+   int64_t g = 0;
+   for( ; i < size; i++ ) {
+      g = global_index( v, row, i );
+      col = cols[ g ];
+      if( col == column ) {
+         vals[ g ] = update( vals[ g ], mult, value );
+         *status = 0;
+         return;
+      }
+      if( col == -1 || col > column )
+         break;
+   }
+   if( i == size ) {
+      *status = 1;
+      return;
+   }
+   if( col != -1 )
+      for( int32_t j = size - 1; j > i; j-- ) {
+         const int64_t g1 = global_index( v, row, j ), g2 = global_index( v, row, j - 1 );
+         cols[ g1 ] = cols[ g2 ];
+         vals[ g1 ] = vals[ g2 ];
+      }
+   cols[ g ] = column;
+   vals[ g ] = from_double< V >( value );
+   *status = 0;
+}
+
+template< typename V >
+__global__ void get_element( SegView v, const int32_t* __restrict__ cols, const V* __restrict__ vals, int64_t row,
+                             int32_t column, double* __restrict__ out )
+{
+   const int32_t size = segment_size( v, row );  // SparseMatrixBase.hpp:184, 252
+   for( int32_t i = 0; i < size; i++ ) {
+      const int64_t g = global_index( v, row, i );
+      if( cols[ g ] == column ) {
+         *out = to_double( vals[ g ] );
+         return;
+      }
+   }
+   *out = 0.0;
+}
+
+template< typename T >
+T fetch_one( const T* d, hipStream_t s )
+{
+   T h;
+   TNLMI_HIP( hipMemcpyAsync( &h, d, sizeof( T ), hipMemcpyDeviceToHost, s ) );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+   return h;
+}
+
+// checks the arguments and swaps (row, column) into the stored lower triangle
+// of a symmetric matrix; false if the swapped position lies outside the matrix
+bool check_element( const tnlmi_matrix* A, int64_t& row, int64_t& column, const char* who )
+{
+   if( ! A || ! A->segments )
+      fail( TNLMI_INVALID_ARGUMENT, std::string( who ) + ": matrix or its segments is NULL" );
+   if( A->rows != A->segments->n )
+      fail( TNLMI_INVALID_ARGUMENT, std::string( who ) + ": matrix rows do not match the segments count" );
+   if( row < 0 || row >= A->rows || column < 0 || column >= A->columns )
+      fail( TNLMI_INVALID_ARGUMENT, std::string( who ) + ": element (" + std::to_string( row ) + ", "
+                                        + std::to_string( column ) + ") is out of the matrix" );
+   if( A->segments->storage > 0 && ( ! A->values || ! A->column_indexes ) )
+      fail( TNLMI_INVALID_ARGUMENT, std::string( who ) + ": matrix storage pointer is NULL" );
+   if( A->matrix_type == TNLMI_SYMMETRIC && row < column ) {  // SparseMatrixBase.hpp:178-182, 246-250
+      std::swap( row, column );
+      return row < A->rows && column < A->columns;
+   }
+   return true;
+}
+
+}  // namespace
+}  // namespace tnlmi
+
+using namespace tnlmi;
+
+extern "C" {
+
+tnlmi_status tnlmi_matrix_add_element( tnlmi_matrix* A, int64_t row, int64_t column, double value,
+                                       double this_element_multiplicator, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( ! check_element( A, row, column, "addElement" ) )
+         fail( TNLMI_INVALID_ARGUMENT, "addElement: the transposed position is out of the symmetric matrix" );
+      hipStream_t s = as_stream( stream );
+      DeviceArray< int32_t > status;
+      status.allocate( 1 );
+      const SegView v = make_view( *A->segments );
+      int32_t* cols = const_cast< int32_t* >( A->column_indexes );
+      switch( A->value_type ) {
+         case TNLMI_F64:
+            hipLaunchKernelGGL( add_element< double >, dim3( 1 ), dim3( 1 ), 0, s, v, cols, (double*) A->values, row,
+                                (int32_t) column, value, this_element_multiplicator, status.ptr );
+            break;
+         case TNLMI_F32:
+            hipLaunchKernelGGL( add_element< float >, dim3( 1 ), dim3( 1 ), 0, s, v, cols, (float*) A->values, row,
+                                (int32_t) column, value, this_element_multiplicator, status.ptr );
+            break;
+         case TNLMI_BF16:
+            hipLaunchKernelGGL( add_element< bf16 >, dim3( 1 ), dim3( 1 ), 0, s, v, cols, (bf16*) A->values, row,
+                                (int32_t) column, value, this_element_multiplicator, status.ptr );
+            break;
+         default:
+            fail( TNLMI_INVALID_ARGUMENT, "addElement: unknown value type" );
+      }
+      TNLMI_HIP( hipGetLastError() );
+      if( fetch_one( status.ptr, s ) != 0 )  // SparseMatrixBase.hpp:198-206
+         fail( TNLMI_LOGIC_ERROR,
+               "The capacity of the sparse matrix row number " + std::to_string( row ) + " was exceeded." );
+   } );
+}
+
+tnlmi_status tnlmi_matrix_get_element( const tnlmi_matrix* A, int64_t row, int64_t column, double* value,
+                                       tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( ! value )
+         fail( TNLMI_INVALID_ARGUMENT, "getElement: value pointer is NULL" );
+      if( ! check_element( A, row, column, "getElement" ) ) {
+         *value = 0.0;
+         return;
+      }
+      hipStream_t s = as_stream( stream );
+      DeviceArray< double > out;
+      out.allocate( 1 );
+      const SegView v = make_view( *A->segments );
+      switch( A->value_type ) {
+         case TNLMI_F64:
+            hipLaunchKernelGGL( get_element< double >, dim3( 1 ), dim3( 1 ), 0, s, v, A->column_indexes,
+                                (const double*) A->values, row, (int32_t) column, out.ptr );
+            break;
+         case TNLMI_F32:
+            hipLaunchKernelGGL( get_element< float >, dim3( 1 ), dim3( 1 ), 0, s, v, A->column_indexes,
+                                (const float*) A->values, row, (int32_t) column, out.ptr );
+            break;
+         case TNLMI_BF16:
+            hipLaunchKernelGGL( get_element< bf16 >, dim3( 1 ), dim3( 1 ), 0, s, v, A->column_indexes,
+                                (const bf16*) A->values, row, (int32_t) column, out.ptr );
+            break;
+         default:
+            fail( TNLMI_INVALID_ARGUMENT, "getElement: unknown value type" );
+      }
+      TNLMI_HIP( hipGetLastError() );
+      *value = fetch_one( out.ptr, s );
+   } );
+}
+
+}  // extern "C"

@@ -166,6 +166,43 @@ __host__ __device__ inline int64_t global_index( const SegView& v, int64_t r, in
    return -1;
 }
 
+// getSegmentSize( r ) of every format: the slots a traversal of row r visits
+// (Ellpack: the width, EllpackBase.hpp; SlicedEllpack: its slice's width,
+// SlicedEllpackBase.hpp; BiEllpack: the widths of the row's groups and
+// ChunkedEllpack: its chunks x the slice's chunk size, getSegmentSizeDirect in
+// detail/BiEllpack.h and detail/ChunkedEllpack.h; the CSR formats: the size)
+__host__ __device__ inline int32_t segment_size( const SegView& v, int64_t r )
+{
+   switch( v.format ) {
+      case TNLMI_ELLPACK:
+         return v.ell_width;
+      case TNLMI_SLICED_ELLPACK:
+         return v.slice_seg_sizes[ r / v.slice_size ];
+      case TNLMI_BIELLPACK: {
+         const int64_t strip = r / v.warp;
+         const int64_t g0 = strip * ( v.logw + 1 );
+         const int32_t rsp = v.perm[ r ] - (int32_t) ( strip * v.warp );
+         int groups = 1;
+         for( int i = 0, bis = 1; i < v.logw + 1; i++, bis *= 2 )
+            if( rsp < bis ) {
+               groups = v.logw + 1 - i;
+               break;
+            }
+         int32_t size = 0;
+         for( int g = 0, height = v.warp; g < groups; g++, height /= 2 )
+            size += ( v.group_ptrs[ g0 + g + 1 ] - v.group_ptrs[ g0 + g ] ) / height;
+         return size;
+      }
+      case TNLMI_CHUNKED_ELLPACK: {
+         const int32_t si = v.seg_to_slice[ r ];
+         const int32_t first = r != v.ch_slices[ 4 * si + 2 ] ? v.seg_to_chunk[ r - 1 ] : 0;
+         return ( v.seg_to_chunk[ r ] - first ) * v.ch_slices[ 4 * si + 1 ];
+      }
+      default:
+         return v.sizes[ r ];
+   }
+}
+
 // The index arrays and counters of a segments object as TNL's segments save
 // and load them (host copies; the fields of other formats stay empty):
 // CSR.hpp:116-130, Ellpack.hpp:105-120, SlicedEllpack.hpp:166-190,

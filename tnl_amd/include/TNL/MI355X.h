@@ -422,6 +422,13 @@ public:
          m[ { r, c } ] = v;
       setElements( m );
    }
+   //! SparseMatrix( rowCapacities, columns ) (SparseMatrix.h): empty rows of the
+   //! given capacities (setRowCapacities)
+   SparseMatrix( const std::initializer_list< int >& rowCapacities, int columns )
+   {
+      setDimensions( (int) rowCapacities.size(), columns );
+      setRowCapacities( std::vector< int >( rowCapacities ) );
+   }
    SparseMatrix( const SparseMatrix& ) = delete;
    SparseMatrix& operator=( const SparseMatrix& ) = delete;
    ~SparseMatrix() { release(); }
@@ -451,9 +458,7 @@ public:
       detail::hip( hipMalloc( (void**) &columnIndexes_, storage * sizeof( int ) ) );
       tnlmi_matrix dst = abi();
       detail::check( tnlmi_matrix_copy( &src, &dst, d_len, nullptr ) );
-      hoff_ = B.hoff_;  // the host mirror in CSR order is the same
-      hcols_ = B.hcols_;
-      hvals_ = B.hvals_;
+      mirror();
    }
    template< typename, typename, typename, typename, template< typename, typename, typename > class, typename >
    friend class SparseMatrix;
@@ -555,15 +560,25 @@ public:
       vectorProduct( in, out, 1.0, 0.0, 0, 0, cfg );
    }
 
-   //! getElement( row, column ) (SparseMatrixBase.hpp:231-265), 0 if absent
+   //! getElement( row, column ) (SparseMatrixBase.hpp:231-265), 0 if absent;
+   //! a symmetric matrix reads (column, row) for row < column
    Real getElement( int row, int column ) const
    {
-      if( MatrixType::isSymmetric() && row < column )  // SparseMatrixBase.hpp:246-249
-         std::swap( row, column );
-      for( int g = hoff_[ row ]; g < hoff_[ row + 1 ]; g++ )
-         if( hcols_[ g ] == column )
-            return hvals_[ g ];
-      return Real( 0 );
+      const tnlmi_matrix m = abi();
+      double v = 0;
+      detail::check( tnlmi_matrix_get_element( &m, row, column, &v, nullptr ) );
+      return (Real) v;
+   }
+   //! setElement( row, column, value ) (SparseMatrixBase.hpp:152-161)
+   void setElement( int row, int column, const Real& value ) { addElement( row, column, value, Real( 0 ) ); }
+   //! addElement( row, column, value, thisElementMultiplicator ) (SparseMatrixBase.hpp:166-231):
+   //! an existing element becomes multiplicator * old + value; a new one is
+   //! inserted in column order into the row's capacity (std::logic_error
+   //! "The capacity of the sparse matrix row number r was exceeded." if full)
+   void addElement( int row, int column, const Real& value, const Real& thisElementMultiplicator = 1.0 )
+   {
+      tnlmi_matrix m = abi();
+      detail::check( tnlmi_matrix_add_element( &m, row, column, value, thisElementMultiplicator, nullptr ) );
    }
    int getRowCapacity( int row ) const { return hoff_[ row + 1 ] - hoff_[ row ]; }
    int64_t getStorageSize() const { return segments_ ? tnlmi_segments_storage_size( segments_ ) : 0; }
@@ -642,8 +657,7 @@ public:
    }
 
 private:
-   // the host mirror in CSR order (row capacities, then each row's slots) of
-   // the device storage, through the segments' global indexes
+   // the row capacities on the host (getRowCapacity)
    void mirror()
    {
       const int32_t* d_sizes = nullptr;
@@ -655,26 +669,6 @@ private:
       hoff_.assign( rows_ + 1, 0 );
       for( int r = 0; r < rows_; r++ )
          hoff_[ r + 1 ] = hoff_[ r ] + sizes[ r ];
-      const int64_t elements = hoff_[ rows_ ], storage = tnlmi_segments_storage_size( segments_ );
-      std::vector< int > gidx( elements ), cols( storage );
-      std::vector< Real > vals( storage );
-      int* d_gidx = nullptr;
-      detail::hip( hipMalloc( (void**) &d_gidx, std::max< int64_t >( 1, elements ) * sizeof( int ) ) );
-      std::unique_ptr< int, void ( * )( int* ) > own( d_gidx, []( int* p ) { (void) hipFree( p ); } );
-      detail::check( tnlmi_segments_global_indexes( segments_, d_gidx, nullptr ) );
-      detail::hip( hipDeviceSynchronize() );
-      if( elements > 0 )
-         detail::hip( hipMemcpy( gidx.data(), d_gidx, elements * sizeof( int ), hipMemcpyDeviceToHost ) );
-      if( storage > 0 ) {
-         detail::hip( hipMemcpy( cols.data(), columnIndexes_, storage * sizeof( int ), hipMemcpyDeviceToHost ) );
-         detail::hip( hipMemcpy( vals.data(), values_, storage * sizeof( Real ), hipMemcpyDeviceToHost ) );
-      }
-      hcols_.resize( elements );
-      hvals_.resize( elements );
-      for( int64_t e = 0; e < elements; e++ ) {
-         hcols_[ e ] = cols[ gidx[ e ] ];
-         hvals_[ e ] = vals[ gidx[ e ] ];
-      }
    }
 
    void release()
@@ -692,9 +686,9 @@ private:
                   const std::vector< Real >& vals )
    {
       release();
-      hoff_ = off;
-      hcols_ = cols;
-      hvals_ = vals;
+      hoff_.assign( rows_ + 1, 0 );
+      for( int r = 0; r < rows_; r++ )
+         hoff_[ r + 1 ] = hoff_[ r ] + caps[ r ];
       int *d_caps = nullptr, *d_off = nullptr, *d_cols = nullptr;
       Real* d_vals = nullptr;
       auto up = [ & ]( auto*& d, const auto& h ) {
@@ -726,8 +720,7 @@ private:
    tnlmi_segments* segments_ = nullptr;
    Real* values_ = nullptr;
    int* columnIndexes_ = nullptr;
-   std::vector< int > hoff_, hcols_;
-   std::vector< Real > hvals_;
+   std::vector< int > hoff_;  // prefix sums of the row capacities
 };
 
 //! copySparseMatrix (Matrices/SparseOperations.hpp:716-721) between MI355X
