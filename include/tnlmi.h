@@ -185,6 +185,15 @@ tnlmi_status tnlmi_matrix_copy( const tnlmi_matrix* src, tnlmi_matrix* dst, cons
  * a position outside the matrix.  Synchronous. */
 tnlmi_status tnlmi_matrix_add_element( tnlmi_matrix* A, int64_t row, int64_t column, double value,
                                        double this_element_multiplicator, tnlmi_stream stream );
+/* Per-row counts over each row's getSegmentSize slots, into d_counts (rows
+ * int32), synchronous (SparseMatrixBase.hpp:55-132; padding slots count as 0):
+ *   0  getRowCapacity( r ): the segment size (Ellpack: the width)
+ *   1  getRowCapacities: the stored (non-padding) slots
+ *   2  getCompressedRowLengths: the stored slots with a nonzero value
+ *   3  getNonzeroElementsCount of a symmetric matrix, per row: a stored slot
+ *      counts 1, plus 1 if it lies off the diagonal inside the matrix
+ * (getNonzeroElementsCount of a general matrix is the sum of kind 2.) */
+tnlmi_status tnlmi_matrix_row_counts( const tnlmi_matrix* A, int32_t kind, int32_t* d_counts, tnlmi_stream stream );
 /* SparseMatrixBase::getElement( row, column ) (SparseMatrixBase.hpp:231-265):
  * *value = the stored element (FP32 / BF16 widened), 0 if the row holds no
  * such column.  Synchronous. */

@@ -267,6 +267,43 @@ struct SymElementsOp
    }
 };
 
+// getRowCapacity( r ) per row, getRowCapacities, getCompressedRowLengths and
+// getNonzeroElementsCount of the matrix build() makes (SparseMatrixBase.hpp:55-132)
+template< class M >
+int64_t counts_impl( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v, int32_t* cap,
+                     int32_t* stored, int32_t* nonzero )
+{
+   M m;
+   build( m, n, ncols, rp, c, v );
+   IVec a, b;
+   m.getRowCapacities( a );
+   m.getCompressedRowLengths( b );
+   for( int64_t r = 0; r < n; r++ ) {
+      cap[ r ] = m.getRowCapacity( r );
+      stored[ r ] = a[ r ];
+      nonzero[ r ] = b[ r ];
+   }
+   return m.getNonzeroElementsCount();
+}
+template< template< class, class, class > class S >
+struct CountsOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v, int32_t* a,
+                       int32_t* b, int32_t* d )
+   {
+      return counts_impl< Mat< S > >( n, ncols, rp, c, v, a, b, d );
+   }
+};
+template< template< class, class, class > class S >
+struct SymCountsOp
+{
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v, int32_t* a,
+                       int32_t* b, int32_t* d )
+   {
+      return counts_impl< SymMat< S > >( n, ncols, rp, c, v, a, b, d );
+   }
+};
+
 template< template< class, class, class > class S >
 struct SymSaveOp
 {
@@ -796,6 +833,14 @@ int64_t ref_add_elements( int format, int col_major, int symmetric, int64_t n, i
                                         out_cols, out_vals, cap );
    return dispatch< ElementsOp >( format, col_major, n, ncols, row_ptr, cols, vals, op_pos, op_val, nops, failed,
                                   out_cols, out_vals, cap );
+}
+
+int64_t ref_row_counts( int format, int col_major, int symmetric, int64_t n, int64_t ncols, const int32_t* row_ptr,
+                        const int32_t* cols, const double* vals, int32_t* capacity, int32_t* stored, int32_t* nonzero )
+{
+   if( symmetric )
+      return dispatch< SymCountsOp >( format, col_major, n, ncols, row_ptr, cols, vals, capacity, stored, nonzero );
+   return dispatch< CountsOp >( format, col_major, n, ncols, row_ptr, cols, vals, capacity, stored, nonzero );
 }
 
 // OpenMP threads of TNL's Host algorithms in the following calls

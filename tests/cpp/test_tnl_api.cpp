@@ -193,6 +193,18 @@ int main()
       EXPECT_EQ( s2.getElement( 0 ), 3.0 );
       EXPECT_EQ( s2.getElement( 1 ), 2.0 );
       EXPECT_EQ( s2.getElement( 2 ), 3.0 );
+      EXPECT_EQ( M.getNonzeroElementsCount(), (int64_t) 25 );
+      EXPECT_EQ( S.getNonzeroElementsCount(), (int64_t) 7 );  // 3 diagonal + 2 x 2 off-diagonal
+      Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Seg::Ellpack > E( { 1, 3 }, 4 );
+      EXPECT_EQ( E.getRowCapacity( 0 ), 3 );  // an Ellpack row has the full width
+      E.setElement( 0, 1, 0.0 );
+      E.setElement( 1, 2, 5.0 );
+      Matrices::SparseMatrix< double, Device >::RowCapacitiesType caps, lens;
+      E.getRowCapacities( caps );
+      E.getCompressedRowLengths( lens );
+      EXPECT_EQ( caps.getElement( 0 ), 1 );
+      EXPECT_EQ( lens.getElement( 0 ), 0 );  // a stored zero
+      EXPECT_EQ( lens.getElement( 1 ), 1 );
       std::cout << "setElement / addElement / getElement: ok\n";
    }
 

@@ -282,7 +282,7 @@ class Reference:
         r.L = ctypes.CDLL(REF_SO)
         for name in ["ref_chunked_ellpack", "ref_adaptive_csr_blocks", "ref_fill", "ref_spmv", "ref_save",
                      "ref_load", "ref_spmv_symmetric", "ref_spmv_transposed", "ref_save_symmetric",
-                     "ref_add_elements"]:
+                     "ref_add_elements", "ref_row_counts"]:
             getattr(r.L, name).restype = ctypes.c_int64
         r.L.ref_last_error.restype = ctypes.c_char_p
         r.L.ref_spmv_csr_time.restype = ctypes.c_double
@@ -375,6 +375,16 @@ class Reference:
         self.L.ref_add_elements(*args, P(oc), P(ov), I64(cap))
         msg = self.L.ref_last_error().decode() if failed[0] >= 0 else ""
         return oc, ov, int(failed[0]), msg
+
+    def row_counts(self, fmt, off, cols, vals, ncols, symmetric=False, col_major=True):
+        """getRowCapacity per row, getRowCapacities, getCompressedRowLengths and
+        getNonzeroElementsCount of the matrix TNL builds from these CSR arrays (as fill)."""
+        off, cols, vals = i32(off), i32(cols), f64(vals)
+        n = len(off) - 1
+        cap, stored, nz = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        total = self.L.ref_row_counts(self.FORMATS[fmt], int(col_major), int(symmetric), I64(n), I64(ncols), P(off),
+                                      P(cols), P(vals), P(cap), P(stored), P(nz))
+        return cap, stored, nz, int(total)
 
     def save_file(self, fmt, off, cols, vals, ncols, path, col_major=True):
         """SparseMatrix::save of the matrix TNL builds from these CSR arrays (as fill)."""

@@ -121,3 +121,40 @@ def test_elements_fp32_and_errors(orc):
     y = torch.zeros(2, dtype=torch.float32, device="cuda")
     A.vectorProduct(torch.ones(6, dtype=torch.float32, device="cuda"), y)
     assert y.cpu().tolist() == [8.0, 1.0]
+
+
+@pytest.mark.parametrize("fmt,param,name", FORMATS, ids=[f[2] for f in FORMATS])
+@pytest.mark.parametrize("symmetric", [False, True], ids=["general", "symmetric"])
+def test_row_counts_match_tnl(fmt, param, name, symmetric):
+    # getRowCapacity (the segment size), getRowCapacities (stored slots),
+    # getCompressedRowLengths (stored nonzeros), getNonzeroElementsCount; stored
+    # zeros and padding slots at row ends
+    rng = np.random.default_rng(40 + fmt * 2 + symmetric)
+    n = 300
+    sizes = rng.integers(0, 9, n).astype(np.int32)
+    sizes[[3, 200]] = [40, 70]
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum(sizes)
+    cols = np.full(off[-1], -1, np.int32)
+    vals = np.zeros(off[-1])
+    for r in range(n):
+        hi = r + 1 if symmetric else n
+        used = min(int(sizes[r]) - int(rng.random() < 0.3 and sizes[r] > 0), hi)
+        c = np.sort(rng.choice(hi, size=used, replace=False))
+        cols[off[r]:off[r] + used] = c
+        vals[off[r]:off[r] + used] = np.where(rng.random(used) < 0.2, 0.0, rng.standard_normal(used))
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param, symmetric=symmetric)
+    row_of = np.repeat(np.arange(n), np.diff(off))
+    stored = cols >= 0
+    want_stored = np.bincount(row_of[stored], minlength=n)
+    want_nz = np.bincount(row_of[stored & (vals != 0)], minlength=n)
+    assert np.array_equal(A.getRowCapacities().cpu().numpy(), want_stored)
+    assert np.array_equal(A.getCompressedRowLengths().cpu().numpy(), want_nz)
+    want_total = int(stored.sum() + (stored & (cols != row_of)).sum()) if symmetric else int(want_nz.sum())
+    assert A.getNonzeroElementsCount() == want_total
+    ref = OL.Reference.load()
+    if ref is not None:
+        cap, rstored, rnz, total = ref.row_counts("CSR" if fmt == tnl.ADAPTIVE_CSR else name, off, cols, vals, n,
+                                                  symmetric=symmetric)
+        assert np.array_equal(A._row_counts(0).cpu().numpy(), cap)
+        assert np.array_equal(rstored, want_stored) and np.array_equal(rnz, want_nz) and total == want_total

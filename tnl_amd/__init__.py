@@ -321,6 +321,30 @@ class SparseMatrix:
         _lib.check(_lib.lib().tnlmi_matrix_add_element(ctypes.byref(self._m), int(row), int(column), float(value),
                                                        float(thisElementMultiplicator), _stream(stream)))
 
+    def _row_counts(self, kind, stream=None):
+        torch = _torch()
+        out = torch.empty(max(1, self.rows), dtype=torch.int32, device="cuda")
+        _lib.check(_lib.lib().tnlmi_matrix_row_counts(ctypes.byref(self._m), int(kind),
+                                                      ctypes.c_void_p(out.data_ptr()), _stream(stream)))
+        return out[:self.rows]
+
+    def getRowCapacity(self, row):
+        """SparseMatrixBase::getRowCapacity: the row's segment size (Ellpack: the width)."""
+        return int(self._row_counts(0)[int(row)])
+
+    def getRowCapacities(self):
+        """SparseMatrixBase::getRowCapacities (SparseMatrixBase.hpp:75-92): stored slots per row."""
+        return self._row_counts(1)
+
+    def getCompressedRowLengths(self):
+        """SparseMatrixBase::getCompressedRowLengths (SparseMatrixBase.hpp:55-73): stored nonzeros per row."""
+        return self._row_counts(2)
+
+    def getNonzeroElementsCount(self):
+        """SparseMatrixBase::getNonzeroElementsCount (SparseMatrixBase.hpp:102-132); a symmetric
+        matrix counts its stored off-diagonal entries twice."""
+        return int(self._row_counts(3 if self.symmetric else 2).long().sum())
+
     def getElement(self, row, column, stream=None):
         """SparseMatrixBase::getElement (SparseMatrixBase.hpp:231-265): 0.0 if not stored."""
         out = ctypes.c_double(0.0)

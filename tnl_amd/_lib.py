@@ -177,6 +177,7 @@ _PROTOS = {
     "tnlmi_matrix_copy": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp]),
     "tnlmi_matrix_add_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.c_double, ctypes.c_double, c_vp]),
+    "tnlmi_matrix_row_counts": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int32, c_vp, c_vp]),
     "tnlmi_matrix_get_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.POINTER(ctypes.c_double), c_vp]),
     "tnlmi_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),

@@ -9,6 +9,8 @@
 // dropped if the row is full, as in the reference).  A row with neither raises
 // the reference's std::logic_error text.  The update rounds the product and the
 // sum separately (no fused multiply-add), as the reference's host build does.
+#include <algorithm>
+
 #include "segments.hpp"
 
 namespace tnlmi {
@@ -93,6 +95,41 @@ __global__ void get_element( SegView v, const int32_t* __restrict__ cols, const 
    *out = 0.0;
 }
 
+__device__ __forceinline__ bool nonzero( double v ) { return v != 0.0; }
+__device__ __forceinline__ bool nonzero( float v ) { return v != 0.0f; }
+__device__ __forceinline__ bool nonzero( bf16 v ) { return ( v.bits & 0x7fffu ) != 0; }
+
+// per row, over its getSegmentSize slots (SparseMatrixBase.hpp:55-132; padding
+// slots count as the reduction's identity, :604-615): kind 0 the segment size
+// (getRowCapacity), 1 the stored slots (getRowCapacities), 2 the stored
+// nonzero values (getCompressedRowLengths), 3 getNonzeroElementsCount's term
+// of a symmetric matrix (an off-diagonal slot inside the matrix counts twice)
+template< typename V >
+__global__ void row_counts( SegView v, const int32_t* __restrict__ cols, const V* __restrict__ vals, int64_t rows,
+                            int64_t columns, int32_t kind, int32_t* __restrict__ out )
+{
+   for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t) gridDim.x * blockDim.x ) {
+      const int32_t size = segment_size( v, r );
+      int32_t count = 0;
+      if( kind == 0 )
+         count = size;
+      else
+         for( int32_t k = 0; k < size; k++ ) {
+            const int64_t g = global_index( v, r, k );
+            const int32_t c = cols[ g ];
+            if( c == -1 )
+               continue;
+            if( kind == 1 )
+               count++;
+            else if( kind == 2 )
+               count += nonzero( vals[ g ] );
+            else
+               count += 1 + ( c != r && c < rows && r < columns );
+         }
+      out[ r ] = count;
+   }
+}
+
 template< typename T >
 T fetch_one( const T* d, hipStream_t s )
 {
@@ -160,6 +197,43 @@ tnlmi_status tnlmi_matrix_add_element( tnlmi_matrix* A, int64_t row, int64_t col
       if( fetch_one( status.ptr, s ) != 0 )  // SparseMatrixBase.hpp:198-206
          fail( TNLMI_LOGIC_ERROR,
                "The capacity of the sparse matrix row number " + std::to_string( row ) + " was exceeded." );
+   } );
+}
+
+tnlmi_status tnlmi_matrix_row_counts( const tnlmi_matrix* A, int32_t kind, int32_t* d_counts, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( ! A || ! A->segments || ( A->rows > 0 && ! d_counts ) )
+         fail( TNLMI_INVALID_ARGUMENT, "matrix_row_counts: null argument" );
+      if( A->rows != A->segments->n )
+         fail( TNLMI_INVALID_ARGUMENT, "matrix_row_counts: matrix rows do not match the segments count" );
+      if( kind < 0 || kind > 3 )
+         fail( TNLMI_INVALID_ARGUMENT, "matrix_row_counts: unknown kind" );
+      if( kind > 0 && A->segments->storage > 0 && ( ! A->values || ! A->column_indexes ) )
+         fail( TNLMI_INVALID_ARGUMENT, "matrix_row_counts: matrix storage pointer is NULL" );
+      hipStream_t s = as_stream( stream );
+      if( A->rows > 0 ) {
+         const SegView v = make_view( *A->segments );
+         const dim3 grid( (unsigned) std::min< int64_t >( 4096, ( A->rows + 255 ) / 256 ) );
+         switch( A->value_type ) {
+            case TNLMI_F64:
+               hipLaunchKernelGGL( row_counts< double >, grid, dim3( 256 ), 0, s, v, A->column_indexes,
+                                   (const double*) A->values, A->rows, A->columns, kind, d_counts );
+               break;
+            case TNLMI_F32:
+               hipLaunchKernelGGL( row_counts< float >, grid, dim3( 256 ), 0, s, v, A->column_indexes,
+                                   (const float*) A->values, A->rows, A->columns, kind, d_counts );
+               break;
+            case TNLMI_BF16:
+               hipLaunchKernelGGL( row_counts< bf16 >, grid, dim3( 256 ), 0, s, v, A->column_indexes,
+                                   (const bf16*) A->values, A->rows, A->columns, kind, d_counts );
+               break;
+            default:
+               fail( TNLMI_INVALID_ARGUMENT, "matrix_row_counts: unknown value type" );
+         }
+         TNLMI_HIP( hipGetLastError() );
+      }
+      TNLMI_HIP( hipStreamSynchronize( s ) );
    } );
 }
 

@@ -580,7 +580,23 @@ public:
       tnlmi_matrix m = abi();
       detail::check( tnlmi_matrix_add_element( &m, row, column, value, thisElementMultiplicator, nullptr ) );
    }
+   //! getRowCapacity( row ): the row's segment size (Ellpack: the width)
    int getRowCapacity( int row ) const { return hoff_[ row + 1 ] - hoff_[ row ]; }
+   //! getRowCapacities (SparseMatrixBase.hpp:75-92): stored (non-padding) slots per row
+   void getRowCapacities( RowCapacitiesType& out ) const { rowCounts( 1, out ); }
+   //! getCompressedRowLengths (SparseMatrixBase.hpp:55-73): stored nonzero values per row
+   void getCompressedRowLengths( RowCapacitiesType& out ) const { rowCounts( 2, out ); }
+   //! getNonzeroElementsCount (SparseMatrixBase.hpp:102-132): a symmetric matrix
+   //! counts its stored off-diagonal entries twice
+   int64_t getNonzeroElementsCount() const
+   {
+      RowCapacitiesType c;
+      rowCounts( MatrixType::isSymmetric() ? 3 : 2, c );
+      int64_t total = 0;
+      for( int v : c.toHost() )
+         total += v;
+      return total;
+   }
    int64_t getStorageSize() const { return segments_ ? tnlmi_segments_storage_size( segments_ ) : 0; }
 
    //! getSerializationType (SparseMatrixBase.hpp:46-53)
@@ -657,15 +673,18 @@ public:
    }
 
 private:
-   // the row capacities on the host (getRowCapacity)
+   void rowCounts( int kind, RowCapacitiesType& out ) const
+   {
+      out.setSize( rows_ );
+      const tnlmi_matrix m = abi();
+      detail::check( tnlmi_matrix_row_counts( &m, kind, out.getData(), nullptr ) );
+   }
+   // the rows' segment sizes on the host (getRowCapacity)
    void mirror()
    {
-      const int32_t* d_sizes = nullptr;
-      int64_t n = 0;
-      detail::check( tnlmi_segments_get_array( segments_, TNLMI_ARR_SIZES, &d_sizes, &n ) );
-      std::vector< int > sizes( rows_ );
-      if( rows_ > 0 )
-         detail::hip( hipMemcpy( sizes.data(), d_sizes, rows_ * sizeof( int ), hipMemcpyDeviceToHost ) );
+      RowCapacitiesType c;
+      rowCounts( 0, c );
+      const std::vector< int > sizes = c.toHost();
       hoff_.assign( rows_ + 1, 0 );
       for( int r = 0; r < rows_; r++ )
          hoff_[ r + 1 ] = hoff_[ r ] + sizes[ r ];
@@ -686,9 +705,6 @@ private:
                   const std::vector< Real >& vals )
    {
       release();
-      hoff_.assign( rows_ + 1, 0 );
-      for( int r = 0; r < rows_; r++ )
-         hoff_[ r + 1 ] = hoff_[ r ] + caps[ r ];
       int *d_caps = nullptr, *d_off = nullptr, *d_cols = nullptr;
       Real* d_vals = nullptr;
       auto up = [ & ]( auto*& d, const auto& h ) {
@@ -714,6 +730,7 @@ private:
       for( void* p : { (void*) d_caps, (void*) d_off, (void*) d_cols, (void*) d_vals } )
          (void) hipFree( p );
       detail::check( st );
+      mirror();
    }
 
    int rows_ = 0, columns_ = 0;
