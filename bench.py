@@ -486,7 +486,7 @@ def wl_batched_cg(args):
     fmt = tune["format_id"]
     A = tnl_amd.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=tune["param"])
     A32 = tnl_amd.SparseMatrix(A.rows, A.columns, A.segments, A.values.float(), A.column_indexes)
-    bc = B.BatchedCG(A, sys_rows, A_lo=A32, block_size=tune["block"])
+    bc = B.BatchedCG(A, sys_rows, A_lo=A32, block_size=tune["block"], lanes_per_row=tune["lanes_per_row"])
     bd = torch.from_numpy(b).cuda()
     tol = 1e-10
     out_res = {}
@@ -551,7 +551,7 @@ def wl_batched_cg(args):
                       "f64 (and f32 inner / f64 refined)",
                       "synthetic (seeded variable-coefficient diffusion batch, b ~ N(0,1))",
                       {"workload": f"{nsys} independent SPD systems, Jacobi-CG to 1e-10 each", "rows": n, "nnz": nnz,
-                       "autotune": {k: tune[k] for k in ("format", "param", "block")},
+                       "autotune": {k: tune[k] for k in ("format", "param", "block", "rows_per_wavefront")},
                        "batched": out_res, "per_system_loop_seconds": round(t_loop, 3),
                        "loop_sample": f"{sample} systems timed, scaled by {nsys}/{sample}"})
     out["roofline"] = roofline(out_res["fp64"]["GBs"])

@@ -489,9 +489,12 @@ tnlmi_status tnlmi_bf16_gemm64( int64_t rows, const uint16_t* d_X, const uint16_
 /* ------------------------------------------------------------ batched CG */
 /* Many independent SPD systems, the diagonal blocks of one block-diagonal
  * matrix A (FP64 values, any segment format; system k owns rows
- * sys_rows[k] .. sys_rows[k+1], a host array of nsys+1 entries), solved by
- * one persistent kernel: each workgroup of block_size threads (128/256/512/
- * 1024; 0 = 512) takes the next system from an atomic counter and runs
+ * sys_rows[k] .. sys_rows[k+1], a host array of nsys+1 entries; every
+ * stored column of a system's rows lies in its own row range, checked at
+ * create: TNLMI_INVALID_ARGUMENT otherwise), solved by one persistent kernel:
+ * each workgroup of block_size threads (128/256/512/1024; 0 = 512), with
+ * lanes_per_row lanes summing each row (1/2/4/8; 0 = 1: 64, 32, 16 or 8 rows
+ * per wavefront), takes the next system from an atomic counter and runs
  * Jacobi-preconditioned CG (CG.hpp:10-108 per system, TNL's stopping rule on
  * the preconditioned residue, default min/divergence settings) to
  * `tolerance`, at most max_iterations.  A_lo (optional): an FP32 copy of A on
@@ -504,7 +507,7 @@ tnlmi_status tnlmi_bf16_gemm64( int64_t rows, const uint16_t* d_X, const uint16_
  * No reference counterpart (TNL solves one system per solver call). */
 typedef struct tnlmi_batched_cg tnlmi_batched_cg;
 tnlmi_status tnlmi_batched_cg_create( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const int64_t* sys_rows,
-                                      int64_t nsys, int32_t block_size, tnlmi_stream stream,
+                                      int64_t nsys, int32_t block_size, int32_t lanes_per_row, tnlmi_stream stream,
                                       tnlmi_batched_cg** out );
 void tnlmi_batched_cg_destroy( tnlmi_batched_cg* batch );
 tnlmi_status tnlmi_batched_cg_solve( tnlmi_batched_cg* batch, int32_t mixed, const double* d_b, double* d_x,

@@ -10,7 +10,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "tnl_amd", "csrc")
-sys.path.insert(0, CSRC)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import barrier_loops  # noqa: E402
 
 HIPCC = "/opt/rocm/bin/hipcc"

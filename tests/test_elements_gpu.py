@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 tnl = pytest.importorskip("tnl_amd")
+from tnl_amd import solvers  # noqa: E402
 
 FORMATS = [(tnl.CSR, 0, "CSR"), (tnl.ADAPTIVE_CSR, 0, "AdaptiveCSR"), (tnl.ELLPACK, 0, "Ellpack"),
            (tnl.SLICED_ELLPACK, 32, "SlicedEllpack32"), (tnl.SLICED_ELLPACK, 64, "SlicedEllpack64"),
@@ -158,3 +159,56 @@ def test_row_counts_match_tnl(fmt, param, name, symmetric):
                                                   symmetric=symmetric)
         assert np.array_equal(A._row_counts(0).cpu().numpy(), cap)
         assert np.array_equal(rstored, want_stored) and np.array_equal(rnz, want_nz) and total == want_total
+
+
+@pytest.mark.parametrize("fmt,param", [(tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32)], ids=["Ellpack", "SlicedEllpack32"])
+@pytest.mark.parametrize("org", [tnl.COLUMN_MAJOR, tnl.ROW_MAJOR], ids=["cm", "rm"])
+def test_slots_past_the_capacity_are_walked(fmt, param, org):
+    # rows 0 and 2 ask for one slot but have the padded width 3
+    # (getSegmentSize); the elements set past the capacity take part in every
+    # per-row walk of the reference -- the symmetric and transposed products,
+    # copySparseMatrix and the Diagonal preconditioner's diagonal (SparseMatrixBase.hpp:
+    # 298-329, 541-588; SparseOperations.hpp:544-597 via row.getSize();
+    # Diagonal.hpp:27-30) -- checked against the dense closed forms and TNL's
+    # own storage after the same operations
+    off = np.array([0, 1, 4, 5], np.int32)
+    n = 3
+    ops_general = [(0, 1, 3.0, 0.0), (0, 2, 4.0, 0.0), (0, 0, 2.0, 0.0), (1, 1, 5.0, 0.0), (2, 0, 1.0, 0.0),
+                   (2, 2, 6.0, 0.0)]  # row 2's diagonal lands in slot 1, past its capacity
+    ops_lower = [(0, 0, 2.0, 0.0), (1, 0, 3.0, 0.0), (1, 1, 5.0, 0.0), (2, 0, 1.0, 0.0), (2, 1, -2.0, 0.0),
+                 (2, 2, 6.0, 0.0)]
+    x_h = np.array([0.5, -1.25, 2.0])
+    x = torch.tensor(x_h, device="cuda")
+    ref = OL.Reference.load()
+    name = "Ellpack" if fmt == tnl.ELLPACK else "SlicedEllpack32"
+    for symmetric, ops in ((False, ops_general), (True, ops_lower)):
+        D = np.zeros((n, n))
+        for r, c, v, _ in ops:
+            D[r, c] = v
+        A = tnl.SparseMatrix.from_csr(n, n, off, np.full(5, -1, np.int32), np.zeros(5), fmt=fmt, param=param,
+                                      organization=org, symmetric=symmetric)
+        apply(A, ops, -1)
+        if ref is not None:
+            rc, rv, rfailed, _ = ref.add_elements(name, off, n, ops, symmetric=symmetric,
+                                                  col_major=org == tnl.COLUMN_MAJOR)
+            assert rfailed == -1
+            assert np.array_equal(A.column_indexes.cpu().numpy()[: len(rc)], rc)
+            assert np.array_equal(A.values.cpu().numpy()[: len(rv)], rv)
+        full = D + D.T - np.diag(np.diag(D)) if symmetric else D
+        y = torch.zeros(n, dtype=torch.float64, device="cuda")
+        A.vectorProduct(x, y)
+        assert np.allclose(y.cpu().numpy(), full @ x_h, rtol=0, atol=1e-15), symmetric
+        if not symmetric:
+            yt = torch.zeros(n, dtype=torch.float64, device="cuda")
+            A.transposedVectorProduct(x, yt)
+            assert np.allclose(yt.cpu().numpy(), D.T @ x_h, rtol=0, atol=1e-15)
+        B = A.convert(tnl.CSR)
+        C = B.convert(fmt, param=param, organization=org)
+        for M in (B, C):
+            got = np.array([[M.getElement(r, c) for c in range(n)] for r in range(n)])
+            assert np.array_equal(got, D), symmetric
+        P = solvers.Diagonal(A)
+        b = torch.tensor([4.0, 10.0, 12.0], dtype=torch.float64, device="cuda")
+        z = torch.zeros(n, dtype=torch.float64, device="cuda")
+        P.solve(b, z)
+        assert np.array_equal(z.cpu().numpy(), b.cpu().numpy() / np.diag(D)), symmetric

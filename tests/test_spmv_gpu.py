@@ -6,8 +6,11 @@
 * random matrices with padding slots, alpha/beta and row ranges: <= 1e-12;
 * FP32 and BF16-stored values: against the oracle in FP32/FP64 with the
   tolerance stated per case;
-* the full power-law matrix (4,194,304 rows, 67M nonzeros) FP64: <= 1e-12
-  against the oracle's OpenMP product;
+* the full power-law matrix (4,194,304 rows, 67M nonzeros): FP64 <= 1e-12
+  against the oracle's OpenMP product in every format and through the CSR
+  launch configurations of the reference (scalar, vector, light, block,
+  dynamic grouping); FP32 values and vectors in every format within the
+  per-row FP32 rounding bound;
 * error behaviour of the reference (size mismatch, invalid threads per segment).
 """
 import numpy as np
@@ -146,14 +149,20 @@ def test_fp32_and_bf16(orc, fmt, param):
     assert relerr(got, orc.spmv(off, cols, vb, x)) <= 1e-12
 
 
-def test_powerlaw_full_size(orc):
-    # the benchmark matrix at full size: 4,194,304 rows, mean 16 nonzeros per row
+@pytest.fixture(scope="module")
+def powerlaw():
     off, cols, vals = tnl.zipf_csr(4_194_304, seed=0)
     n = len(off) - 1
+    x = tnl.uniform(n, -1.0, 1.0, seed=0)
+    return n, off, cols, vals, x
+
+
+def test_powerlaw_full_size(orc, powerlaw):
+    # the benchmark matrix at full size: 4,194,304 rows, mean 16 nonzeros per row
+    n, off, cols, vals, x = powerlaw
     assert int(off[-1]) == 16 * n
     sizes = np.diff(off)
     assert sizes.min() >= 1 and sizes.max() <= 4096
-    x = tnl.uniform(n, -1.0, 1.0, seed=0)
     want = orc.spmv_omp(off, cols, vals, x, 16)
     for fmt, param in [(tnl.CSR, 0), (tnl.SLICED_ELLPACK, 64), (tnl.BIELLPACK, 32), (tnl.CHUNKED_ELLPACK, 0),
                        (tnl.ADAPTIVE_CSR, 0), (tnl.SORTED_CSR, 0)]:
@@ -164,6 +173,51 @@ def test_powerlaw_full_size(orc):
     # Ellpack is infeasible here: width 4096 x 4,194,304 rows overflows int32 (Ellpack.hpp:91-95)
     with pytest.raises(OverflowError):
         tnl.Segments(tnl.ELLPACK, sizes)
+
+
+# the CSR launch configurations of SparseMatrixVectorProductTest.hpp at the
+# benchmark matrix's size: scalar (Fixed 1), VariableVector (Fixed 2..32),
+# LightMultivector (Fixed 64..256), Vector (Warp), Block, BlockMerged and
+# DynamicGrouping (ReducingOperations_CSR.h:87-332)
+POWERLAW_LAUNCHES = [(tnl.FIXED, 1), (tnl.FIXED, 4), (tnl.FIXED, 16), (tnl.FIXED, 64), (tnl.FIXED, 256),
+                     (tnl.WARP, 1), (tnl.BLOCK, 1), (tnl.BLOCK_MERGED, 1), (tnl.DYNAMIC_GROUPING, 1),
+                     (tnl.DYNAMIC_GROUPING, 32), (tnl.AUTO, 1)]
+
+
+def test_powerlaw_full_size_csr_launches(orc, powerlaw):
+    n, off, cols, vals, x = powerlaw
+    want = orc.spmv_omp(off, cols, vals, x, 16)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    for launch in POWERLAW_LAUNCHES:
+        print("power-law CSR FP64 launch", launch, flush=True)
+        assert relerr(run(A, x, launch=launch), want) <= 1e-12, launch
+
+
+def test_powerlaw_full_size_fp32(orc, powerlaw):
+    # FP32 values, vectors and accumulation in any summation order:
+    # |y - y_exact| <= (len + 3) 2^-24 sum_j |a_ij x_j| per row (products,
+    # sums and the rounding of x), y_exact from the FP64 oracle on the
+    # FP32-rounded inputs
+    n, off, cols, vals, x = powerlaw
+    v32 = vals.astype(np.float32)
+    x32 = x.astype(np.float32)
+    v64, xr = v32.astype(np.float64), x32.astype(np.float64)
+    want = orc.spmv_omp(off, cols, v64, xr, 16)
+    absrow = orc.spmv_omp(off, cols, np.abs(v64), np.abs(xr), 16)
+    lens = np.diff(off).astype(np.float64)
+    tol = (lens + 3) * 2.0**-24 * absrow * 1.01 + np.abs(want) * 2.0**-24 + 1e-30
+    for fmt, param in [(tnl.CSR, 0), (tnl.ADAPTIVE_CSR, 0), (tnl.SLICED_ELLPACK, 64), (tnl.BIELLPACK, 32),
+                       (tnl.CHUNKED_ELLPACK, 0), (tnl.SORTED_CSR, 0)]:
+        print("power-law FP32", tnl.FORMAT_NAMES[fmt], flush=True)
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, v32, fmt=fmt, param=param, value_dtype=torch.float32)
+        launches = [None, (tnl.FIXED, 1), (tnl.WARP, 1), (tnl.FIXED, 16), (tnl.DYNAMIC_GROUPING, 1)] \
+            if fmt == tnl.CSR else [None]
+        for launch in launches:
+            got = run(A, x32, dtype=torch.float32, launch=launch)
+            bad = np.nonzero(np.abs(got - want) > tol)[0]
+            assert len(bad) == 0, (tnl.FORMAT_NAMES[fmt], launch, bad[:5])
+        del A
+        torch.cuda.empty_cache()
 
 
 def test_errors():

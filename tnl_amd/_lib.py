@@ -169,7 +169,7 @@ _PROTOS = {
                                              ctypes.c_float, ctypes.POINTER(ctypes.c_double), c_vp]),
     "tnlmi_bf16_gemm64": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "tnlmi_batched_cg_create": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_i64,
-                                               ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)]),
+                                               ctypes.c_int32, ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)]),
     "tnlmi_batched_cg_destroy": (None, [c_vp]),
     "tnlmi_batched_cg_solve": (ctypes.c_int, [c_vp, ctypes.c_int32, c_vp, c_vp, ctypes.c_double, c_i64,
                                               ctypes.c_double, c_vp, c_vp, ctypes.c_int32, c_vp]),

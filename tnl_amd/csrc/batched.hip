@@ -18,6 +18,7 @@
 // No reference counterpart (TNL solves one system per solver call).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <tuple>
 
@@ -31,7 +32,9 @@ __device__ __forceinline__ T row_dot( const SegView& v, const int32_t* __restric
                                       const T* __restrict__ x, int64_t r )
 {
    T acc = 0;
-   const int32_t len = v.sizes[ r ];
+   // every getSegmentSize slot: an Ellpack-type row may hold entries past its
+   // requested capacity
+   const int32_t len = segment_size( v, r );
    if( v.format == TNLMI_CSR ) {
       const int64_t b = v.offsets[ r ];
       for( int32_t k = 0; k < len; k++ ) {
@@ -48,6 +51,40 @@ __device__ __forceinline__ T row_dot( const SegView& v, const int32_t* __restric
             acc += (T) load_val< V, T >( vals, g ) * x[ c ];
       }
    return acc;
+}
+
+// The same row sum spread over the T lanes of a group (T = 1, 2, 4, 8: the
+// rows per wavefront are 64 / T): lane j sums slots j, j + T, ..., then an xor
+// butterfly inside the group; every lane returns the row's sum.  All T lanes
+// of a group hold the same row, so the group is active or idle as a whole.
+template< int T, typename A, typename V >
+__device__ __forceinline__ A row_dot_group( const SegView& v, const int32_t* __restrict__ cols,
+                                            const V* __restrict__ vals, const A* __restrict__ x, int64_t r, int lane )
+{
+   if constexpr( T == 1 )
+      return row_dot< A, V >( v, cols, vals, x, r );
+   else {
+      A acc = 0;
+      const int32_t len = segment_size( v, r );
+      if( v.format == TNLMI_CSR ) {
+         const int64_t b = v.offsets[ r ];
+         for( int32_t k = lane; k < len; k += T ) {
+            const int32_t c = cols[ b + k ];
+            if( c >= 0 )
+               acc += (A) load_val< V, A >( vals, b + k ) * x[ c ];
+         }
+      }
+      else
+         for( int32_t k = lane; k < len; k += T ) {
+            const int64_t g = global_index( v, r, k );
+            const int32_t c = cols[ g ];
+            if( c >= 0 )
+               acc += (A) load_val< V, A >( vals, g ) * x[ c ];
+         }
+      for( int off = T / 2; off > 0; off /= 2 )
+         acc += __shfl_xor( acc, off, T );
+      return acc;
+   }
 }
 
 struct BatchArgs
@@ -77,6 +114,13 @@ struct BatchArgs
    double* res;
 };
 
+inline uint64_t double_bits( double v )
+{
+   uint64_t u;
+   std::memcpy( &u, &v, sizeof( u ) );
+   return u;
+}
+
 __global__ void reset_counter( int* c )
 {
    c[ 0 ] = 0;
@@ -90,7 +134,8 @@ __global__ void batch_inv_diag( SegView v, const int32_t* __restrict__ cols, con
    if( r >= v.n )
       return;
    double d = 0;
-   for( int32_t k = 0; k < v.sizes[ r ]; k++ ) {
+   const int32_t size = segment_size( v, r );
+   for( int32_t k = 0; k < size; k++ ) {
       const int64_t g = global_index( v, r, k );
       if( cols[ g ] == r ) {
          d = (double) load_val< V, double >( vals, g );
@@ -98,6 +143,32 @@ __global__ void batch_inv_diag( SegView v, const int32_t* __restrict__ cols, con
       }
    }
    dinv[ r ] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// Sets *bad when a stored column of row r (padding, -1, aside) lies outside the
+// row range of r's system: the persistent kernel solves each system alone and
+// would read the vectors of another system that a concurrent workgroup updates.
+__global__ void check_blocks( SegView v, const int32_t* __restrict__ cols, const int64_t* __restrict__ sys_rows,
+                              int64_t nsys, int32_t* bad )
+{
+   const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( r >= v.n )
+      return;
+   int64_t lo = 0, hi = nsys;  // the system k with sys_rows[k] <= r < sys_rows[k+1]
+   while( hi - lo > 1 ) {
+      const int64_t mid = ( lo + hi ) / 2;
+      if( sys_rows[ mid ] <= r )
+         lo = mid;
+      else
+         hi = mid;
+   }
+   const int64_t first = sys_rows[ lo ], last = sys_rows[ lo + 1 ];
+   const int32_t size = segment_size( v, r );
+   for( int32_t k = 0; k < size; k++ ) {
+      const int32_t c = cols[ global_index( v, r, k ) ];
+      if( c != -1 && ( c < first || c >= last ) )
+         *bad = 1;
+   }
 }
 
 // The next system for the whole workgroup.  Only whole wavefronts branch
@@ -128,23 +199,27 @@ __device__ __forceinline__ void put_result( int64_t* iters, double* res, int64_t
 }
 
 // FP64 Jacobi-CG per system, TNL's stopping rule with default min/divergence
-template< int B >
+template< int B, int T >
 __global__ void __launch_bounds__( B ) batched_cg64( BatchArgs a )
 {
+   constexpr int G = B / T;  // row groups of T lanes
+   const int grp = threadIdx.x / T, lane = threadIdx.x % T;
    __shared__ double red[ B / 64 ];
    __shared__ int slot;
    for( int64_t sys = next_system( a.counter, &slot ); sys < a.nsys; sys = next_system( a.counter, &slot ) ) {
       const int64_t r0 = a.sys_rows[ sys ], r1 = a.sys_rows[ sys + 1 ];
       // r = b - A x, p = z = D^{-1} r, s1 = (r, z), normb = ||D^{-1} b||
       double acc1 = 0, acc2 = 0;
-      for( int64_t i = r0 + threadIdx.x; i < r1; i += B ) {
-         const double ri = a.b[ i ] - row_dot< double, double >( a.v, a.cols, a.vals, a.x, i );
-         const double zi = ri * a.dinv[ i ];
-         a.r[ i ] = ri;
-         a.p[ i ] = zi;
-         acc1 += ri * zi;
-         const double mb = a.b[ i ] * a.dinv[ i ];
-         acc2 += mb * mb;
+      for( int64_t i = r0 + grp; i < r1; i += G ) {
+         const double ri = a.b[ i ] - row_dot_group< T, double, double >( a.v, a.cols, a.vals, a.x, i, lane );
+         if( lane == 0 ) {
+            const double zi = ri * a.dinv[ i ];
+            a.r[ i ] = ri;
+            a.p[ i ] = zi;
+            acc1 += ri * zi;
+            const double mb = a.b[ i ] * a.dinv[ i ];
+            acc2 += mb * mb;
+         }
       }
       double s1 = block_sum< B >( acc1, red );
       const double normb = sqrt( block_sum< B >( acc2, red ) );
@@ -159,10 +234,12 @@ __global__ void __launch_bounds__( B ) batched_cg64( BatchArgs a )
          while( ! ( isnan( res ) || it > a.maxit || res < a.tol ) ) {
             it++;
             double acc = 0;
-            for( int64_t i = r0 + threadIdx.x; i < r1; i += B ) {
-               const double api = row_dot< double, double >( a.v, a.cols, a.vals, a.p, i );
-               a.Ap[ i ] = api;
-               acc += api * a.p[ i ];
+            for( int64_t i = r0 + grp; i < r1; i += G ) {
+               const double api = row_dot_group< T, double, double >( a.v, a.cols, a.vals, a.p, i, lane );
+               if( lane == 0 ) {
+                  a.Ap[ i ] = api;
+                  acc += api * a.p[ i ];
+               }
             }
             const double s2 = block_sum< B >( acc, red );
             if( s2 == 0.0 ) {
@@ -192,9 +269,11 @@ __global__ void __launch_bounds__( B ) batched_cg64( BatchArgs a )
 
 // Mixed: FP64 refinement around FP32 Jacobi-CG (FP32 values, vectors, SpMV
 // accumulation; workgroup reductions in FP64).  iters[] = inner iterations.
-template< int B >
+template< int B, int T >
 __global__ void __launch_bounds__( B ) batched_cg_mixed( BatchArgs a )
 {
+   constexpr int G = B / T;
+   const int grp = threadIdx.x / T, lane = threadIdx.x % T;
    __shared__ double red[ B / 64 ];
    __shared__ int slot;
    for( int64_t sys = next_system( a.counter, &slot ); sys < a.nsys; sys = next_system( a.counter, &slot ) ) {
@@ -208,10 +287,12 @@ __global__ void __launch_bounds__( B ) batched_cg_mixed( BatchArgs a )
       while( bnorm > 0.0 ) {
          // r = b - A x (FP64) and the true residual
          acc = 0;
-         for( int64_t i = r0 + threadIdx.x; i < r1; i += B ) {
-            const double ri = a.b[ i ] - row_dot< double, double >( a.v, a.cols, a.vals, a.x, i );
-            a.r[ i ] = ri;
-            acc += ri * ri;
+         for( int64_t i = r0 + grp; i < r1; i += G ) {
+            const double ri = a.b[ i ] - row_dot_group< T, double, double >( a.v, a.cols, a.vals, a.x, i, lane );
+            if( lane == 0 ) {
+               a.r[ i ] = ri;
+               acc += ri * ri;
+            }
          }
          const double rn = sqrt( block_sum< B >( acc, red ) );
          res = rn / bnorm;
@@ -234,10 +315,12 @@ __global__ void __launch_bounds__( B ) batched_cg_mixed( BatchArgs a )
          for( int64_t k = 0; k < a.inner_maxit && s1 > a.inner_tol * a.inner_tol * s10; k++ ) {
             inner_total++;
             double q = 0;
-            for( int64_t i = r0 + threadIdx.x; i < r1; i += B ) {
-               const float api = row_dot< float, float >( a.v, a.cols, a.vals_lo, a.p32, i );
-               a.Ap32[ i ] = api;
-               q += (double) api * (double) a.p32[ i ];
+            for( int64_t i = r0 + grp; i < r1; i += G ) {
+               const float api = row_dot_group< T, float, float >( a.v, a.cols, a.vals_lo, a.p32, i, lane );
+               if( lane == 0 ) {
+                  a.Ap32[ i ] = api;
+                  q += (double) api * (double) a.p32[ i ];
+               }
             }
             const double s2 = block_sum< B >( q, red );
             if( s2 == 0.0 )
@@ -276,13 +359,16 @@ struct tnlmi_batched_cg
    const tnlmi_matrix* A_lo = nullptr;
    int64_t nsys = 0, rows = 0;
    int32_t block = 512;
+   int32_t lanes = 1;  // lanes per row in the row sums (64 / lanes rows per wavefront)
    int64_t grid = 1;  // persistent workgroups: a few per CU, at most one per system
    DeviceArray< int64_t > sys_rows;
    DeviceArray< double > dinv, r, p, Ap;
    DeviceArray< float > r32, p32, Ap32, d32;
    DeviceArray< int > counter;
    // captured launches keyed by their arguments
-   std::map< std::tuple< const void*, const void*, const void*, const void*, double, int64_t, int >, hipGraphExec_t >
+   // (the tolerances by their bit patterns: a NaN key still orders)
+   std::map< std::tuple< const void*, const void*, const void*, const void*, uint64_t, uint64_t, int64_t, int >,
+            hipGraphExec_t >
       graphs;
    hipStream_t capture_stream = nullptr;
    ~tnlmi_batched_cg()
@@ -300,20 +386,22 @@ void launch( tnlmi_batched_cg& P, const BatchArgs& a, bool mixed, hipStream_t s 
 {
    hipLaunchKernelGGL( reset_counter, dim3( 1 ), dim3( 1 ), 0, s, a.counter );
    const dim3 grid( (unsigned) P.grid );
-#define TNLMI_BATCH( BS )                                                                                             \
-   if( P.block == BS ) {                                                                                              \
+#define TNLMI_BATCH( BS, T )                                                                                          \
+   if( P.block == BS && P.lanes == T ) {                                                                              \
       if( mixed )                                                                                                     \
-         hipLaunchKernelGGL( batched_cg_mixed< BS >, grid, dim3( BS ), 0, s, a );                                     \
+         hipLaunchKernelGGL( ( batched_cg_mixed< BS, T > ), grid, dim3( BS ), 0, s, a );                              \
       else                                                                                                            \
-         hipLaunchKernelGGL( batched_cg64< BS >, grid, dim3( BS ), 0, s, a );                                         \
+         hipLaunchKernelGGL( ( batched_cg64< BS, T > ), grid, dim3( BS ), 0, s, a );                                  \
       return;                                                                                                         \
    }
-   TNLMI_BATCH( 128 )
-   TNLMI_BATCH( 256 )
-   TNLMI_BATCH( 512 )
-   TNLMI_BATCH( 1024 )
+#define TNLMI_BATCH_T( BS ) TNLMI_BATCH( BS, 1 ) TNLMI_BATCH( BS, 2 ) TNLMI_BATCH( BS, 4 ) TNLMI_BATCH( BS, 8 )
+   TNLMI_BATCH_T( 128 )
+   TNLMI_BATCH_T( 256 )
+   TNLMI_BATCH_T( 512 )
+   TNLMI_BATCH_T( 1024 )
+#undef TNLMI_BATCH_T
 #undef TNLMI_BATCH
-   fail( TNLMI_INVALID_ARGUMENT, "batched CG: block size must be 128, 256, 512 or 1024" );
+   fail( TNLMI_INVALID_ARGUMENT, "batched CG: block size must be 128, 256, 512 or 1024 and lanes per row 1, 2, 4 or 8" );
 }
 
 }  // namespace
@@ -321,7 +409,8 @@ void launch( tnlmi_batched_cg& P, const BatchArgs& a, bool mixed, hipStream_t s 
 extern "C" {
 
 tnlmi_status tnlmi_batched_cg_create( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const int64_t* sys_rows,
-                                      int64_t nsys, int32_t block_size, tnlmi_stream stream, tnlmi_batched_cg** out )
+                                      int64_t nsys, int32_t block_size, int32_t lanes_per_row, tnlmi_stream stream,
+                                      tnlmi_batched_cg** out )
 {
    return guarded( [ & ] {
       if( ! out || ! A || ! A->segments || ! sys_rows || nsys <= 0 )
@@ -348,6 +437,9 @@ tnlmi_status tnlmi_batched_cg_create( const tnlmi_matrix* A, const tnlmi_matrix*
          P->block = block_size ? block_size : 512;
          if( P->block != 128 && P->block != 256 && P->block != 512 && P->block != 1024 )
             fail( TNLMI_INVALID_ARGUMENT, "batched CG: block size must be 128, 256, 512 or 1024" );
+         P->lanes = lanes_per_row ? lanes_per_row : 1;
+         if( P->lanes != 1 && P->lanes != 2 && P->lanes != 4 && P->lanes != 8 )
+            fail( TNLMI_INVALID_ARGUMENT, "batched CG: lanes per row must be 1, 2, 4 or 8" );
          int dev = 0, cus = 256;
          TNLMI_HIP( hipGetDevice( &dev ) );
          TNLMI_HIP( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
@@ -362,6 +454,22 @@ tnlmi_status tnlmi_batched_cg_create( const tnlmi_matrix* A, const tnlmi_matrix*
             for( auto* v : { &P->r32, &P->p32, &P->Ap32, &P->d32 } )
                v->allocate( n );
          P->counter.allocate( 1 );
+         {
+            DeviceArray< int32_t > bad;
+            bad.allocate( 1 );
+            TNLMI_HIP( hipMemsetAsync( bad.ptr, 0, sizeof( int32_t ), s ) );
+            if( n )
+               hipLaunchKernelGGL( check_blocks, dim3( (unsigned) ceil_div( n, 256 ) ), dim3( 256 ), 0, s,
+                                   make_view( *A->segments ), A->column_indexes, P->sys_rows.ptr, nsys, bad.ptr );
+            TNLMI_HIP( hipGetLastError() );
+            int32_t h = 0;
+            TNLMI_HIP( hipMemcpyAsync( &h, bad.ptr, sizeof( h ), hipMemcpyDeviceToHost, s ) );
+            TNLMI_HIP( hipStreamSynchronize( s ) );
+            if( h )
+               fail( TNLMI_INVALID_ARGUMENT,
+                     "batched_cg_create: a stored column lies outside its system's row range (the systems must be "
+                     "the diagonal blocks of the matrix)" );
+         }
          hipLaunchKernelGGL( batch_inv_diag< double >, dim3( (unsigned) ceil_div( n, 256 ) ), dim3( 256 ), 0, s,
                              make_view( *A->segments ), A->column_indexes, (const double*) A->values, P->dinv.ptr );
          TNLMI_HIP( hipGetLastError() );
@@ -422,8 +530,8 @@ tnlmi_status tnlmi_batched_cg_solve( tnlmi_batched_cg* P, int32_t mixed, const d
       }
       // one captured launch per argument set, replayed on later calls
       const auto key = std::make_tuple( (const void*) d_b, (const void*) d_x, (const void*) d_iterations,
-                                        (const void*) d_residues, tolerance + 1e3 * inner_tolerance,
-                                        max_iterations, (int) mixed );
+                                        (const void*) d_residues, double_bits( tolerance ),
+                                        double_bits( inner_tolerance ), max_iterations, (int) mixed );
       auto it = P->graphs.find( key );
       if( it == P->graphs.end() ) {
          if( ! P->capture_stream )

@@ -15,7 +15,7 @@ namespace {
 __global__ void copy_row_lengths( SegView v, const int32_t* __restrict__ cols, int64_t ncols, int32_t* __restrict__ len )
 {
    for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < v.n; r += (int64_t) gridDim.x * blockDim.x ) {
-      const int32_t size = v.sizes[ r ];
+      const int32_t size = segment_size( v, r );
       int32_t L = 0;
       for( int32_t k = 0; k < size; k++ ) {
          if( cols[ global_index( v, r, k ) ] < ncols )
@@ -45,7 +45,7 @@ __global__ void copy_rows( SegView a, T* __restrict__ avals, int32_t* __restrict
 {
    for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < b.n; r += (int64_t) gridDim.x * blockDim.x ) {
       const int32_t L = len[ r ];
-      if( L < 0 || L > a.sizes[ r ] || L > b.sizes[ r ] ) {
+      if( L < 0 || L > segment_size( a, r ) || L > segment_size( b, r ) ) {
          *bad = 1;
          continue;
       }

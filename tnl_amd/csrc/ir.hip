@@ -137,7 +137,8 @@ __global__ void inv_diag32( SegView v, const int32_t* __restrict__ cols, const V
    if( r >= v.n )
       return;
    float d = 0.0f;
-   for( int32_t k = 0; k < v.sizes[ r ]; k++ ) {
+   const int32_t size = segment_size( v, r );
+   for( int32_t k = 0; k < size; k++ ) {
       const int64_t g = global_index( v, r, k );
       if( cols[ g ] == r ) {
          d = load_val< V, float >( vals, g );

@@ -45,7 +45,8 @@ __global__ void diag_kernel( SegView v, const int32_t* __restrict__ cols, const 
    if( r >= v.n )
       return;
    double dv = 0.0;
-   for( int32_t k = 0; k < v.sizes[ r ]; k++ ) {
+   const int32_t size = segment_size( v, r );
+   for( int32_t k = 0; k < size; k++ ) {
       const int64_t g = global_index( v, r, k );
       if( cols[ g ] == r ) {
          dv = (double) load_val< V, double >( vals, g );
@@ -55,16 +56,26 @@ __global__ void diag_kernel( SegView v, const int32_t* __restrict__ cols, const 
    d[ r ] = dv;
 }
 
-// CSR-ordered copy of any format (padding dropped later on the host)
+// getSegmentSize of every row (an Ellpack-type row may hold entries past its
+// requested capacity)
+__global__ void seg_size_kernel( SegView v, int32_t* __restrict__ out )
+{
+   const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( r < v.n )
+      out[ r ] = segment_size( v, r );
+}
+
+// CSR-ordered copy of every slot of any format, row r at eoff[r] (the scan of
+// the segment sizes; padding dropped later on the host)
 template< typename V >
 __global__ void to_csr_kernel( SegView v, const int32_t* __restrict__ cols, const V* __restrict__ vals,
-                               int32_t* __restrict__ oc, double* __restrict__ ov )
+                               const int32_t* __restrict__ eoff, int32_t* __restrict__ oc, double* __restrict__ ov )
 {
    const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    if( r >= v.n )
       return;
-   const int32_t b = v.offsets[ r ];
-   for( int32_t k = 0; k < v.sizes[ r ]; k++ ) {
+   const int32_t b = eoff[ r ], size = eoff[ r + 1 ] - b;
+   for( int32_t k = 0; k < size; k++ ) {
       const int64_t g = global_index( v, r, k );
       oc[ b + k ] = cols[ g ];
       ov[ b + k ] = (double) load_val< V, double >( vals, g );
@@ -163,7 +174,15 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
 {
    const tnlmi_segments& seg = *A.segments;
    const int64_t N = A.rows;
-   const int64_t nel = seg.elements;
+   DeviceArray< int32_t > dsz, doff;
+   dsz.allocate( std::max< int64_t >( 1, N ) );
+   doff.allocate( N + 1 );
+   if( N )
+      hipLaunchKernelGGL( seg_size_kernel, dim3( (unsigned) ceil_div( N, 256 ) ), dim3( 256 ), 0, s, make_view( seg ),
+                          dsz.ptr );
+   const int64_t nel = exclusive_scan_i32( dsz.ptr, doff.ptr, N, s );
+   if( nel > INT32_MAX )
+      fail( TNLMI_OVERFLOW_ERROR, "ILU0: the matrix has more than 2^31 - 1 slots" );
    DeviceArray< int32_t > dc;
    DeviceArray< double > dv;
    dc.allocate( std::max< int64_t >( 1, nel ) );
@@ -172,11 +191,11 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
       using V = std::remove_const_t< std::remove_pointer_t< decltype( tag ) > >;
       if( N )
          hipLaunchKernelGGL( to_csr_kernel< V >, dim3( (unsigned) ceil_div( N, 256 ) ), dim3( 256 ), 0, s,
-                             make_view( seg ), A.column_indexes, (const V*) A.values, dc.ptr, dv.ptr );
+                             make_view( seg ), A.column_indexes, (const V*) A.values, doff.ptr, dc.ptr, dv.ptr );
    } );
    std::vector< int32_t > off( N + 1 ), col( nel );
    std::vector< double > val( nel );
-   TNLMI_HIP( hipMemcpyAsync( off.data(), seg.offsets.ptr, 4 * ( N + 1 ), hipMemcpyDeviceToHost, s ) );
+   TNLMI_HIP( hipMemcpyAsync( off.data(), doff.ptr, 4 * ( N + 1 ), hipMemcpyDeviceToHost, s ) );
    if( nel ) {
       TNLMI_HIP( hipMemcpyAsync( col.data(), dc.ptr, 4 * nel, hipMemcpyDeviceToHost, s ) );
       TNLMI_HIP( hipMemcpyAsync( val.data(), dv.ptr, 8 * nel, hipMemcpyDeviceToHost, s ) );

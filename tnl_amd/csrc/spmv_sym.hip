@@ -56,7 +56,7 @@ __global__ void __launch_bounds__( kSymRows )
    C sum = 0;
    if( r < end ) {
       const C xr = (C) x[ r ];
-      const int32_t size = v.sizes[ r ];
+      const int32_t size = segment_size( v, r );
       for( int32_t k = 0; k < size; k++ ) {
          const int64_t g = global_index( v, r, k );
          const int32_t c = cols[ g ];
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__( 256 )
 {
    for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < v.n; r += (int64_t) gridDim.x * blockDim.x ) {
       const C xr = (C) x[ r ];
-      const int32_t size = v.sizes[ r ];
+      const int32_t size = segment_size( v, r );
       for( int32_t k = 0; k < size; k++ ) {
          const int64_t g = global_index( v, r, k );
          const int32_t c = cols[ g ];
