@@ -361,6 +361,78 @@ def _cpu_cg_per_iteration(off, cols, vals, b):
     return max(ts[1] - ts[0], 1e-9) / 20
 
 
+def time_to_solution_poisson(side, delta, log_prefix=""):
+    """Jacobi-CG to ||b - A x|| / ||b|| <= 1e-10 on the 3D 7-point Poisson
+    matrix of side^3 rows (b ~ N(0,1), seed 1), every variant stopped on that
+    same true-residual rule: FP64 CG (int32 columns, and the 8-bit column
+    codes the mixed solver reads) against the reliable-update FP32 CG on BF16
+    and FP32 values; TNL's own rule (the preconditioned residue) reported
+    beside them.  Returns a dict of results and the speed-ups."""
+    import torch
+    import tnl_amd
+    from tnl_amd import solvers
+
+    t0 = time.perf_counter()
+    off, cols, vals = tnl_amd.poisson3d(side, side, side)
+    n, nnz = len(off) - 1, int(off[-1])
+    b = tnl_amd.normal(n, seed=1)
+    log(f"{log_prefix}Poisson {side}^3: {n} rows, {nnz} nnz, generated in {time.perf_counter() - t0:.1f}s")
+    bd = torch.from_numpy(b).cuda()
+    tol = 1e-10
+    res = {}
+    A = tnl_amd.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=tnl_amd.ELLPACK)
+    D = solvers.Diagonal(A)
+
+    def run(label, solver):
+        solver.setMatrix(A)
+        solver.setConvergenceResidue(tol)
+        solver.setMaxIterations(20_000)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        solver.solve(bd, x)  # warm-up (kernels, column codes, plan)
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok = solver.solve(bd, x)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        tr = solvers.true_residual(A, x, bd)
+        rec = {"seconds": round(t, 4), "iterations": solver.getIterations(), "true_residual": tr,
+               "converged_below_1e-10": bool(ok and tr <= tol)}
+        if hasattr(solver, "getUpdates"):
+            rec["fp64_residual_updates"] = solver.getUpdates()
+        log(f"{log_prefix}{label}: {rec}")
+        res[label] = rec
+
+    for codes in (False, True):
+        cg = solvers.TrueResidualCG(column_codes=codes)
+        cg.setPreconditioner(D)
+        run(f"fp64_cg[Ellpack, {'8-bit column codes' if codes else 'int32 columns'}]", cg)
+    tnl_rule = solvers.CG()
+    tnl_rule.setPreconditioner(D)
+    run("fp64_cg_tnl_rule[Ellpack, int32 columns]", tnl_rule)
+    for lo_name, lo_dtype in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        A_lo = tnl_amd.SparseMatrix(A.rows, A.columns, A.segments, A.values.to(lo_dtype), A.column_indexes)
+        m = solvers.ReliableMixedCG()
+        m.setInnerMatrix(A_lo)
+        m.setDelta(delta)
+        run(f"reliable_{lo_name}[Ellpack, 8-bit column codes]", m)
+        del A_lo
+    del A, D
+    torch.cuda.empty_cache()
+    ok = {k: v for k, v in res.items() if v["converged_below_1e-10"]}
+    t32 = ok.get("fp64_cg[Ellpack, int32 columns]", {}).get("seconds")
+    t8 = ok.get("fp64_cg[Ellpack, 8-bit column codes]", {}).get("seconds")
+    mixed = {k: v for k, v in ok.items() if k.startswith("reliable")}
+    best = min(mixed, key=lambda k: mixed[k]["seconds"]) if mixed else None
+    tmx = mixed[best]["seconds"] if best else None
+    return {"workload": f"Poisson {side}^3 ({n} rows, {nnz} nonzeros), Jacobi-CG, b ~ N(0,1) seed 1",
+            "stopping_rule": "||b - A x|| / ||b|| <= 1e-10 for every variant (FP64 b - A x confirmed at the stop)",
+            "best_mixed": best,
+            "speedup_vs_fp64_int32_columns": round(t32 / tmx, 3) if t32 and tmx else None,
+            "speedup_vs_fp64_same_column_codes": round(t8 / tmx, 3) if t8 and tmx else None,
+            "results": res}
+
+
 def wl_cg_poisson(args):
     import torch
     import tnl_amd

@@ -319,6 +319,19 @@ typedef struct tnlmi_spmv_plan tnlmi_spmv_plan;
 tnlmi_status tnlmi_spmv_plan_create( const tnlmi_matrix* A, int32_t parts, tnlmi_stream stream,
                                      tnlmi_spmv_plan** out );
 void tnlmi_spmv_plan_destroy( tnlmi_spmv_plan* plan );
+/* Under TNLMI_MAP_AUTO, tnlmi_spmv runs the products of a general CSR matrix
+ * with FP64 or FP32 values, vectors and compute type through a windowed plan
+ * it builds once and keeps with the segments (20 extra bytes per nonzero),
+ * when x is wider than an L2 (at least 2^20 columns, 2^22 stored elements)
+ * and the columns of sampled rows mostly lie far from the diagonal: the
+ * reference's default vectorProduct call then gets the fastest product of
+ * such a matrix.  The plan belongs to the matrix's value and column-index
+ * arrays; the library drops it when it writes them (tnlmi_matrix_fill,
+ * tnlmi_matrix_add_element, tnlmi_matrix_copy into them).  A caller that
+ * writes them itself calls tnlmi_matrix_modified( A ) before the next
+ * product.  TNLMI_AUTO_PLAN=0 in the environment turns this off.
+ * No reference counterpart (the reference's matrices keep no derived data). */
+tnlmi_status tnlmi_matrix_modified( const tnlmi_matrix* A );
 tnlmi_status tnlmi_spmv_plan_execute( const tnlmi_spmv_plan* plan, tnlmi_dtype x_type, const void* d_x, void* d_y,
                                       tnlmi_dtype compute_type, double alpha, double beta, int64_t begin, int64_t end,
                                       tnlmi_stream stream );
@@ -402,6 +415,20 @@ tnlmi_status tnlmi_tfqmr_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* pl
  * restarting_max <= 31).  Jacobi: Jacobi.hpp:65-125. */
 tnlmi_status tnlmi_cg_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
                              const double* d_b, double* d_x, tnlmi_solver_state* state, tnlmi_stream stream );
+/* CG's iteration (CG.hpp:10-108, no or the Diagonal preconditioner) stopped
+ * on the residual ||b - A x|| / ||b|| <= convergence_residue instead of the
+ * reference's preconditioned residue: the recurrence residual is tracked in
+ * every iteration and confirmed by an FP64 b - A x at the stop (a drift
+ * above the tolerance continues from x).  state->residue is that true
+ * residual.  The stopping rule of tnlmi_mixed_cg_solve, for like-for-like
+ * comparisons of FP64 and mixed-precision solves.  column_codes != 0: a
+ * column-major Ellpack / SlicedEllpack A with at most 255 distinct offsets
+ * column - row is read through 8-bit column codes, as the mixed solver reads
+ * its inner matrix (identical results, fewer bytes).  No reference
+ * counterpart. */
+tnlmi_status tnlmi_cg_solve_true_residual( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan,
+                                           const tnlmi_preconditioner* M, int32_t column_codes, const double* d_b,
+                                           double* d_x, tnlmi_solver_state* state, tnlmi_stream stream );
 typedef enum {
    TNLMI_GMRES_CGS = 0,
    TNLMI_GMRES_CGSR = 1,

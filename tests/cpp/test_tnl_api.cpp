@@ -2,9 +2,17 @@
 // against TNL's class templates with Devices::MI355X in place of
 // Devices::Host (src/UnitTests/Matrices/SparseMatrixVectorProductTest.hpp,
 // Documentation/Examples/Solvers/Linear/IterativeLinearSolverExample.cpp).
+// The reference's own headers come first: the device is added beside them,
+// and the solver example is one template instantiated for Devices::Host (the
+// reference's code) and for Devices::MI355X (this library).
 // Prints "ALL OK" and exits 0 when every check passes.
 #include <sstream>
 #include <unistd.h>
+
+#include <TNL/Containers/Vector.h>
+#include <TNL/Matrices/SparseMatrix.h>
+#include <TNL/Solvers/Linear/CG.h>
+#include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
 
 #include <TNL/MI355X.h>
 
@@ -17,6 +25,17 @@ using namespace TNL;
 using Device = Devices::MI355X;
 using Vector = Containers::Vector< double, Device, int >;
 namespace Seg = Algorithms::Segments;
+template< typename D, typename I, typename A >
+using SlicedEllpack64 = Seg::SlicedEllpack< D, I, A, Seg::ColumnMajorOrder, 64 >;
+
+// the reference's full parameter lists, allocators spelled out, name the
+// same types as its defaults
+static_assert( std::is_same_v< Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Seg::CSR, double,
+                                                      Allocators::MI355X< double >, Allocators::MI355X< int > >,
+                               Matrices::SparseMatrix< double, Device > > );
+static_assert( std::is_same_v< Containers::Vector< double, Device, int, Allocators::MI355X< double > >, Vector > );
+static_assert( std::is_base_of_v< Solvers::Linear::LinearSolver< Matrices::SparseMatrix< double, Device > >,
+                                  Solvers::Linear::CG< Matrices::SparseMatrix< double, Device > > > );
 
 static int failures = 0;
 #define EXPECT_EQ( a, b )                                                                                            \
@@ -28,13 +47,16 @@ static int failures = 0;
    } while( 0 )
 
 template< template< class, class, class > class Segments >
-void vectorProductKATs( const char* name )
+void vectorProductKATs( const char* name, bool csr = false )
 {
    using Matrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Segments >;
    std::vector< Seg::LaunchConfiguration > configs{ Seg::LaunchConfiguration() };
-   if( std::is_same_v< Segments< Device, int, void >, Seg::CSR< Device, int, void > > )
-      for( int t : { 1, 2, 4, 8, 16, 32, 64, 128, 256 } )
+   if( csr ) {
+      for( int t : { 2, 4, 8, 16, 32, 64, 128, 256 } )
          configs.emplace_back( Seg::ThreadsToSegmentsMapping::Fixed, t );
+      configs.emplace_back( Seg::ThreadsToSegmentsMapping::Warp, 1 );
+      configs.emplace_back( Seg::ThreadsToSegmentsMapping::DynamicGrouping, 1 );
+   }
    for( const auto& cfg : configs ) {
       // zeroMatrix: capacities [1, 2, 1, 1], no entries
       {
@@ -115,10 +137,13 @@ void vectorProductKATs( const char* name )
    std::cout << name << " KATs done\n";
 }
 
+// The solver example written once for any device: the reference's code for
+// Devices::Host, this library for Devices::MI355X.  Returns the iterations.
 template< typename Solver, typename Matrix >
-void solveExample( const std::shared_ptr< Matrix >& A, const char* name, bool diag = false )
+int solveExample( const std::shared_ptr< Matrix >& A, const char* name, bool diag = false )
 {
-   Vector x( 5, 1.0 ), b( 5 );
+   using V = Containers::Vector< double, typename Matrix::DeviceType, int >;
+   V x( 5, 1.0 ), b( 5 );
    A->vectorProduct( x, b );
    x = 0.0;
    Solver s;
@@ -131,20 +156,34 @@ void solveExample( const std::shared_ptr< Matrix >& A, const char* name, bool di
    s.setConvergenceResidue( 1e-12 );
    const bool ok = s.solve( b, x );
    EXPECT_EQ( ok, true );
-   const auto h = x.toHost();
-   for( double v : h )
-      EXPECT_EQ( std::fabs( v - 1.0 ) < 1e-10, true );
+   for( int i = 0; i < 5; i++ )
+      EXPECT_EQ( std::fabs( x.getElement( i ) - 1.0 ) < 1e-10, true );
    EXPECT_EQ( Solvers::Linear::LinearResidueGetter::getResidue( *A, x, b ) < 1e-12, true );
-   std::cout << name << ": " << s.getIterations() << " iterations, residue " << s.getResidue() << '\n';
+   const char* device = std::is_same_v< typename Matrix::DeviceType, Devices::Host > ? "Host" : "MI355X";
+   std::cout << name << " (" << device << "): " << s.getIterations()
+             << " iterations, residue " << s.getResidue() << '\n';
+   return s.getIterations();
+}
+
+// IterativeLinearSolverExample.cpp's matrix, tridiag(-1, 2.5, -1), on any device
+template< typename D >
+auto exampleMatrix()
+{
+   using M = Matrices::SparseMatrix< double, D, int >;
+   return std::make_shared< M >( 5, 5,
+                                 std::initializer_list< std::tuple< int, int, double > >{
+                                    { 0, 0, 2.5 }, { 0, 1, -1 }, { 1, 0, -1 }, { 1, 1, 2.5 }, { 1, 2, -1 },
+                                    { 2, 1, -1 },  { 2, 2, 2.5 }, { 2, 3, -1 }, { 3, 2, -1 }, { 3, 3, 2.5 },
+                                    { 3, 4, -1 },  { 4, 3, -1 },  { 4, 4, 2.5 } } );
 }
 
 int main()
 {
-   vectorProductKATs< Seg::CSR >( "CSR" );
+   vectorProductKATs< Seg::CSR >( "CSR", true );
    vectorProductKATs< Seg::AdaptiveCSR >( "AdaptiveCSR" );
    vectorProductKATs< Seg::Ellpack >( "Ellpack" );
    vectorProductKATs< Seg::SlicedEllpack >( "SlicedEllpack" );
-   vectorProductKATs< Seg::SlicedEllpack64 >( "SlicedEllpack<64>" );
+   vectorProductKATs< SlicedEllpack64 >( "SlicedEllpack<64>" );
    vectorProductKATs< Seg::BiEllpack >( "BiEllpack" );
    vectorProductKATs< Seg::ChunkedEllpack >( "ChunkedEllpack" );
    vectorProductKATs< Seg::SortedCSR >( "SortedCSR" );
@@ -215,6 +254,31 @@ int main()
                                            { 0, 0, 2.5 },  { 0, 1, -1 },  { 1, 0, -1 },  { 1, 1, 2.5 }, { 1, 2, -1 },
                                            { 2, 1, -1 },   { 2, 2, 2.5 }, { 2, 3, -1 },  { 3, 2, -1 },  { 3, 3, 2.5 },
                                            { 3, 4, -1 },   { 4, 3, -1 },  { 4, 4, 2.5 } } );
+   // the same code on the reference's host device and on this one: the same
+   // iterations
+   {
+      using HostMatrix = Matrices::SparseMatrix< double, Devices::Host, int >;
+      auto H = exampleMatrix< Devices::Host >();
+      auto G = exampleMatrix< Device >();
+      EXPECT_EQ( solveExample< Solvers::Linear::CG< HostMatrix > >( H, "CG" ),
+                 solveExample< Solvers::Linear::CG< Matrix > >( G, "CG" ) );
+      EXPECT_EQ( solveExample< Solvers::Linear::CG< HostMatrix > >( H, "CG + Diagonal", true ),
+                 solveExample< Solvers::Linear::CG< Matrix > >( G, "CG + Diagonal", true ) );
+      // cross-device assignment of the reference's host matrix and vector
+      Matrix D;
+      D = *H;
+      Containers::Vector< double, Devices::Host, int > hx( 5 );
+      for( int i = 0; i < 5; i++ )
+         hx[ i ] = 1.0 + i;
+      Vector dx, y1( 5 ), y2( 5 );
+      dx = hx;
+      D.vectorProduct( dx, y1 );
+      G->vectorProduct( dx, y2 );
+      EXPECT_EQ( y1 == y2, true );
+      EXPECT_EQ( y1.getElement( 4 ), 2.5 * 5 - 4 );
+      EXPECT_EQ( ( dx, dx ), 55.0 );
+      EXPECT_EQ( Containers::l2Norm( dx - dx ), 0.0 );
+   }
    solveExample< Solvers::Linear::CG< Matrix > >( A, "CG" );
    solveExample< Solvers::Linear::CG< Matrix > >( A, "CG + Diagonal", true );
    // GMRES with the class default orthogonalisation (CWY, GMRES.h:212)

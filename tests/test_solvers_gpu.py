@@ -294,6 +294,46 @@ def test_reliable_mixed_cg(orc, vt, fmt, param):
     assert 2 <= s.getUpdates() <= 40
 
 
+@pytest.mark.parametrize("fmt,param", [(tnl.CSR, 0), (tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32)])
+@pytest.mark.parametrize("pre", [False, True], ids=["none", "jacobi"])
+def test_true_residual_cg(orc, fmt, param, pre):
+    # CG's iteration stopped on ||b - A x|| / ||b|| (the mixed solvers' rule):
+    # the true residual meets the tolerance, x is the oracle's CG solution,
+    # and it stops no later than an oracle CG run to the same true residual
+    # needs (the oracle stops on the preconditioned residue, pinned to TNL's
+    # CG; its iterate count at a tolerance tight enough to reach 1e-10 true
+    # bounds this one); 8-bit column codes give the identical solve
+    off, cols, vals = tnl.poisson3d(20, 18, 16)
+    n = len(off) - 1
+    b = np.random.default_rng(31).standard_normal(n)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt, param=param)
+    out = []
+    for codes in (False, True):
+        s = S.TrueResidualCG(column_codes=codes)
+        s.setMatrix(A)
+        if pre:
+            s.setPreconditioner(S.Diagonal(A))
+        s.setConvergenceResidue(1e-10)
+        s.setMaxIterations(5000)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        assert s.solve(dev(b), x)
+        xs = x.cpu().numpy()
+        r = orc.true_residual(off, cols, vals, xs, b)
+        assert r <= 1e-10 and abs(r - s.getResidue()) <= 1e-13
+        out.append((xs, s.getIterations()))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+    xo, so = orc.cg(off, cols, vals, b, precond=1 if pre else 0, tol=1e-12)
+    np.testing.assert_allclose(out[0][0], xo, rtol=1e-7, atol=1e-9)
+    assert out[0][1] <= so.iterations
+    # the recurrence residual stops the solve: a looser tolerance stops earlier
+    s = S.TrueResidualCG()
+    s.setMatrix(A)
+    s.setConvergenceResidue(1e-4)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    assert s.getIterations() < out[0][1] and orc.true_residual(off, cols, vals, x.cpu().numpy(), b) <= 1e-4
+
+
 @pytest.mark.parametrize("fmt,param", [(tnl.ELLPACK, 0), (tnl.SLICED_ELLPACK, 32)])
 def test_mixed_cg_column_codes_identical(monkeypatch, fmt, param):
     # the inner SpMV of an Ellpack / SlicedEllpack stencil matrix reads 8-bit

@@ -220,6 +220,45 @@ def test_powerlaw_full_size_fp32(orc, powerlaw):
         torch.cuda.empty_cache()
 
 
+def test_auto_plan_follows_the_matrix(orc):
+    # 2^20 rows and columns with 4 columns per row spread over the whole
+    # width: the automatic launch runs the windowed plan the library builds
+    # and keeps (the explicit plan's result bit for bit, the oracle's within
+    # 1e-12); a changed element reaches the next product (the library drops
+    # its plan), and so does a direct write announced by modified()
+    n = 1 << 20
+    rng = np.random.default_rng(5)
+    q = n // 4
+    cols = (np.arange(4, dtype=np.int64) * q + rng.integers(0, q, (n, 4))).astype(np.int32).reshape(-1)
+    off = np.arange(0, 4 * n + 1, 4, dtype=np.int32)
+    vals = rng.standard_normal(4 * n)
+    x = tnl.uniform(n, -1.0, 1.0, seed=3)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    y_auto = run(A, x)
+    P = tnl.SpmvPlan(A, 0)
+    y_plan = torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.vectorProduct(gpu(x), y_plan)
+    torch.cuda.synchronize()
+    assert np.array_equal(y_auto, y_plan.cpu().numpy())
+    assert relerr(y_auto, orc.spmv(off, cols, vals, x)) <= 1e-12
+    assert np.array_equal(run(A, x), y_auto)  # the kept plan again
+    vals2 = vals.copy()
+    vals2[28] = 100.0
+    A.setElement(7, int(cols[28]), 100.0)
+    y2 = run(A, x)
+    assert relerr(y2, orc.spmv(off, cols, vals2, x)) <= 1e-12
+    A.values.mul_(2.0)
+    A.modified()
+    assert relerr(run(A, x), 2.0 * orc.spmv(off, cols, vals2, x)) <= 1e-12
+    # FP32 values and vectors take the plan too; a launch configuration of the
+    # reference runs its own kernel (scalar: rows summed left to right)
+    A32 = tnl.SparseMatrix.from_csr(n, n, off, cols, vals2, value_dtype=torch.float32)
+    y32 = run(A32, x.astype(np.float32), dtype=torch.float32)
+    want32 = orc.spmv(off, cols, vals2.astype(np.float32).astype(np.float64), x.astype(np.float32))
+    assert relerr(y32, want32) <= 1e-6
+    assert relerr(run(A, x, launch=(tnl.FIXED, 1)), 2.0 * orc.spmv(off, cols, vals2, x)) <= 1e-12
+
+
 def test_errors():
     off, cols, vals = tnl.laplace2d(8, 8)
     A = tnl.SparseMatrix.from_csr(64, 64, off, cols, vals)

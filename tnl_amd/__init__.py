@@ -179,6 +179,15 @@ class SparseMatrix:
         self.symmetric = bool(symmetric)
         self._m = _lib.Matrix(self.rows, self.columns, segments.handle.value, dtype_code(values.dtype),
                               values.data_ptr(), column_indexes.data_ptr(), int(self.symmetric))
+        # arrays handed in by the caller: no product plan of earlier arrays at
+        # these addresses may serve them
+        self.modified()
+
+    def modified(self):
+        """Drops what the library derived from the value and column-index
+        arrays (the automatic product plan of vectorProduct); call after
+        writing those tensors directly (tnlmi_matrix_modified)."""
+        _lib.check(_lib.lib().tnlmi_matrix_modified(ctypes.byref(self._m)))
 
     @classmethod
     def from_csr(cls, rows, columns, offsets, cols, vals, fmt=CSR, value_dtype=None, organization=COLUMN_MAJOR,

@@ -139,6 +139,8 @@ _PROTOS = {
     "tnlmi_preconditioner_solve": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "tnlmi_cg_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(SolverState),
                                       c_vp]),
+    "tnlmi_cg_solve_true_residual": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int32, c_vp, c_vp,
+                                                    ctypes.POINTER(SolverState), c_vp]),
     "tnlmi_gmres_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int, ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_vp, c_vp,
                                          ctypes.POINTER(SolverState), c_vp]),
@@ -174,6 +176,7 @@ _PROTOS = {
     "tnlmi_batched_cg_solve": (ctypes.c_int, [c_vp, ctypes.c_int32, c_vp, c_vp, ctypes.c_double, c_i64,
                                               ctypes.c_double, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "tnlmi_matrix_row_lengths": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp]),
+    "tnlmi_matrix_modified": (ctypes.c_int, [ctypes.POINTER(Matrix)]),
     "tnlmi_matrix_copy": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp]),
     "tnlmi_matrix_add_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.c_double, ctypes.c_double, c_vp]),

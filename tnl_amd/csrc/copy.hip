@@ -8,6 +8,7 @@
 #include <algorithm>
 
 #include "segments.hpp"
+#include "spmv.hpp"
 
 namespace tnlmi {
 namespace {
@@ -126,6 +127,7 @@ tnlmi_status tnlmi_matrix_copy( const tnlmi_matrix* src, tnlmi_matrix* dst, cons
       if( dst->segments->storage > 0 && ( ! dst->values || ! dst->column_indexes ) )
          fail( TNLMI_INVALID_ARGUMENT, "matrix_copy: the target's storage arrays are NULL" );
       hipStream_t s = as_stream( stream );
+      drop_auto_plans( *dst->segments, dst->values, dst->column_indexes );
       switch( src->value_type ) {
          case TNLMI_F64:
             copy_typed< double >( *src, *dst, d_lengths, s );

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "segments.hpp"
+#include "spmv.hpp"
 
 namespace tnlmi {
 namespace {
@@ -173,6 +174,7 @@ tnlmi_status tnlmi_matrix_add_element( tnlmi_matrix* A, int64_t row, int64_t col
       if( ! check_element( A, row, column, "addElement" ) )
          fail( TNLMI_INVALID_ARGUMENT, "addElement: the transposed position is out of the symmetric matrix" );
       hipStream_t s = as_stream( stream );
+      drop_auto_plans( *A->segments, A->values, A->column_indexes );
       DeviceArray< int32_t > status;
       status.allocate( 1 );
       const SegView v = make_view( *A->segments );

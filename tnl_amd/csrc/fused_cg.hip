@@ -64,6 +64,7 @@ struct StopRule
 {
    int64_t maxit, minit;
    double tol, div;
+   int32_t kind = 0;  // residue: 0 TNL's sqrt((r, z)) / ||M^-1 b||, 1 ||r|| / ||b|| (the recurrence residual)
 };
 
 __device__ __forceinline__ bool check_next( const StopRule& R, double res, int64_t it )
@@ -454,8 +455,8 @@ __global__ void __launch_bounds__( kFT ) cg_upd( int64_t n, const double* __rest
 // (IterativeSolver.hpp:98-121; nextIteration() counts every check).
 template< typename T, typename D >
 __global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __restrict__ part1,
-                                                 const double* __restrict__ part2, const T* __restrict__ r,
-                                                 const D* __restrict__ d, T* __restrict__ p,
+                                                 const double* __restrict__ part2, const double* __restrict__ part3,
+                                                 const T* __restrict__ r, const D* __restrict__ d, T* __restrict__ p,
                                                  const CgDev* __restrict__ cur, CgDev* __restrict__ nxt, StopRule R )
 {
    __shared__ double sm[ kFT / 64 ];
@@ -478,7 +479,7 @@ __global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __rest
    }
    const double s1 = cur->s1;
    const double beta = s1 == 0.0 ? 0.0 : s1n / s1;
-   const double res = sqrt( s1n ) / cur->normb;
+   const double res = sqrt( R.kind == 1 ? sum_parts( part3, sm ) : s1n ) / cur->normb;
    const bool go = check_next( R, res, cur->it );
    if( lead ) {
       *nxt = *cur;
@@ -554,11 +555,11 @@ __global__ void __launch_bounds__( kFT ) mx_dir( int64_t n, const double* __rest
 }
 
 // ---- set-up kernels
-__global__ void cg_start( CgDev* st, const double* s1, const double* normb, StopRule R )
+__global__ void cg_start( CgDev* st, const double* s1, const double* normb, const double* rr, StopRule R )
 {
    st->s1 = s1[ 0 ];
    st->normb = normb[ 0 ];
-   st->res = sqrt( s1[ 0 ] ) / normb[ 0 ];
+   st->res = sqrt( R.kind == 1 ? rr[ 0 ] : s1[ 0 ] ) / normb[ 0 ];
    st->beta = 0;
    st->s2zero = 0;
    st->want_update = 0;
@@ -859,7 +860,7 @@ struct HostState
 }  // namespace
 
 void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x,
-                     tnlmi_solver_state* st, hipStream_t s )
+                     tnlmi_solver_state* st, hipStream_t s, int residue_kind, bool column_codes )
 {
    const int64_t n = A.rows;
    const double* d = precond_diagonal( M );
@@ -873,9 +874,10 @@ void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const dou
    Parts parts;
    RedWork w;
    HostState hs;
-   const StopRule R{ st->max_iterations, st->min_iterations, st->convergence_residue, st->divergence_residue };
-   // normb = ||M^{-1} b|| (CG.hpp:24-29)
-   if( d ) {
+   const StopRule R{ st->max_iterations, st->min_iterations, st->convergence_residue, st->divergence_residue,
+                     residue_kind };
+   // normb = ||M^{-1} b|| (CG.hpp:24-29); ||b|| for the recurrence residual
+   if( d && residue_kind == 0 ) {
       precond_apply( M, n, b, r.ptr, s );
       nrm2( n, r.ptr, sc.ptr + 1, w, s );
    }
@@ -896,9 +898,11 @@ void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const dou
    rsub( n, b, r.ptr, s );
    precond_apply( M, n, r.ptr, p.ptr, s );
    dot( n, r.ptr, p.ptr, sc.ptr + 0, w, s );
-   hipLaunchKernelGGL( cg_start, dim3( 1 ), dim3( 1 ), 0, s, dev.ptr, sc.ptr + 0, sc.ptr + 1, R );
+   if( residue_kind == 1 )
+      dot( n, r.ptr, r.ptr, sc.ptr + 2, w, s );
+   hipLaunchKernelGGL( cg_start, dim3( 1 ), dim3( 1 ), 0, s, dev.ptr, sc.ptr + 0, sc.ptr + 1, sc.ptr + 2, R );
    TNLMI_HIP( hipGetLastError() );
-   const Stage S = make_stage( A.A, A.plan, s );
+   const Stage S = make_stage( A.A, A.plan, s, column_codes );
    // the state is double-buffered by iteration parity: iteration k reads
    // dev[k & 1] and its last kernel writes dev[(k + 1) & 1]
    int par = 0;
@@ -907,10 +911,11 @@ void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const dou
       for( int k = 0; k < kBatch; k++, par ^= 1 ) {
          CgDev* cur = dev.ptr + par;
          stage_launch< double >( S, p.ptr, Ap.ptr, parts.a.ptr, cur, s );
+         double* part3 = residue_kind == 1 ? parts.c.ptr : nullptr;
          hipLaunchKernelGGL( ( cg_upd< double, double, double > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
-                             r.ptr, p.ptr, Ap.ptr, x, d, parts.b.ptr, (double*) nullptr, cur );
+                             r.ptr, p.ptr, Ap.ptr, x, d, parts.b.ptr, part3, cur );
          hipLaunchKernelGGL( ( cg_dir< double, double > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
-                             parts.b.ptr, r.ptr, d, p.ptr, cur, dev.ptr + ( par ^ 1 ), R );
+                             parts.b.ptr, part3, r.ptr, d, p.ptr, cur, dev.ptr + ( par ^ 1 ), R );
       }
       TNLMI_HIP( hipGetLastError() );
       h = &hs.read( dev.ptr + par, s );

@@ -1098,6 +1098,44 @@ tnlmi_status tnlmi_cg_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan,
    } );
 }
 
+tnlmi_status tnlmi_cg_solve_true_residual( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan,
+                                           const tnlmi_preconditioner* M, int32_t column_codes, const double* d_b,
+                                           double* d_x, tnlmi_solver_state* st, tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      check_state( st );
+      if( M && precond_kind( M ) != TNLMI_PRECOND_DIAGONAL )
+         fail( TNLMI_NOT_IMPLEMENTED, "cg_solve_true_residual: no or the Diagonal preconditioner" );
+      const Operator op = make_op( A, plan );
+      hipStream_t s = as_stream( stream );
+      tnlmi_solver_state total = *st;
+      total.iterations = 0;
+      while( true ) {
+         tnlmi_solver_state part = *st;
+         part.max_iterations = st->max_iterations - total.iterations;
+         fused_cg_solve( op, M, d_b, d_x, &part, s, 1, column_codes != 0 );
+         TNLMI_HIP( hipGetLastError() );
+         total.iterations += part.iterations;
+         total.residue = part.residue;
+         total.converged = part.converged;
+         if( ! part.converged )
+            break;
+         // the recurrence residual met the tolerance: confirm on b - A x in
+         // FP64; a drift above it continues from x (r recomputed)
+         double res = 0;
+         const tnlmi_status rc = tnlmi_true_residual( A, d_x, d_b, &res, stream );
+         if( rc != TNLMI_OK )
+            fail( rc, tnlmi_last_error() );
+         total.residue = res;
+         if( res <= st->convergence_residue || total.iterations >= st->max_iterations || part.iterations == 0 ) {
+            total.converged = res <= st->convergence_residue ? 1 : 0;
+            break;
+         }
+      }
+      *st = total;
+   } );
+}
+
 tnlmi_status tnlmi_gmres_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, const tnlmi_preconditioner* M,
                                 tnlmi_gmres_variant variant, int32_t restarting_min, int32_t restarting_max,
                                 int32_t step_min, int32_t step_max, const double* d_b, double* d_x,

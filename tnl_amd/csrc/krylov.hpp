@@ -81,7 +81,11 @@ struct HostScalars
 };
 
 // CG with fused kernels for no or Diagonal preconditioning (fused_cg.hip).
+// residue_kind 0: TNL's residue sqrt((r, z)) / ||M^-1 b||; 1: the recurrence
+// residual ||r|| / ||b|| (the true-residual stopping rule of the mixed solvers).
+// column_codes: 8-bit column codes for a column-major Ellpack / SlicedEllpack
+// matrix of few distinct column offsets (as the mixed solver's inner matrix).
 void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x,
-                     tnlmi_solver_state* st, hipStream_t s );
+                     tnlmi_solver_state* st, hipStream_t s, int residue_kind = 0, bool column_codes = false );
 
 }  // namespace tnlmi

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "segments.hpp"
+#include "spmv.hpp"
 
 namespace tnlmi {
 
@@ -1054,6 +1055,7 @@ tnlmi_status tnlmi_matrix_fill( const tnlmi_segments* seg, const int32_t* d_csr_
       if( seg->n && ! d_csr_offsets )
          fail( TNLMI_INVALID_ARGUMENT, "matrix_fill: CSR offsets are NULL" );
       hipStream_t s = as_stream( stream );
+      drop_auto_plans( *seg, d_values, d_column_indexes );
       if( csr_type == TNLMI_F64 && value_type == TNLMI_F64 )
          fill_typed< double, double >( *seg, d_csr_offsets, d_csr_columns, d_csr_values, d_values, d_column_indexes, s );
       else if( csr_type == TNLMI_F64 && value_type == TNLMI_F32 )

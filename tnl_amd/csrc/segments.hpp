@@ -2,9 +2,14 @@
 // Segments) and their global-index functions.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
+
+namespace tnlmi {
+struct AutoPlans;
+}
 
 // The opaque handle of the C ABI.  Every format also keeps the CSR offsets of
 // its sizes (used only by assembly and the CSR-ordered index export, never by
@@ -62,6 +67,11 @@ struct tnlmi_segments
    // `elements` is the sum of the sizes as getSegmentSize derives them), written
    // back by save.  -1 otherwise.
    int64_t file_size = -1;
+
+   // Product plans tnlmi_spmv built by itself for matrices on these segments
+   // (keyed by their value and column-index arrays): dropped when the library
+   // writes such a matrix, or on tnlmi_matrix_modified.
+   mutable std::shared_ptr< tnlmi::AutoPlans > auto_plans;
 };
 
 namespace tnlmi {

@@ -726,6 +726,12 @@ extern "C" tnlmi_status tnlmi_spmv( const tnlmi_matrix* A, tnlmi_dtype x_type, c
          cfg = *launch;
       if( cfg.threads_per_segment <= 0 || ( cfg.threads_per_segment & ( cfg.threads_per_segment - 1 ) ) )
          fail( TNLMI_INVALID_ARGUMENT, "Threads per segment count must be a positive power of two." );
-      spmv_dispatch( *A, x_type, d_x, d_y, compute_type, alpha, beta, begin, end, cfg, as_stream( stream ) );
+      hipStream_t s = as_stream( stream );
+      if( cfg.mapping == TNLMI_MAP_AUTO )
+         if( const tnlmi_spmv_plan* P = auto_plan( *A, x_type, compute_type, s ) ) {
+            plan_execute( *P, x_type, d_x, d_y, compute_type, alpha, beta, begin, end, s );
+            return;
+         }
+      spmv_dispatch( *A, x_type, d_x, d_y, compute_type, alpha, beta, begin, end, cfg, s );
    } );
 }

@@ -15,7 +15,9 @@
 // No reference counterpart: TNL's vectorProduct has no analysis phase.  The
 // plain CSR path (spmv.hip) reads the CSR arrays themselves.
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "segments.hpp"
@@ -368,5 +370,147 @@ extern "C" tnlmi_status tnlmi_spmv_plan_execute( const tnlmi_spmv_plan* P, tnlmi
       if( ( P->columns && ! d_x ) || ( P->rows && ! d_y ) )
          fail( TNLMI_INVALID_ARGUMENT, "vectorProduct: vector pointer is NULL" );
       plan_execute( *P, x_type, d_x, d_y, compute_type, alpha, beta, begin, end, as_stream( stream ) );
+   } );
+}
+
+namespace tnlmi {
+
+// The automatic plans of one segments object.
+struct AutoPlans
+{
+   struct Entry
+   {
+      const void* values;
+      const int32_t* cols;
+      int32_t value_type;
+      int64_t rows, columns;
+      tnlmi_spmv_plan* plan;  // NULL: A's own kernels were chosen
+      uint64_t last_use;
+   };
+   std::mutex m;
+   std::vector< Entry > entries;
+   uint64_t clock = 0;
+   ~AutoPlans()
+   {
+      for( auto& e : entries )
+         tnlmi_spmv_plan_destroy( e.plan );
+   }
+};
+
+namespace {
+
+constexpr int64_t kAutoMinColumns = int64_t( 1 ) << 20;   // x (FP32) of 4 MiB: wider than an XCD's L2
+constexpr int64_t kAutoMinElements = int64_t( 1 ) << 22;
+constexpr int64_t kAutoSampleRows = 4096;
+constexpr int64_t kAutoFar = 65536;                        // columns from the row's diagonal position
+constexpr std::size_t kAutoEntries = 4;
+
+// stored entries of evenly spaced sample rows, and those whose column lies
+// more than kAutoFar from the row's position scaled to the columns
+__global__ void column_spread( const int32_t* __restrict__ off, const int32_t* __restrict__ cols, int64_t rows,
+                               int64_t columns, int64_t samples, unsigned long long* counts )
+{
+   const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( i >= samples )
+      return;
+   const int64_t r = i * rows / samples;
+   const double diag = (double) r * (double) columns / (double) rows;
+   unsigned long long total = 0, far = 0;
+   for( int32_t g = off[ r ]; g < off[ r + 1 ]; g++ ) {
+      const int32_t c = cols[ g ];
+      if( c < 0 )
+         continue;
+      total++;
+      far += fabs( (double) c - diag ) > (double) kAutoFar;
+   }
+   atomicAdd( counts, total );
+   atomicAdd( counts + 1, far );
+}
+
+bool columns_spread( const tnlmi_matrix& A, hipStream_t s )
+{
+   const int64_t samples = std::min< int64_t >( A.rows, kAutoSampleRows );
+   DeviceArray< unsigned long long > counts;
+   counts.allocate( 2 );
+   TNLMI_HIP( hipMemsetAsync( counts.ptr, 0, 2 * sizeof( unsigned long long ), s ) );
+   hipLaunchKernelGGL( column_spread, dim3( (unsigned) ceil_div( samples, 256 ) ), dim3( 256 ), 0, s,
+                       A.segments->offsets.ptr, A.column_indexes, A.rows, A.columns, samples, counts.ptr );
+   TNLMI_HIP( hipGetLastError() );
+   unsigned long long h[ 2 ] = { 0, 0 };
+   TNLMI_HIP( hipMemcpyAsync( h, counts.ptr, sizeof( h ), hipMemcpyDeviceToHost, s ) );
+   TNLMI_HIP( hipStreamSynchronize( s ) );
+   return h[ 0 ] > 0 && 2 * h[ 1 ] >= h[ 0 ];
+}
+
+}  // namespace
+
+const tnlmi_spmv_plan* auto_plan( const tnlmi_matrix& A, tnlmi_dtype xt, tnlmi_dtype ct, hipStream_t s )
+{
+   const tnlmi_segments& seg = *A.segments;
+   if( seg.format != TNLMI_CSR || A.matrix_type == TNLMI_SYMMETRIC
+       || ( A.value_type != TNLMI_F64 && A.value_type != TNLMI_F32 ) || xt != A.value_type || ct != A.value_type
+       || A.columns < kAutoMinColumns || seg.elements < kAutoMinElements )
+      return nullptr;
+   static const bool disabled = [] {
+      const char* e = std::getenv( "TNLMI_AUTO_PLAN" );
+      return e && e[ 0 ] == '0';
+   }();
+   if( disabled )
+      return nullptr;
+   if( ! seg.auto_plans )
+      seg.auto_plans = std::make_shared< AutoPlans >();
+   AutoPlans& P = *seg.auto_plans;
+   std::lock_guard< std::mutex > lock( P.m );
+   for( auto& e : P.entries )
+      if( e.values == A.values && e.cols == A.column_indexes && e.value_type == A.value_type && e.rows == A.rows
+          && e.columns == A.columns ) {
+         e.last_use = ++P.clock;
+         return e.plan;
+      }
+   std::unique_ptr< tnlmi_spmv_plan > built;
+   if( columns_spread( A, s ) ) {
+      built = std::make_unique< tnlmi_spmv_plan >();
+      built->rows = A.rows;
+      built->columns = A.columns;
+      built->parts = 0;
+      built->value_type = A.value_type;
+      built->window = std::make_unique< WindowPlan >();
+      window_build( *built->window, A, s );
+   }
+   tnlmi_spmv_plan* plan = built.release();
+   if( P.entries.size() >= kAutoEntries ) {
+      auto lru = std::min_element( P.entries.begin(), P.entries.end(),
+                                   []( const auto& a, const auto& b ) { return a.last_use < b.last_use; } );
+      tnlmi_spmv_plan_destroy( lru->plan );
+      P.entries.erase( lru );
+   }
+   P.entries.push_back( { A.values, A.column_indexes, A.value_type, A.rows, A.columns, plan, ++P.clock } );
+   return plan;
+}
+
+void drop_auto_plans( const tnlmi_segments& seg, const void* values, const int32_t* cols )
+{
+   if( ! seg.auto_plans )
+      return;
+   AutoPlans& P = *seg.auto_plans;
+   std::lock_guard< std::mutex > lock( P.m );
+   for( auto it = P.entries.begin(); it != P.entries.end(); ) {
+      if( ( ! values && ! cols ) || it->values == values || it->cols == cols ) {
+         tnlmi_spmv_plan_destroy( it->plan );
+         it = P.entries.erase( it );
+      }
+      else
+         ++it;
+   }
+}
+
+}  // namespace tnlmi
+
+extern "C" tnlmi_status tnlmi_matrix_modified( const tnlmi_matrix* A )
+{
+   return guarded( [ & ] {
+      if( ! A || ! A->segments )
+         fail( TNLMI_INVALID_ARGUMENT, "matrix_modified: NULL argument" );
+      drop_auto_plans( *A->segments, A->values, A->column_indexes );
    } );
 }

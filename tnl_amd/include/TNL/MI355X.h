@@ -1,16 +1,29 @@
-// TNL/MI355X.h -- the MI355X device for TNL's SparseMatrix / Segments /
-// Solvers::Linear API, over the C ABI of libtnlmi.so (include/tnlmi.h).
+// TNL/MI355X.h -- the MI355X device for TNL's own class templates, over the C
+// ABI of libtnlmi.so (include/tnlmi.h).
 //
 // A caller of the reference writes
 //     TNL::Matrices::SparseMatrix< double, TNL::Devices::Host, int > A( ... );
 //     TNL::Solvers::Linear::CG< decltype( A ) > cg;  cg.setMatrix( ptr ); cg.solve( b, x );
-// and switches to the GPU by replacing Devices::Host with Devices::MI355X.
-// This header provides that device's specialisations of the class templates
-// the hot path uses, with the reference's names, argument meaning and
-// exceptions (std::invalid_argument, std::overflow_error, std::runtime_error,
-// std::logic_error, Exceptions::NotImplementedError / BackendRuntimeError).
-// It is self-contained (it does not include the reference); inside a TNL tree
-// the same specialisations are added next to Devices::Host.
+// and moves to the GPU by replacing Devices::Host with Devices::MI355X.  This
+// header includes the reference's headers and adds the device the way the
+// reference adds its own: a new Devices tag, Allocators::Default for it, and
+// partial specialisations of the reference's primary templates for that tag
+// --
+//   Containers::Vector< Real, MI355X, Index, Allocator >, VectorView,
+//   Matrices::SparseMatrix< Real, MI355X, Index, MatrixType, Segments,
+//                           ComputeReal, RealAllocator, IndexAllocator >,
+//   Matrices::MatrixReader< Matrix, MI355X >,
+//   Solvers::Linear::{CG, GMRES, BICGStab, BICGStabL, IDRs, TFQMR, Jacobi}
+//      for those matrices (deriving from the reference's LinearSolver, so
+//      setup / setMatrix / setPreconditioner / setConvergenceResidue /
+//      getIterations / getResidue are the reference's own),
+//   Preconditioners::Diagonal, and ILU0 through the reference's ILU0_impl
+//      device hook,
+// with the reference's names, argument meaning and exceptions.  Any segments
+// template the reference defines (CSR, AdaptiveCSR, Ellpack, SlicedEllpack,
+// BiEllpack, ChunkedEllpack, SortedSegments over CSR) selects its format.
+// The reference's LinearResidueGetter and copySparseMatrix work unchanged on
+// these types (the latter through an overload for two MI355X matrices).
 //
 // Every call is synchronous on return, as the reference's reduceSegments is
 // (Algorithms/Segments/detail/ReducingOperations_CSR.h:327).
@@ -19,6 +32,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
+#include <cstdint>
 #include <initializer_list>
 #include <iostream>
 #include <iterator>
@@ -31,29 +45,53 @@
 #include <tuple>
 #include <type_traits>
 #include <utility>
-#include <variant>
 #include <vector>
+
+#include <TNL/Algorithms/Segments/AdaptiveCSR.h>
+#include <TNL/Algorithms/Segments/BiEllpack.h>
+#include <TNL/Algorithms/Segments/CSR.h>
+#include <TNL/Algorithms/Segments/ChunkedEllpack.h>
+#include <TNL/Algorithms/Segments/Ellpack.h>
+#include <TNL/Algorithms/Segments/LaunchConfiguration.h>
+#include <TNL/Algorithms/Segments/SlicedEllpack.h>
+#include <TNL/Algorithms/Segments/SortedSegments.h>
+#include <TNL/Allocators/Default.h>
+#include <TNL/Config/ParameterContainer.h>
+#include <TNL/Containers/Vector.h>
+#include <TNL/Containers/VectorView.h>
+#include <TNL/Exceptions/BackendBadAlloc.h>
+#include <TNL/Exceptions/BackendRuntimeError.h>
+#include <TNL/Exceptions/NotImplementedError.h>
+#include <TNL/Matrices/MatrixReader.h>
+#include <TNL/Matrices/MatrixType.h>
+#include <TNL/Matrices/SparseMatrix.h>
+#include <TNL/Matrices/SparseOperations.h>
+#include <TNL/Solvers/Linear/BICGStab.h>
+#include <TNL/Solvers/Linear/BICGStabL.h>
+#include <TNL/Solvers/Linear/CG.h>
+#include <TNL/Solvers/Linear/GMRES.h>
+#include <TNL/Solvers/Linear/IDRs.h>
+#include <TNL/Solvers/Linear/Jacobi.h>
+#include <TNL/Solvers/Linear/LinearSolver.h>
+#include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
+#include <TNL/Solvers/Linear/Preconditioners/ILU0.h>
+#include <TNL/Solvers/Linear/TFQMR.h>
+#include <TNL/Solvers/Linear/Utils/LinearResidueGetter.h>
 
 #include "tnlmi.h"
 
 namespace TNL {
 
-namespace Exceptions {
-struct NotImplementedError : std::runtime_error
+// ------------------------------------------------------------------ device
+namespace Devices {
+//! The MI355X GPU (gfx950, wave64), driven through libtnlmi.so.
+struct MI355X
 {
-   using std::runtime_error::runtime_error;
+   static std::string getDeviceType() { return "Devices::MI355X"; }
 };
-struct BackendRuntimeError : std::runtime_error
-{
-   using std::runtime_error::runtime_error;
-};
-struct BackendBadAlloc : std::bad_alloc
-{
-   const char* what() const noexcept override { return "BackendBadAlloc: device memory allocation failed"; }
-};
-}  // namespace Exceptions
+}  // namespace Devices
 
-namespace detail {
+namespace MI355XDetail {
 // status -> the exception the reference throws in that case
 inline void check( tnlmi_status s )
 {
@@ -68,7 +106,7 @@ inline void check( tnlmi_status s )
       case TNLMI_NOT_IMPLEMENTED:
          throw Exceptions::NotImplementedError( m );
       case TNLMI_BACKEND_ERROR:
-         throw Exceptions::BackendRuntimeError( m );
+         throw Exceptions::BackendRuntimeError( Backend::error_t( 1 ), m );
       case TNLMI_BAD_ALLOC:
          throw Exceptions::BackendBadAlloc();
       case TNLMI_LOGIC_ERROR:
@@ -82,42 +120,184 @@ inline void hip( hipError_t e )
    if( e == hipErrorOutOfMemory )
       throw Exceptions::BackendBadAlloc();
    if( e != hipSuccess )
-      throw Exceptions::BackendRuntimeError( hipGetErrorString( e ) );
+      throw Exceptions::BackendRuntimeError( Backend::error_t( e ), hipGetErrorString( e ) );
 }
 template< typename T >
 constexpr tnlmi_dtype dtype()
 {
-   static_assert( std::is_same_v< T, double > || std::is_same_v< T, float >, "FP64 or FP32 values" );
-   return std::is_same_v< T, double > ? TNLMI_F64 : TNLMI_F32;
+   static_assert( std::is_same_v< std::remove_const_t< T >, double > || std::is_same_v< std::remove_const_t< T >, float >,
+                  "Devices::MI355X vectors and matrices hold double or float values" );
+   return std::is_same_v< std::remove_const_t< T >, double > ? TNLMI_F64 : TNLMI_F32;
 }
-}  // namespace detail
+}  // namespace MI355XDetail
 
-namespace Devices {
-//! The MI355X GPU (gfx950, wave64).
+// --------------------------------------------------------------- allocator
+namespace Allocators {
+//! Device memory of the MI355X (hipMalloc / hipFree), the std::allocator
+//! interface of Allocators::Hip.
+template< class T >
 struct MI355X
 {
-   struct LaunchConfiguration
-   {};
-   static std::string getDeviceType() { return "Devices::MI355X"; }
+   using value_type = T;
+   using size_type = std::size_t;
+   using difference_type = std::ptrdiff_t;
+   MI355X() = default;
+   template< class U >
+   MI355X( const MI355X< U >& ) noexcept
+   {}
+   [[nodiscard]] T* allocate( size_type n )
+   {
+      T* p = nullptr;
+      MI355XDetail::hip( hipMalloc( (void**) &p, std::max< size_type >( 1, n ) * sizeof( T ) ) );
+      return p;
+   }
+   void deallocate( T* p, size_type ) noexcept { (void) hipFree( (void*) p ); }
 };
-struct Host;
-}  // namespace Devices
+template< class T1, class T2 >
+bool operator==( const MI355X< T1 >&, const MI355X< T2 >& )
+{
+   return true;
+}
+template< class T1, class T2 >
+bool operator!=( const MI355X< T1 >&, const MI355X< T2 >& )
+{
+   return false;
+}
+
+//! Allocators/Default.h: the default allocator of Devices::MI355X
+template<>
+struct Default< Devices::MI355X >
+{
+   template< typename T >
+   using Allocator = Allocators::MI355X< T >;
+};
+}  // namespace Allocators
 
 // ------------------------------------------------------------------ vectors
 namespace Containers {
 
-template< typename Real, typename Device, typename Index = int >
-class Vector;
+template< typename Real, typename Index, typename Allocator >
+class Vector< Real, Devices::MI355X, Index, Allocator >;
 
-//! Containers::Vector on the device: an owning, contiguous device array.
+//! Containers::VectorView on the MI355X: a non-owning (data, size) pair.
+//! Copy construction binds; assignment copies elements (ArrayView semantics).
 template< typename Real, typename Index >
-class Vector< Real, Devices::MI355X, Index >
+class VectorView< Real, Devices::MI355X, Index >
+{
+   using NonConstReal = std::remove_const_t< Real >;
+
+public:
+   using RealType = Real;
+   using ValueType = Real;
+   using DeviceType = Devices::MI355X;
+   using IndexType = Index;
+   using ViewType = VectorView< Real, Devices::MI355X, Index >;
+   using ConstViewType = VectorView< std::add_const_t< Real >, Devices::MI355X, Index >;
+
+   VectorView() = default;
+   VectorView( Real* data, Index size ) : data_( data ), size_( size ) {}
+   VectorView( const VectorView& ) = default;
+   //! a view of const values from a view of non-const ones
+   template< typename R, std::enable_if_t< std::is_same_v< std::add_const_t< R >, Real > && ! std::is_same_v< R, Real >,
+                                           bool > = true >
+   VectorView( const VectorView< R, Devices::MI355X, Index >& v ) : data_( v.getData() ), size_( v.getSize() )
+   {}
+   //! the view of a vector (Array::getView / getConstView)
+   template< typename Allocator >
+   VectorView( Vector< NonConstReal, Devices::MI355X, Index, Allocator >& v ) : data_( v.getData() ), size_( v.getSize() )
+   {}
+   template< typename Allocator, typename R = Real, std::enable_if_t< std::is_const_v< R >, bool > = true >
+   VectorView( const Vector< NonConstReal, Devices::MI355X, Index, Allocator >& v )
+   : data_( v.getData() ), size_( v.getSize() )
+   {}
+
+   void bind( Real* data, Index size )
+   {
+      data_ = data;
+      size_ = size;
+   }
+   //! element-wise copy from a view of the same size (ArrayView::operator=)
+   VectorView& operator=( const VectorView& v )
+   {
+      copyFrom( v.getData(), v.getSize() );
+      return *this;
+   }
+   template< typename R >
+   VectorView& operator=( const VectorView< R, Devices::MI355X, Index >& v )
+   {
+      copyFrom( v.getData(), v.getSize() );
+      return *this;
+   }
+   VectorView& operator=( NonConstReal value )
+   {
+      setValue( value );
+      return *this;
+   }
+
+   Index getSize() const { return size_; }
+   Real* getData() const { return data_; }
+   ViewType getView() const { return *this; }
+   ConstViewType getConstView() const { return ConstViewType( data_, size_ ); }
+
+   void setValue( NonConstReal value )
+   {
+      std::vector< NonConstReal > h( size_, value );
+      if( size_ )
+         MI355XDetail::hip( hipMemcpy( (void*) data_, h.data(), sizeof( Real ) * size_, hipMemcpyHostToDevice ) );
+   }
+   NonConstReal getElement( Index i ) const
+   {
+      NonConstReal v;
+      MI355XDetail::hip( hipMemcpy( &v, data_ + i, sizeof( Real ), hipMemcpyDeviceToHost ) );
+      return v;
+   }
+   void setElement( Index i, NonConstReal v ) const
+   {
+      MI355XDetail::hip( hipMemcpy( (void*) ( data_ + i ), &v, sizeof( Real ), hipMemcpyHostToDevice ) );
+   }
+   //! the elements on the host (an extension of this device)
+   std::vector< NonConstReal > toHost() const
+   {
+      std::vector< NonConstReal > h( size_ );
+      if( size_ )
+         MI355XDetail::hip( hipMemcpy( h.data(), data_, sizeof( Real ) * size_, hipMemcpyDeviceToHost ) );
+      return h;
+   }
+   template< typename R >
+   bool operator==( const VectorView< R, Devices::MI355X, Index >& o ) const
+   {
+      return toHost() == o.toHost();
+   }
+
+private:
+   void copyFrom( const void* src, Index size )
+   {
+      if( size != size_ )
+         throw std::invalid_argument( "VectorView::operator=: the sizes of the views differ" );
+      if( size_ && src != data_ )
+         MI355XDetail::hip( hipMemcpy( (void*) data_, src, sizeof( Real ) * size_, hipMemcpyDeviceToDevice ) );
+   }
+
+   Real* data_ = nullptr;
+   Index size_ = 0;
+};
+
+//! Containers::Vector on the MI355X: an owning, contiguous device array
+//! allocated through Allocator (Allocators::MI355X by default).
+template< typename Real, typename Index, typename Allocator >
+class Vector< Real, Devices::MI355X, Index, Allocator >
 {
 public:
    using RealType = Real;
    using ValueType = Real;
    using DeviceType = Devices::MI355X;
    using IndexType = Index;
+   using AllocatorType = Allocator;
+   using ViewType = VectorView< Real, Devices::MI355X, Index >;
+   using ConstViewType = VectorView< std::add_const_t< Real >, Devices::MI355X, Index >;
+   template< typename R, typename D = Devices::MI355X, typename I = Index,
+             typename A = typename Allocators::Default< D >::template Allocator< R > >
+   using Self = Vector< R, D, I, A >;
 
    Vector() = default;
    explicit Vector( Index size ) { setSize( size ); }
@@ -136,7 +316,7 @@ public:
       if( this != &o ) {
          setSize( o.size_ );
          if( size_ )
-            detail::hip( hipMemcpy( data_, o.data_, sizeof( Real ) * size_, hipMemcpyDeviceToDevice ) );
+            MI355XDetail::hip( hipMemcpy( data_, o.data_, sizeof( Real ) * size_, hipMemcpyDeviceToDevice ) );
       }
       return *this;
    }
@@ -145,11 +325,30 @@ public:
       swap( o );
       return *this;
    }
+   //! a copy of a view's elements
+   template< typename R >
+   Vector& operator=( const VectorView< R, Devices::MI355X, Index >& v )
+   {
+      setSize( v.getSize() );
+      if( size_ && (const void*) v.getData() != (const void*) data_ )
+         MI355XDetail::hip( hipMemcpy( data_, v.getData(), sizeof( Real ) * size_, hipMemcpyDeviceToDevice ) );
+      return *this;
+   }
+   //! cross-device assignment from the reference's host vector
+   template< typename HostAllocator >
+   Vector& operator=( const Vector< Real, Devices::Host, Index, HostAllocator >& h )
+   {
+      setSize( h.getSize() );
+      if( size_ )
+         MI355XDetail::hip( hipMemcpy( data_, h.getData(), sizeof( Real ) * size_, hipMemcpyHostToDevice ) );
+      return *this;
+   }
+   //! from host elements (an extension of this device)
    Vector& operator=( const std::vector< Real >& h )
    {
       setSize( (Index) h.size() );
       if( size_ )
-         detail::hip( hipMemcpy( data_, h.data(), sizeof( Real ) * size_, hipMemcpyHostToDevice ) );
+         MI355XDetail::hip( hipMemcpy( data_, h.data(), sizeof( Real ) * size_, hipMemcpyHostToDevice ) );
       return *this;
    }
    Vector& operator=( Real value )
@@ -161,12 +360,12 @@ public:
    void setSize( Index size )
    {
       if( size < 0 )
-         throw std::invalid_argument( "Vector size must be non-negative" );
+         throw std::invalid_argument( "Array size must be non-negative." );
       if( size == size_ )
          return;
       reset();
       if( size > 0 )
-         detail::hip( hipMalloc( (void**) &data_, sizeof( Real ) * size ) );
+         data_ = Allocator().allocate( (std::size_t) size );
       size_ = size;
    }
    template< typename V >
@@ -177,35 +376,29 @@ public:
    void reset()
    {
       if( data_ )
-         (void) hipFree( data_ );
+         Allocator().deallocate( data_, (std::size_t) size_ );
       data_ = nullptr;
       size_ = 0;
    }
    Index getSize() const { return size_; }
    Real* getData() { return data_; }
    const Real* getData() const { return data_; }
+   ViewType getView() { return ViewType( data_, size_ ); }
+   ConstViewType getView() const { return ConstViewType( data_, size_ ); }
+   ConstViewType getConstView() const { return ConstViewType( data_, size_ ); }
+   operator ViewType() { return getView(); }
+   operator ConstViewType() const { return getConstView(); }
 
-   void setValue( Real value )
+   void setValue( Real value ) { getView().setValue( value ); }
+   Real getElement( Index i ) const { return getConstView().getElement( i ); }
+   void setElement( Index i, Real v ) { getView().setElement( i, v ); }
+   //! the elements on the host (an extension of this device)
+   std::vector< Real > toHost() const { return getConstView().toHost(); }
+   template< typename A >
+   bool operator==( const Vector< Real, Devices::MI355X, Index, A >& o ) const
    {
-      std::vector< Real > h( size_, value );
-      if( size_ )
-         detail::hip( hipMemcpy( data_, h.data(), sizeof( Real ) * size_, hipMemcpyHostToDevice ) );
+      return toHost() == o.toHost();
    }
-   Real getElement( Index i ) const
-   {
-      Real v;
-      detail::hip( hipMemcpy( &v, data_ + i, sizeof( Real ), hipMemcpyDeviceToHost ) );
-      return v;
-   }
-   void setElement( Index i, Real v ) { detail::hip( hipMemcpy( data_ + i, &v, sizeof( Real ), hipMemcpyHostToDevice ) ); }
-   std::vector< Real > toHost() const
-   {
-      std::vector< Real > h( size_ );
-      if( size_ )
-         detail::hip( hipMemcpy( h.data(), data_, sizeof( Real ) * size_, hipMemcpyDeviceToHost ) );
-      return h;
-   }
-   bool operator==( const Vector& o ) const { return toHost() == o.toHost(); }
    void swap( Vector& o ) noexcept
    {
       std::swap( data_, o.data_ );
@@ -217,220 +410,299 @@ private:
    Index size_ = 0;
 };
 
+// The reference's expression templates evaluate through Algorithms on the
+// vector's device; this device provides its operations below instead.
+namespace Expressions {
+template< typename Real, typename Index, typename Allocator >
+struct HasEnabledExpressionTemplates< Vector< Real, Devices::MI355X, Index, Allocator > > : std::false_type
+{};
 template< typename Real, typename Index >
-std::ostream& operator<<( std::ostream& s, const Vector< Real, Devices::MI355X, Index >& v )
+struct HasEnabledExpressionTemplates< VectorView< Real, Devices::MI355X, Index > > : std::false_type
+{};
+}  // namespace Expressions
+
+template< typename Real, typename Index >
+std::ostream& operator<<( std::ostream& s, const VectorView< Real, Devices::MI355X, Index >& v )
 {
    const auto h = v.toHost();
    s << "[ ";
-   for( size_t i = 0; i < h.size(); i++ )
+   for( std::size_t i = 0; i < h.size(); i++ )
       s << ( i ? ", " : "" ) << h[ i ];
    return s << " ]";
 }
+template< typename Real, typename Index, typename Allocator >
+std::ostream& operator<<( std::ostream& s, const Vector< Real, Devices::MI355X, Index, Allocator >& v )
+{
+   return s << v.getConstView();
+}
+
+namespace detail_mi355x {
+// a library reduction into one device double, read back
+template< typename F >
+double device_scalar( F&& f )
+{
+   Vector< double, Devices::MI355X, int > d( 1 );
+   MI355XDetail::check( f( d.getData() ) );
+   return d.getElement( 0 );
+}
+template< typename Index >
+double dot( VectorView< const double, Devices::MI355X, Index > a, VectorView< const double, Devices::MI355X, Index > b )
+{
+   if( a.getSize() != b.getSize() )
+      throw std::invalid_argument( "the vectors of a scalar product differ in size" );
+   return device_scalar( [ & ]( double* d ) { return tnlmi_dot( a.getSize(), a.getData(), b.getData(), d, nullptr ); } );
+}
+template< typename Index >
+double nrm2( VectorView< const double, Devices::MI355X, Index > a )
+{
+   return device_scalar( [ & ]( double* d ) { return tnlmi_nrm2( a.getSize(), a.getData(), d, nullptr ); } );
+}
+template< typename Index >
+Vector< double, Devices::MI355X, Index > minus( VectorView< const double, Devices::MI355X, Index > a,
+                                                VectorView< const double, Devices::MI355X, Index > b )
+{
+   if( a.getSize() != b.getSize() )
+      throw std::invalid_argument( "the vectors of a difference differ in size" );
+   Vector< double, Devices::MI355X, Index > out;
+   out = a;
+   MI355XDetail::check( tnlmi_axpby( out.getSize(), -1.0, b.getData(), 1.0, out.getData(), nullptr ) );
+   MI355XDetail::hip( hipDeviceSynchronize() );
+   return out;
+}
+}  // namespace detail_mi355x
 
 //! (a, b): the scalar product of Containers/Expressions/ExpressionTemplates.h:372-384
-template< typename Index >
-double operator,( const Vector< double, Devices::MI355X, Index >& a, const Vector< double, Devices::MI355X, Index >& b )
+template< typename I, typename A1, typename A2 >
+double operator,( const Vector< double, Devices::MI355X, I, A1 >& a, const Vector< double, Devices::MI355X, I, A2 >& b )
 {
-   double* d;
-   detail::hip( hipMalloc( (void**) &d, sizeof( double ) ) );
-   const tnlmi_status st = tnlmi_dot( a.getSize(), a.getData(), b.getData(), d, nullptr );
-   double h = 0;
-   if( st == TNLMI_OK )
-      detail::hip( hipMemcpy( &h, d, sizeof( double ), hipMemcpyDeviceToHost ) );
-   (void) hipFree( d );
-   detail::check( st );
-   return h;
+   return detail_mi355x::dot< I >( a.getConstView(), b.getConstView() );
 }
-
-template< typename Index >
-double l2Norm( const Vector< double, Devices::MI355X, Index >& a )
+template< typename R1, typename R2, typename I >
+double operator,( const VectorView< R1, Devices::MI355X, I >& a, const VectorView< R2, Devices::MI355X, I >& b )
 {
-   double* d;
-   detail::hip( hipMalloc( (void**) &d, sizeof( double ) ) );
-   const tnlmi_status st = tnlmi_nrm2( a.getSize(), a.getData(), d, nullptr );
-   double h = 0;
-   if( st == TNLMI_OK )
-      detail::hip( hipMemcpy( &h, d, sizeof( double ), hipMemcpyDeviceToHost ) );
-   (void) hipFree( d );
-   detail::check( st );
-   return h;
+   return detail_mi355x::dot< I >( a.getConstView(), b.getConstView() );
 }
-
-template< typename Index >
-double lpNorm( const Vector< double, Devices::MI355X, Index >& a, double p )
+//! l2Norm / lpNorm( ., 2 ) (ExpressionTemplates.h:442-460)
+template< typename I, typename A >
+double l2Norm( const Vector< double, Devices::MI355X, I, A >& a )
+{
+   return detail_mi355x::nrm2< I >( a.getConstView() );
+}
+template< typename R, typename I >
+double l2Norm( const VectorView< R, Devices::MI355X, I >& a )
+{
+   return detail_mi355x::nrm2< I >( a.getConstView() );
+}
+template< typename I, typename A >
+double lpNorm( const Vector< double, Devices::MI355X, I, A >& a, double p )
 {
    if( p != 2.0 )
-      throw Exceptions::NotImplementedError( "lpNorm on MI355X is implemented for p = 2" );
+      throw Exceptions::NotImplementedError( "lpNorm on Devices::MI355X is implemented for p = 2" );
    return l2Norm( a );
+}
+template< typename R, typename I >
+double lpNorm( const VectorView< R, Devices::MI355X, I >& a, double p )
+{
+   if( p != 2.0 )
+      throw Exceptions::NotImplementedError( "lpNorm on Devices::MI355X is implemented for p = 2" );
+   return l2Norm( a );
+}
+//! a - b into a new vector (the expression LinearResidueGetter.hpp:21 evaluates)
+template< typename I, typename A1, typename A2 >
+Vector< double, Devices::MI355X, I > operator-( const Vector< double, Devices::MI355X, I, A1 >& a,
+                                                 const Vector< double, Devices::MI355X, I, A2 >& b )
+{
+   return detail_mi355x::minus< I >( a.getConstView(), b.getConstView() );
+}
+template< typename R, typename I, typename A >
+Vector< double, Devices::MI355X, I > operator-( const Vector< double, Devices::MI355X, I, A >& a,
+                                                 const VectorView< R, Devices::MI355X, I >& b )
+{
+   return detail_mi355x::minus< I >( a.getConstView(), b.getConstView() );
+}
+template< typename R, typename I, typename A >
+Vector< double, Devices::MI355X, I > operator-( const VectorView< R, Devices::MI355X, I >& a,
+                                                 const Vector< double, Devices::MI355X, I, A >& b )
+{
+   return detail_mi355x::minus< I >( a.getConstView(), b.getConstView() );
+}
+template< typename R1, typename R2, typename I >
+Vector< double, Devices::MI355X, I > operator-( const VectorView< R1, Devices::MI355X, I >& a,
+                                                 const VectorView< R2, Devices::MI355X, I >& b )
+{
+   return detail_mi355x::minus< I >( a.getConstView(), b.getConstView() );
 }
 
 }  // namespace Containers
 
 // ----------------------------------------------------------------- segments
-namespace Algorithms::Segments {
+namespace MI355XDetail {
 
-enum ElementsOrganization : std::uint8_t { ColumnMajorOrder = 0, RowMajorOrder };
-
-enum class ThreadsToSegmentsMapping : std::uint8_t { Fixed, Warp, Block, BlockMerged, DynamicGrouping };
-
-//! Segments/LaunchConfiguration.h:18-91; the default-constructed
-//! configuration selects the MI355X-tuned kernel of each format.
-struct LaunchConfiguration
-{
-   LaunchConfiguration() = default;
-   LaunchConfiguration( ThreadsToSegmentsMapping mapping, int threadsPerSegmentCount ) : mapping_( mapping ), set_( true )
-   {
-      setThreadsPerSegmentCount( threadsPerSegmentCount );
-   }
-   void setThreadsPerSegmentCount( int t )
-   {
-      if( t > 0 && ( t & ( t - 1 ) ) == 0 )
-         tps_ = t;
-      else
-         throw std::invalid_argument( "Threads per segment count must be a positive power of two." );
-      set_ = true;
-   }
-   void setThreadsToSegmentsMapping( ThreadsToSegmentsMapping m )
-   {
-      mapping_ = m;
-      set_ = true;
-   }
-   int getThreadsPerSegmentCount() const { return tps_; }
-   ThreadsToSegmentsMapping getThreadsToSegmentsMapping() const { return mapping_; }
-   tnlmi_launch toABI() const
-   {
-      if( ! set_ )
-         return { TNLMI_MAP_AUTO, 1 };
-      return { (int32_t) mapping_, tps_ };
-   }
-
-private:
-   ThreadsToSegmentsMapping mapping_ = ThreadsToSegmentsMapping::Fixed;
-   int tps_ = 1;
-   bool set_ = false;
-};
-
-// Segment format tags: the class templates of Algorithms/Segments with the
-// reference's template parameters; each names its C ABI format.
-template< typename Device, typename Index, typename IndexAllocator = void >
-struct CSR
+//! The C ABI format of each segments template of the reference (instantiated
+//! on the host device only to name its parameters).
+template< typename S >
+struct SegmentsFormat;
+template< typename D, typename I, typename A >
+struct SegmentsFormat< Algorithms::Segments::CSR< D, I, A > >
 {
    static constexpr tnlmi_format format = TNLMI_CSR;
    static constexpr int param = 0;
    static constexpr tnlmi_organization organization = TNLMI_COLUMN_MAJOR;
 };
-template< typename Device, typename Index, typename IndexAllocator = void >
-struct AdaptiveCSR
+template< typename D, typename I, typename A >
+struct SegmentsFormat< Algorithms::Segments::AdaptiveCSR< D, I, A > >
 {
    static constexpr tnlmi_format format = TNLMI_ADAPTIVE_CSR;
    static constexpr int param = 0;
    static constexpr tnlmi_organization organization = TNLMI_COLUMN_MAJOR;
 };
-template< typename Device, typename Index, typename IndexAllocator = void,
-          ElementsOrganization Organization = ColumnMajorOrder, int Alignment = 32 >
-struct Ellpack
+template< typename D, typename I, typename A, Algorithms::Segments::ElementsOrganization O, int Alignment >
+struct SegmentsFormat< Algorithms::Segments::Ellpack< D, I, A, O, Alignment > >
 {
+   static_assert( Alignment == 32, "Ellpack on Devices::MI355X keeps the reference's default alignment 32" );
    static constexpr tnlmi_format format = TNLMI_ELLPACK;
    static constexpr int param = 0;
-   static constexpr tnlmi_organization organization = (tnlmi_organization) Organization;
+   static constexpr tnlmi_organization organization = (tnlmi_organization) O;
 };
-template< typename Device, typename Index, typename IndexAllocator = void,
-          ElementsOrganization Organization = ColumnMajorOrder, int SliceSize = 32 >
-struct SlicedEllpack
+template< typename D, typename I, typename A, Algorithms::Segments::ElementsOrganization O, int SliceSize >
+struct SegmentsFormat< Algorithms::Segments::SlicedEllpack< D, I, A, O, SliceSize > >
 {
    static constexpr tnlmi_format format = TNLMI_SLICED_ELLPACK;
    static constexpr int param = SliceSize;
-   static constexpr tnlmi_organization organization = (tnlmi_organization) Organization;
+   static constexpr tnlmi_organization organization = (tnlmi_organization) O;
 };
-template< typename Device, typename Index, typename IndexAllocator = void,
-          ElementsOrganization Organization = ColumnMajorOrder, int WarpSize = 32 >
-struct BiEllpack
+template< typename D, typename I, typename A, Algorithms::Segments::ElementsOrganization O, int WarpSize >
+struct SegmentsFormat< Algorithms::Segments::BiEllpack< D, I, A, O, WarpSize > >
 {
    static constexpr tnlmi_format format = TNLMI_BIELLPACK;
    static constexpr int param = WarpSize;
-   static constexpr tnlmi_organization organization = (tnlmi_organization) Organization;
+   static constexpr tnlmi_organization organization = (tnlmi_organization) O;
 };
-template< typename Device, typename Index, typename IndexAllocator = void,
-          ElementsOrganization Organization = ColumnMajorOrder >
-struct ChunkedEllpack
+template< typename D, typename I, typename A, Algorithms::Segments::ElementsOrganization O >
+struct SegmentsFormat< Algorithms::Segments::ChunkedEllpack< D, I, A, O > >
 {
    static constexpr tnlmi_format format = TNLMI_CHUNKED_ELLPACK;
    static constexpr int param = 0;
-   static constexpr tnlmi_organization organization = (tnlmi_organization) Organization;
+   static constexpr tnlmi_organization organization = (tnlmi_organization) O;
 };
-template< typename Device, typename Index, typename IndexAllocator = void >
-struct SortedCSR
+template< typename D, typename I, typename A, typename A2 >
+struct SegmentsFormat< Algorithms::Segments::SortedSegments< Algorithms::Segments::CSR< D, I, A >, A2 > >
 {
    static constexpr tnlmi_format format = TNLMI_SORTED_CSR;
    static constexpr int param = 0;
    static constexpr tnlmi_organization organization = TNLMI_COLUMN_MAJOR;
 };
-// three-parameter aliases for SparseMatrix< ..., Segments > (the form the
-// reference's template template parameter takes)
-template< typename D, typename I, typename A >
-using SlicedEllpack64 = SlicedEllpack< D, I, A, ColumnMajorOrder, 64 >;
 
-}  // namespace Algorithms::Segments
+//! LaunchConfiguration.h:18-91 -> the C ABI.  The reference's
+//! default-constructed configuration (Fixed, 1) asks for no particular kernel
+//! and selects this device's tuned product (TNLMI_MAP_AUTO); every other
+//! configuration runs the reference's kernel of that mapping.
+inline tnlmi_launch launch( const Algorithms::Segments::LaunchConfiguration& c )
+{
+   using M = Algorithms::Segments::ThreadsToSegmentsMapping;
+   if( c.getThreadsToSegmentsMapping() == M::Fixed && c.getThreadsPerSegmentCount() == 1 )
+      return { TNLMI_MAP_AUTO, 1 };
+   return { (int32_t) c.getThreadsToSegmentsMapping(), c.getThreadsPerSegmentCount() };
+}
+}  // namespace MI355XDetail
 
 // ------------------------------------------------------------------- matrix
 namespace Matrices {
 
-struct GeneralMatrix
-{
-   static constexpr bool isSymmetric() { return false; }
-};
-//! MatrixType.h: the storage holds the lower triangle and the diagonal
-struct SymmetricMatrix
-{
-   static constexpr bool isSymmetric() { return true; }
-};
-
-template< typename Real, typename Device, typename Index = int, typename MatrixType = GeneralMatrix,
-          template< typename, typename, typename > class Segments = Algorithms::Segments::CSR,
-          typename ComputeReal = Real >
-class SparseMatrix
+//! Matrices::SparseMatrix on the MI355X (SparseMatrix.h:41-56): general or
+//! symmetric (lower triangle stored), any segments template of the
+//! reference, FP64 or FP32 values, int indices.  Values and column indexes
+//! live in device memory from RealAllocator / IndexAllocator.
+template< typename Real, typename Index, typename MatrixType, template< typename, typename, typename > class Segments,
+          typename ComputeReal, typename RealAllocator, typename IndexAllocator >
+class SparseMatrix< Real, Devices::MI355X, Index, MatrixType, Segments, ComputeReal, RealAllocator, IndexAllocator >
 {
    // the device kernels take int32 offsets and column indexes (TNL's default
    // Index = int); 64-bit indices are not provided
    static_assert( std::is_same_v< Index, int >, "SparseMatrix on Devices::MI355X supports Index = int only" );
-};
-
-//! Matrices::SparseMatrix on the MI355X (SparseMatrix.h:41-56), general
-//! matrix, int32 indices, FP64 or FP32 values.
-template< typename Real, typename MatrixType, template< typename, typename, typename > class Segments,
-          typename ComputeReal >
-class SparseMatrix< Real, Devices::MI355X, int, MatrixType, Segments, ComputeReal >
-{
-   using SegTag = Segments< Devices::MI355X, int, void >;
+   using SegTag = MI355XDetail::SegmentsFormat<
+      Segments< Devices::Host, Index, typename Allocators::Default< Devices::Host >::template Allocator< Index > > >;
 
 public:
    using RealType = Real;
    using DeviceType = Devices::MI355X;
-   using IndexType = int;
+   using IndexType = Index;
    using ComputeRealType = ComputeReal;
-   using VectorType = Containers::Vector< Real, Devices::MI355X, int >;
-   using RowCapacitiesType = Containers::Vector< int, Devices::MI355X, int >;
+   using RealAllocatorType = RealAllocator;
+   using IndexAllocatorType = IndexAllocator;
+   using ValuesVectorType = Containers::Vector< Real, Devices::MI355X, Index, RealAllocator >;
+   using ColumnIndexesVectorType = Containers::Vector< Index, Devices::MI355X, Index, IndexAllocator >;
+   using RowCapacitiesType = Containers::Vector< Index, Devices::MI355X, Index, IndexAllocator >;
+   template< typename _Real = Real, typename _Device = Devices::MI355X, typename _Index = Index,
+             typename _MatrixType = MatrixType,
+             template< typename, typename, typename > class _Segments = Segments,
+             typename _ComputeReal = ComputeReal,
+             typename _RealAllocator = typename Allocators::Default< _Device >::template Allocator< _Real >,
+             typename _IndexAllocator = typename Allocators::Default< _Device >::template Allocator< _Index > >
+   using Self = SparseMatrix< _Real, _Device, _Index, _MatrixType, _Segments, _ComputeReal, _RealAllocator,
+                              _IndexAllocator >;
 
    SparseMatrix() = default;
-   SparseMatrix( int rows, int columns ) { setDimensions( rows, columns ); }
+   SparseMatrix( Index rows, Index columns ) { setDimensions( rows, columns ); }
    //! SparseMatrix( rows, columns, { { row, column, value }, ... } ) (SparseMatrix.hpp:336-367)
-   SparseMatrix( int rows, int columns, const std::initializer_list< std::tuple< int, int, Real > >& data )
+   SparseMatrix( Index rows, Index columns, const std::initializer_list< std::tuple< Index, Index, Real > >& data )
    {
       setDimensions( rows, columns );
-      std::map< std::pair< int, int >, Real > m;
+      std::map< std::pair< Index, Index >, Real > m;
       for( const auto& [ r, c, v ] : data )
          m[ { r, c } ] = v;
       setElements( m );
    }
-   //! SparseMatrix( rowCapacities, columns ) (SparseMatrix.h): empty rows of the
-   //! given capacities (setRowCapacities)
-   SparseMatrix( const std::initializer_list< int >& rowCapacities, int columns )
+   //! SparseMatrix( rows, columns, std::map ) (SparseMatrix.hpp:369-378)
+   template< typename MapIndex, typename MapValue >
+   SparseMatrix( Index rows, Index columns, const std::map< std::pair< MapIndex, MapIndex >, MapValue >& map )
    {
-      setDimensions( (int) rowCapacities.size(), columns );
-      setRowCapacities( std::vector< int >( rowCapacities ) );
+      setDimensions( rows, columns );
+      setElements( map );
    }
-   SparseMatrix( const SparseMatrix& ) = delete;
-   SparseMatrix& operator=( const SparseMatrix& ) = delete;
+   //! SparseMatrix( rowCapacities, columns ): empty rows of the given capacities
+   SparseMatrix( const std::initializer_list< Index >& rowCapacities, Index columns )
+   {
+      setDimensions( (Index) rowCapacities.size(), columns );
+      setRowCapacities( std::vector< Index >( rowCapacities ) );
+   }
+   SparseMatrix( const SparseMatrix& o ) { *this = o; }
+   SparseMatrix& operator=( const SparseMatrix& o )
+   {
+      if( this == &o )
+         return *this;
+      if( o.segments_ )
+         copyFrom( o );
+      else
+         setDimensions( o.rows_, o.columns_ );
+      return *this;
+   }
+   //! cross-device assignment from the reference's host matrix
+   //! (SparseMatrix.hpp:545-575): every row's stored (column, value) pairs in
+   //! slot order, padding skipped
+   template< typename R2, typename M2, template< typename, typename, typename > class S2, typename C2, typename RA2,
+             typename IA2 >
+   SparseMatrix& operator=( const SparseMatrix< R2, Devices::Host, Index, M2, S2, C2, RA2, IA2 >& host )
+   {
+      static_assert( M2::isSymmetric() == MatrixType::isSymmetric(),
+                     "a general and a symmetric matrix cannot be assigned to each other" );
+      const Index rows = host.getRows();
+      std::vector< Index > off( rows + 1, 0 ), cols;
+      std::vector< Real > vals;
+      for( Index r = 0; r < rows; r++ ) {
+         const auto row = host.getRow( r );
+         for( Index k = 0; k < row.getSize(); k++ )
+            if( row.getColumnIndex( k ) >= 0 ) {
+               cols.push_back( row.getColumnIndex( k ) );
+               vals.push_back( (Real) row.getValue( k ) );
+            }
+         off[ r + 1 ] = (Index) cols.size();
+      }
+      rows_ = rows;
+      columns_ = host.getColumns();
+      setFromCSR( off, cols, vals );
+      return *this;
+   }
    ~SparseMatrix() { release(); }
 
    //! copySparseMatrix( *this, B ) (Matrices/SparseOperations.hpp:600-690) from
@@ -443,59 +715,72 @@ public:
       static_assert( std::is_same_v< Real, typename Matrix2::RealType >, "The matrices must have the same RealType." );
       static_assert( std::is_same_v< typename Matrix2::DeviceType, Devices::MI355X >,
                      "The matrices must be allocated on the same device." );
+      const tnlmi_matrix src = B.abi();
       release();
       rows_ = B.getRows();
       columns_ = B.getColumns();
-      const tnlmi_matrix src = B.abi();
-      int* d_len = nullptr;
-      detail::hip( hipMalloc( (void**) &d_len, std::max( 1, rows_ ) * sizeof( int ) ) );
-      std::unique_ptr< int, void ( * )( int* ) > own( d_len, []( int* p ) { (void) hipFree( p ); } );
-      detail::check( tnlmi_matrix_row_lengths( &src, d_len, nullptr ) );
-      detail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, d_len, rows_, nullptr,
-                                            &segments_ ) );
-      const int64_t storage = std::max< int64_t >( 1, tnlmi_segments_storage_size( segments_ ) );
-      detail::hip( hipMalloc( (void**) &values_, storage * sizeof( Real ) ) );
-      detail::hip( hipMalloc( (void**) &columnIndexes_, storage * sizeof( int ) ) );
+      RowCapacitiesType len( std::max< Index >( 1, rows_ ) );
+      MI355XDetail::check( tnlmi_matrix_row_lengths( &src, len.getData(), nullptr ) );
+      MI355XDetail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param,
+                                                    len.getData(), rows_, nullptr, &segments_ ) );
+      allocateStorage();
       tnlmi_matrix dst = abi();
-      detail::check( tnlmi_matrix_copy( &src, &dst, d_len, nullptr ) );
+      MI355XDetail::check( tnlmi_matrix_copy( &src, &dst, len.getData(), nullptr ) );
       mirror();
    }
-   template< typename, typename, typename, typename, template< typename, typename, typename > class, typename >
-   friend class SparseMatrix;
 
-   void setDimensions( int rows, int columns )
+   void setDimensions( Index rows, Index columns )
    {
       if( rows < 0 || columns < 0 )
          throw std::invalid_argument( "setDimensions: negative dimensions" );
       rows_ = rows;
       columns_ = columns;
-      std::vector< int > zero( rows, 0 );
-      setRowCapacities( zero );
+      setRowCapacities( std::vector< Index >( rows, 0 ) );
    }
-   int getRows() const { return rows_; }
-   int getColumns() const { return columns_; }
+   //! setDimensions and row capacities of another matrix (SparseMatrix.hpp:238-255)
+   template< typename Matrix_ >
+   void setLike( const Matrix_& m )
+   {
+      rows_ = m.getRows();
+      columns_ = m.getColumns();
+      RowCapacitiesType caps;
+      m.getRowCapacities( caps );
+      setRowCapacities( caps );
+   }
+   Index getRows() const { return rows_; }
+   Index getColumns() const { return columns_; }
 
    //! setRowCapacities (SparseMatrix.hpp:301-327): builds the format's
    //! segments; values = 0, column indexes = -1 (padding)
-   void setRowCapacities( const std::vector< int >& capacities )
+   void setRowCapacities( const std::vector< Index >& capacities )
    {
-      if( (int) capacities.size() != rows_ )
+      if( (Index) capacities.size() != rows_ )
          throw std::invalid_argument( "setRowCapacities: the number of row capacities differs from the rows" );
-      std::vector< int > off( rows_ + 1, 0 );
-      for( int r = 0; r < rows_; r++ )
+      std::vector< Index > off( rows_ + 1, 0 );
+      for( Index r = 0; r < rows_; r++ )
          off[ r + 1 ] = off[ r ] + capacities[ r ];
-      std::vector< int > cols( off[ rows_ ], -1 );
+      std::vector< Index > cols( off[ rows_ ], -1 );
       std::vector< Real > vals( off[ rows_ ], Real( 0 ) );
       assemble( capacities, off, cols, vals );
    }
-   void setRowCapacities( const RowCapacitiesType& capacities ) { setRowCapacities( capacities.toHost() ); }
+   template< typename A >
+   void setRowCapacities( const Containers::Vector< Index, Devices::MI355X, Index, A >& capacities )
+   {
+      setRowCapacities( capacities.toHost() );
+   }
+   template< typename A >
+   void setRowCapacities( const Containers::Vector< Index, Devices::Host, Index, A >& capacities )
+   {
+      setRowCapacities( std::vector< Index >( capacities.getData(), capacities.getData() + capacities.getSize() ) );
+   }
 
    //! setElements( map ) (SparseMatrix.hpp:380-470, Complete encoding):
    //! capacities = entries per row; a symmetric matrix keeps the entries on
    //! and below the diagonal and checks that the others mirror them
-   void setElements( const std::map< std::pair< int, int >, Real >& m )
+   template< typename MapIndex, typename MapValue >
+   void setElements( const std::map< std::pair< MapIndex, MapIndex >, MapValue >& m )
    {
-      std::vector< int > caps( rows_, 0 );
+      std::vector< Index > caps( rows_, 0 );
       for( const auto& [ rc, v ] : m ) {
          if( rc.first < 0 || rc.first >= rows_ || rc.second < 0 || rc.second >= columns_ )
             throw std::invalid_argument( "setElements: element out of the matrix" );
@@ -511,27 +796,28 @@ public:
          }
          caps[ rc.first ]++;
       }
-      std::vector< int > off( rows_ + 1, 0 );
-      for( int r = 0; r < rows_; r++ )
+      std::vector< Index > off( rows_ + 1, 0 );
+      for( Index r = 0; r < rows_; r++ )
          off[ r + 1 ] = off[ r ] + caps[ r ];
-      std::vector< int > cols( off[ rows_ ] ), pos( off.begin(), off.end() - 1 );
+      std::vector< Index > cols( off[ rows_ ] ), pos( off.begin(), off.end() - 1 );
       std::vector< Real > vals( off[ rows_ ] );
       for( const auto& [ rc, v ] : m ) {  // std::map is ordered: columns ascend within a row
          if( MatrixType::isSymmetric() && rc.first < rc.second )
             continue;
-         cols[ pos[ rc.first ] ] = rc.second;
-         vals[ pos[ rc.first ]++ ] = v;
+         cols[ pos[ rc.first ] ] = (Index) rc.second;
+         vals[ pos[ rc.first ]++ ] = (Real) v;
       }
       assemble( caps, off, cols, vals );
    }
 
    //! Assembly from CSR-ordered host arrays with row capacities taken from
-   //! `offsets` (the device build of TNL fills a host matrix and copies it,
-   //! SparseMatrix.hpp:360-367).
-   void setFromCSR( const std::vector< int >& offsets, const std::vector< int >& columns, const std::vector< Real >& values )
+   //! `offsets` (an extension of this device; the reference's device build
+   //! fills a host matrix and copies it, SparseMatrix.hpp:360-367).
+   void setFromCSR( const std::vector< Index >& offsets, const std::vector< Index >& columns,
+                    const std::vector< Real >& values )
    {
-      std::vector< int > caps( rows_ );
-      for( int r = 0; r < rows_; r++ )
+      std::vector< Index > caps( rows_ );
+      for( Index r = 0; r < rows_; r++ )
          caps[ r ] = offsets[ r + 1 ] - offsets[ r ];
       assemble( caps, offsets, columns, values );
    }
@@ -539,20 +825,20 @@ public:
    //! vectorProduct (SparseMatrixBase.hpp:267-391): out = a*A*in + b*out on [begin, end)
    template< typename InVector, typename OutVector >
    void vectorProduct( const InVector& in, OutVector& out, ComputeReal matrixMultiplicator = 1.0,
-                       ComputeReal outVectorMultiplicator = 0.0, int begin = 0, int end = 0,
+                       ComputeReal outVectorMultiplicator = 0.0, Index begin = 0, Index end = 0,
                        const Algorithms::Segments::LaunchConfiguration& launchConfig = {} ) const
    {
       if( getColumns() != in.getSize() )
          throw std::invalid_argument( "vectorProduct: size of the input vector does not match the number of matrix columns" );
       if( getRows() != out.getSize() )
          throw std::invalid_argument( "vectorProduct: size of the output vector does not match the number of matrix rows" );
-      using OutReal = typename OutVector::RealType;
+      using OutReal = std::remove_const_t< typename OutVector::RealType >;
       const tnlmi_matrix m = abi();
-      const tnlmi_launch cfg = launchConfig.toABI();
-      detail::check( tnlmi_spmv( &m, detail::dtype< OutReal >(), in.getData(), out.getData(),
-                                 detail::dtype< ComputeReal >(), matrixMultiplicator, outVectorMultiplicator, begin, end,
-                                 &cfg, nullptr ) );
-      detail::hip( hipDeviceSynchronize() );
+      const tnlmi_launch cfg = MI355XDetail::launch( launchConfig );
+      MI355XDetail::check( tnlmi_spmv( &m, MI355XDetail::dtype< OutReal >(), in.getData(), out.getData(),
+                                         MI355XDetail::dtype< ComputeReal >(), matrixMultiplicator,
+                                         outVectorMultiplicator, begin, end, &cfg, nullptr ) );
+      MI355XDetail::hip( hipDeviceSynchronize() );
    }
    template< typename InVector, typename OutVector >
    void vectorProduct( const InVector& in, OutVector& out, const Algorithms::Segments::LaunchConfiguration& cfg ) const
@@ -560,104 +846,10 @@ public:
       vectorProduct( in, out, 1.0, 0.0, 0, 0, cfg );
    }
 
-   //! getElement( row, column ) (SparseMatrixBase.hpp:231-265), 0 if absent;
-   //! a symmetric matrix reads (column, row) for row < column
-   Real getElement( int row, int column ) const
-   {
-      const tnlmi_matrix m = abi();
-      double v = 0;
-      detail::check( tnlmi_matrix_get_element( &m, row, column, &v, nullptr ) );
-      return (Real) v;
-   }
-   //! setElement( row, column, value ) (SparseMatrixBase.hpp:152-161)
-   void setElement( int row, int column, const Real& value ) { addElement( row, column, value, Real( 0 ) ); }
-   //! addElement( row, column, value, thisElementMultiplicator ) (SparseMatrixBase.hpp:166-231):
-   //! an existing element becomes multiplicator * old + value; a new one is
-   //! inserted in column order into the row's capacity (std::logic_error
-   //! "The capacity of the sparse matrix row number r was exceeded." if full)
-   void addElement( int row, int column, const Real& value, const Real& thisElementMultiplicator = 1.0 )
-   {
-      tnlmi_matrix m = abi();
-      detail::check( tnlmi_matrix_add_element( &m, row, column, value, thisElementMultiplicator, nullptr ) );
-   }
-   //! getRowCapacity( row ): the row's segment size (Ellpack: the width)
-   int getRowCapacity( int row ) const { return hoff_[ row + 1 ] - hoff_[ row ]; }
-   //! getRowCapacities (SparseMatrixBase.hpp:75-92): stored (non-padding) slots per row
-   void getRowCapacities( RowCapacitiesType& out ) const { rowCounts( 1, out ); }
-   //! getCompressedRowLengths (SparseMatrixBase.hpp:55-73): stored nonzero values per row
-   void getCompressedRowLengths( RowCapacitiesType& out ) const { rowCounts( 2, out ); }
-   //! getNonzeroElementsCount (SparseMatrixBase.hpp:102-132): a symmetric matrix
-   //! counts its stored off-diagonal entries twice
-   int64_t getNonzeroElementsCount() const
-   {
-      RowCapacitiesType c;
-      rowCounts( MatrixType::isSymmetric() ? 3 : 2, c );
-      int64_t total = 0;
-      for( int v : c.toHost() )
-         total += v;
-      return total;
-   }
-   int64_t getStorageSize() const { return segments_ ? tnlmi_segments_storage_size( segments_ ) : 0; }
-
-   //! getSerializationType (SparseMatrixBase.hpp:46-53)
-   static std::string getSerializationType()
-   {
-      tnlmi_segments* empty = nullptr;
-      detail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, nullptr, 0, nullptr,
-                                            &empty ) );
-      std::unique_ptr< tnlmi_segments, void ( * )( tnlmi_segments* ) > own( empty, tnlmi_segments_destroy );
-      const tnlmi_matrix m{ 0, 0, empty, detail::dtype< Real >(), nullptr, nullptr, matrixType() };
-      int64_t n = 0;
-      detail::check( tnlmi_matrix_serialization_type( &m, nullptr, 0, &n ) );
-      std::string t( (size_t) n + 1, '\0' );
-      detail::check( tnlmi_matrix_serialization_type( &m, t.data(), n + 1, &n ) );
-      t.resize( (size_t) n );
-      return t;
-   }
-
-   //! save( fileName ) (SparseMatrix.hpp:640-646): TNL's binary format
-   void save( const std::string& fileName ) const
-   {
-      const tnlmi_matrix m = abi();
-      detail::check( tnlmi_matrix_save( &m, fileName.c_str(), nullptr ) );
-   }
-
-   //! load( fileName ) (SparseMatrix.hpp:656-694): a file of this matrix type
-   //! (other types: "object type does not match"), its segments arrays as saved
-   void load( const std::string& fileName )
-   {
-      tnlmi_matrix_file* f = nullptr;
-      detail::check( tnlmi_matrix_file_read( fileName.c_str(), &f ) );
-      std::unique_ptr< tnlmi_matrix_file, void ( * )( tnlmi_matrix_file* ) > own( f, tnlmi_matrix_file_destroy );
-      const std::string found = tnlmi_matrix_file_type( f ), expected = getSerializationType();
-      if( found != expected )
-         throw std::runtime_error( "Failed to deserialize an object from the file '" + fileName
-                                   + "': object type does not match (expected " + expected + ", found " + found + ")." );
-      int64_t rows = 0, columns = 0, storage = 0;
-      detail::check( tnlmi_matrix_file_info( f, &rows, &columns, nullptr, nullptr, nullptr, nullptr, &storage,
-                                                     nullptr ) );
-      release();
-      rows_ = (int) rows;
-      columns_ = (int) columns;
-      detail::check( tnlmi_matrix_file_segments( f, nullptr, &segments_ ) );
-      detail::hip( hipMalloc( (void**) &values_, std::max< int64_t >( 1, storage ) * sizeof( Real ) ) );
-      detail::hip( hipMalloc( (void**) &columnIndexes_, std::max< int64_t >( 1, storage ) * sizeof( int ) ) );
-      detail::check( tnlmi_matrix_file_upload( f, values_, columnIndexes_, nullptr ) );
-      mirror();
-   }
-
-   //! The C ABI description of this matrix.
-   tnlmi_matrix abi() const
-   {
-      return { rows_, columns_, segments_, detail::dtype< Real >(), values_, columnIndexes_, matrixType() };
-   }
-   static constexpr bool isSymmetric() { return MatrixType::isSymmetric(); }
-   static constexpr int32_t matrixType() { return MatrixType::isSymmetric() ? TNLMI_SYMMETRIC : TNLMI_GENERAL; }
-
    //! transposedVectorProduct (SparseMatrixBase.hpp:541-588): out = a*A^T*in + b*out on columns [begin, end)
    template< typename InVector, typename OutVector >
    void transposedVectorProduct( const InVector& in, OutVector& out, ComputeReal matrixMultiplicator = 1.0,
-                                 ComputeReal outVectorMultiplicator = 0.0, int begin = 0, int end = 0 ) const
+                                 ComputeReal outVectorMultiplicator = 0.0, Index begin = 0, Index end = 0 ) const
    {
       if( getRows() != in.getSize() )
          throw std::invalid_argument(
@@ -665,91 +857,207 @@ public:
       if( getColumns() != out.getSize() )
          throw std::invalid_argument(
             "transposedVectorProduct: size of the output vector does not match the number of matrix columns" );
-      using OutReal = typename OutVector::RealType;
+      using OutReal = std::remove_const_t< typename OutVector::RealType >;
       const tnlmi_matrix m = abi();
-      detail::check( tnlmi_spmv_transposed( &m, detail::dtype< OutReal >(), in.getData(), out.getData(),
-                                            matrixMultiplicator, outVectorMultiplicator, begin, end, nullptr ) );
-      detail::hip( hipDeviceSynchronize() );
+      MI355XDetail::check( tnlmi_spmv_transposed( &m, MI355XDetail::dtype< OutReal >(), in.getData(), out.getData(),
+                                                    matrixMultiplicator, outVectorMultiplicator, begin, end,
+                                                    nullptr ) );
+      MI355XDetail::hip( hipDeviceSynchronize() );
    }
 
-private:
-   void rowCounts( int kind, RowCapacitiesType& out ) const
+   //! getElement( row, column ) (SparseMatrixBase.hpp:231-265), 0 if absent;
+   //! a symmetric matrix reads (column, row) for row < column
+   Real getElement( Index row, Index column ) const
    {
-      out.setSize( rows_ );
       const tnlmi_matrix m = abi();
-      detail::check( tnlmi_matrix_row_counts( &m, kind, out.getData(), nullptr ) );
+      double v = 0;
+      MI355XDetail::check( tnlmi_matrix_get_element( &m, row, column, &v, nullptr ) );
+      return (Real) v;
+   }
+   //! setElement( row, column, value ) (SparseMatrixBase.hpp:152-161)
+   void setElement( Index row, Index column, const Real& value ) { addElement( row, column, value, Real( 0 ) ); }
+   //! addElement( row, column, value, thisElementMultiplicator ) (SparseMatrixBase.hpp:166-231):
+   //! an existing element becomes multiplicator * old + value; a new one is
+   //! inserted in column order into the row's capacity (std::logic_error
+   //! "The capacity of the sparse matrix row number r was exceeded." if full)
+   void addElement( Index row, Index column, const Real& value, const Real& thisElementMultiplicator = 1.0 )
+   {
+      tnlmi_matrix m = abi();
+      MI355XDetail::check( tnlmi_matrix_add_element( &m, row, column, value, thisElementMultiplicator, nullptr ) );
+   }
+   //! getRowCapacity( row ): the row's segment size (Ellpack: the width)
+   Index getRowCapacity( Index row ) const { return hoff_[ row + 1 ] - hoff_[ row ]; }
+   //! getRowCapacities (SparseMatrixBase.hpp:75-92): stored (non-padding) slots per row
+   template< typename V >
+   void getRowCapacities( V& out ) const
+   {
+      rowCounts( 1, out );
+   }
+   //! getCompressedRowLengths (SparseMatrixBase.hpp:55-73): stored nonzero values per row
+   template< typename V >
+   void getCompressedRowLengths( V& out ) const
+   {
+      rowCounts( 2, out );
+   }
+   //! getNonzeroElementsCount (SparseMatrixBase.hpp:102-132): a symmetric matrix
+   //! counts its stored off-diagonal entries twice
+   Index getNonzeroElementsCount() const
+   {
+      RowCapacitiesType c;
+      rowCounts( MatrixType::isSymmetric() ? 3 : 2, c );
+      int64_t total = 0;
+      for( Index v : c.toHost() )
+         total += v;
+      return (Index) total;
+   }
+   Index getAllocatedElementsCount() const
+   {
+      return segments_ ? (Index) tnlmi_segments_storage_size( segments_ ) : 0;
+   }
+   //! the values and column indexes on the device (an extension of this
+   //! device: writing through them changes the matrix)
+   Real* getValuesData() const { return values_; }
+   Index* getColumnIndexesData() const { return columnIndexes_; }
+
+   //! getSerializationType (SparseMatrixBase.hpp:46-53)
+   static std::string getSerializationType()
+   {
+      tnlmi_segments* empty = nullptr;
+      MI355XDetail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, nullptr, 0,
+                                                    nullptr, &empty ) );
+      std::unique_ptr< tnlmi_segments, void ( * )( tnlmi_segments* ) > own( empty, tnlmi_segments_destroy );
+      const tnlmi_matrix m{ 0, 0, empty, MI355XDetail::dtype< Real >(), nullptr, nullptr, matrixType() };
+      int64_t n = 0;
+      MI355XDetail::check( tnlmi_matrix_serialization_type( &m, nullptr, 0, &n ) );
+      std::string t( (std::size_t) n + 1, '\0' );
+      MI355XDetail::check( tnlmi_matrix_serialization_type( &m, t.data(), n + 1, &n ) );
+      t.resize( (std::size_t) n );
+      return t;
+   }
+
+   //! save( fileName ) (SparseMatrix.hpp:640-646): TNL's binary format
+   void save( const std::string& fileName ) const
+   {
+      const tnlmi_matrix m = abi();
+      MI355XDetail::check( tnlmi_matrix_save( &m, fileName.c_str(), nullptr ) );
+   }
+
+   //! load( fileName ) (SparseMatrix.hpp:656-694): a file of this matrix type
+   //! (other types: "object type does not match"), its segments arrays as saved
+   void load( const std::string& fileName )
+   {
+      tnlmi_matrix_file* f = nullptr;
+      MI355XDetail::check( tnlmi_matrix_file_read( fileName.c_str(), &f ) );
+      std::unique_ptr< tnlmi_matrix_file, void ( * )( tnlmi_matrix_file* ) > own( f, tnlmi_matrix_file_destroy );
+      const std::string found = tnlmi_matrix_file_type( f ), expected = getSerializationType();
+      if( found != expected )
+         throw std::runtime_error( "Failed to deserialize an object from the file '" + fileName
+                                   + "': object type does not match (expected " + expected + ", found " + found + ")." );
+      int64_t rows = 0, columns = 0, storage = 0;
+      MI355XDetail::check(
+         tnlmi_matrix_file_info( f, &rows, &columns, nullptr, nullptr, nullptr, nullptr, &storage, nullptr ) );
+      release();
+      rows_ = (Index) rows;
+      columns_ = (Index) columns;
+      MI355XDetail::check( tnlmi_matrix_file_segments( f, nullptr, &segments_ ) );
+      allocateStorage();
+      MI355XDetail::check( tnlmi_matrix_file_upload( f, values_, columnIndexes_, nullptr ) );
+      mirror();
+   }
+
+   //! The C ABI description of this matrix.
+   tnlmi_matrix abi() const
+   {
+      return { rows_, columns_, segments_, MI355XDetail::dtype< Real >(), values_, columnIndexes_, matrixType() };
+   }
+   static constexpr bool isSymmetric() { return MatrixType::isSymmetric(); }
+   static constexpr int32_t matrixType() { return MatrixType::isSymmetric() ? TNLMI_SYMMETRIC : TNLMI_GENERAL; }
+
+private:
+   template< typename V >
+   void rowCounts( int kind, V& out ) const
+   {
+      RowCapacitiesType c( rows_ );
+      const tnlmi_matrix m = abi();
+      MI355XDetail::check( tnlmi_matrix_row_counts( &m, kind, c.getData(), nullptr ) );
+      if constexpr( std::is_same_v< typename V::DeviceType, Devices::MI355X > )
+         out = c.getConstView();
+      else {
+         out.setSize( rows_ );
+         const auto h = c.toHost();
+         for( Index r = 0; r < rows_; r++ )
+            out[ r ] = h[ r ];
+      }
    }
    // the rows' segment sizes on the host (getRowCapacity)
    void mirror()
    {
       RowCapacitiesType c;
       rowCounts( 0, c );
-      const std::vector< int > sizes = c.toHost();
+      const std::vector< Index > sizes = c.toHost();
       hoff_.assign( rows_ + 1, 0 );
-      for( int r = 0; r < rows_; r++ )
+      for( Index r = 0; r < rows_; r++ )
          hoff_[ r + 1 ] = hoff_[ r ] + sizes[ r ];
    }
-
+   void allocateStorage()
+   {
+      const std::size_t storage = (std::size_t) std::max< int64_t >( 1, tnlmi_segments_storage_size( segments_ ) );
+      values_ = RealAllocator().allocate( storage );
+      columnIndexes_ = IndexAllocator().allocate( storage );
+      storage_ = storage;
+   }
    void release()
    {
       if( segments_ )
          tnlmi_segments_destroy( segments_ );
-      for( void* p : { (void*) values_, (void*) columnIndexes_ } )
-         if( p )
-            (void) hipFree( p );
+      if( values_ )
+         RealAllocator().deallocate( values_, storage_ );
+      if( columnIndexes_ )
+         IndexAllocator().deallocate( columnIndexes_, storage_ );
       segments_ = nullptr;
       values_ = nullptr;
       columnIndexes_ = nullptr;
+      storage_ = 0;
    }
-   void assemble( const std::vector< int >& caps, const std::vector< int >& off, const std::vector< int >& cols,
+   void assemble( const std::vector< Index >& caps, const std::vector< Index >& off, const std::vector< Index >& cols,
                   const std::vector< Real >& vals )
    {
       release();
-      int *d_caps = nullptr, *d_off = nullptr, *d_cols = nullptr;
-      Real* d_vals = nullptr;
-      auto up = [ & ]( auto*& d, const auto& h ) {
-         const size_t bytes = std::max< size_t >( 1, h.size() ) * sizeof( h[ 0 ] );
-         detail::hip( hipMalloc( (void**) &d, bytes ) );
-         if( ! h.empty() )
-            detail::hip( hipMemcpy( d, h.data(), h.size() * sizeof( h[ 0 ] ), hipMemcpyHostToDevice ) );
-      };
-      up( d_caps, caps );
-      up( d_off, off );
-      up( d_cols, cols );
-      up( d_vals, vals );
-      tnlmi_status st = tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param, d_caps, rows_,
-                                               nullptr, &segments_ );
-      if( st == TNLMI_OK ) {
-         const int64_t storage = std::max< int64_t >( 1, tnlmi_segments_storage_size( segments_ ) );
-         detail::hip( hipMalloc( (void**) &values_, storage * sizeof( Real ) ) );
-         detail::hip( hipMalloc( (void**) &columnIndexes_, storage * sizeof( int ) ) );
-         st = tnlmi_matrix_fill( segments_, d_off, d_cols, d_vals, detail::dtype< Real >(), values_,
-                                 detail::dtype< Real >(), columnIndexes_, nullptr );
-         detail::hip( hipDeviceSynchronize() );
-      }
-      for( void* p : { (void*) d_caps, (void*) d_off, (void*) d_cols, (void*) d_vals } )
-         (void) hipFree( p );
-      detail::check( st );
+      RowCapacitiesType d_caps, d_off, d_cols;
+      ValuesVectorType d_vals;
+      d_caps = caps;
+      d_off = off;
+      d_cols = cols;
+      d_vals = vals;
+      MI355XDetail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param,
+                                                    d_caps.getData(), rows_, nullptr, &segments_ ) );
+      allocateStorage();
+      MI355XDetail::check( tnlmi_matrix_fill( segments_, d_off.getData(), d_cols.getData(), d_vals.getData(),
+                                                MI355XDetail::dtype< Real >(), values_,
+                                                MI355XDetail::dtype< Real >(), columnIndexes_, nullptr ) );
+      MI355XDetail::hip( hipDeviceSynchronize() );
       mirror();
    }
 
-   int rows_ = 0, columns_ = 0;
+   Index rows_ = 0, columns_ = 0;
    tnlmi_segments* segments_ = nullptr;
    Real* values_ = nullptr;
-   int* columnIndexes_ = nullptr;
-   std::vector< int > hoff_;  // prefix sums of the row capacities
+   Index* columnIndexes_ = nullptr;
+   std::size_t storage_ = 0;
+   std::vector< Index > hoff_{ 0 };  // prefix sums of the row capacities
 };
 
-//! copySparseMatrix (Matrices/SparseOperations.hpp:716-721) between MI355X
-//! matrices of any formats
-template< typename Matrix1, typename Matrix2 >
-void copySparseMatrix( Matrix1& A, const Matrix2& B )
+//! copySparseMatrix (Matrices/SparseOperations.hpp:716-721) between two
+//! MI355X matrices of any formats: a more specialised overload than the
+//! reference's generic template, so the call of the reference picks it.
+template< typename R1, typename I, typename M1, template< typename, typename, typename > class S1, typename C1,
+          typename RA1, typename IA1, typename R2, typename M2, template< typename, typename, typename > class S2,
+          typename C2, typename RA2, typename IA2 >
+void copySparseMatrix( SparseMatrix< R1, Devices::MI355X, I, M1, S1, C1, RA1, IA1 >& A,
+                       const SparseMatrix< R2, Devices::MI355X, I, M2, S2, C2, RA2, IA2 >& B )
 {
    A.copyFrom( B );
 }
-
-template< typename Matrix, typename Device = typename Matrix::DeviceType >
-class MatrixReader;
 
 //! MatrixReader::readMtx (Matrices/MatrixReader.hpp:16-32): a MatrixMarket
 //! coordinate file (real / integer / pattern, general / symmetric) into the
@@ -761,17 +1069,21 @@ template< typename Matrix >
 class MatrixReader< Matrix, Devices::MI355X >
 {
 public:
+   using RealType = typename Matrix::RealType;
+   using DeviceType = Devices::MI355X;
+   using IndexType = typename Matrix::IndexType;
+
    static void readMtx( const std::string& fileName, Matrix& matrix, bool verbose = false )
    {
       tnlmi_host_csr* h = nullptr;
-      detail::check( tnlmi_mtx_read( fileName.c_str(), &h ) );
+      MI355XDetail::check( tnlmi_mtx_read( fileName.c_str(), &h ) );
       load( h, matrix, verbose );
    }
    static void readMtx( std::istream& str, Matrix& matrix, bool verbose = false )
    {
       const std::string text( ( std::istreambuf_iterator< char >( str ) ), std::istreambuf_iterator< char >() );
       tnlmi_host_csr* h = nullptr;
-      detail::check( tnlmi_mtx_parse( text.data(), (int64_t) text.size(), &h ) );
+      MI355XDetail::check( tnlmi_mtx_parse( text.data(), (int64_t) text.size(), &h ) );
       load( h, matrix, verbose );
    }
 
@@ -782,198 +1094,168 @@ private:
       int64_t rows = 0, columns = 0, nnz = 0;
       const int32_t *off = nullptr, *cols = nullptr;
       const double* vals = nullptr;
-      detail::check( tnlmi_host_csr_get( h, &rows, &columns, &nnz, &off, &cols, &vals ) );
+      MI355XDetail::check( tnlmi_host_csr_get( h, &rows, &columns, &nnz, &off, &cols, &vals ) );
       if( verbose )
          std::cout << "Matrix dimensions are " << rows << " x " << columns << '\n';
-      matrix.setDimensions( (int) rows, (int) columns );
-      using Real = typename Matrix::RealType;
-      matrix.setFromCSR( std::vector< int >( off, off + rows + 1 ), std::vector< int >( cols, cols + nnz ),
-                         std::vector< Real >( vals, vals + nnz ) );
+      matrix.setDimensions( (IndexType) rows, (IndexType) columns );
+      matrix.setFromCSR( std::vector< IndexType >( off, off + rows + 1 ), std::vector< IndexType >( cols, cols + nnz ),
+                         std::vector< RealType >( vals, vals + nnz ) );
    }
 };
 
 }  // namespace Matrices
 
 // ------------------------------------------------------------------ solvers
-namespace Config {
-//! The subset of Config::ParameterContainer the solvers' setup() reads.
-class ParameterContainer
+namespace MI355XDetail {
+//! What a solver of this device needs of a preconditioner: its handle in the
+//! library (Diagonal and ILU0 of this device have one).
+struct DevicePreconditioner
 {
-public:
-   using Value = std::variant< int, double, std::string >;
-   template< typename T >
-   void addParameter( const std::string& name, const T& value )
-   {
-      if constexpr( std::is_same_v< T, const char* > || std::is_convertible_v< T, std::string > )
-         values_[ name ] = std::string( value );
-      else if constexpr( std::is_integral_v< T > )
-         values_[ name ] = (int) value;
-      else
-         values_[ name ] = (double) value;
-   }
-   bool checkParameter( const std::string& name ) const { return values_.count( name ) > 0; }
-   template< typename T >
-   T getParameter( const std::string& name ) const
-   {
-      const Value& v = values_.at( name );
-      if constexpr( std::is_same_v< T, std::string > )
-         return std::get< std::string >( v );
-      else if( std::holds_alternative< int >( v ) )
-         return (T) std::get< int >( v );
-      else
-         return (T) std::get< double >( v );
-   }
-
-private:
-   std::map< std::string, Value > values_;
-};
-}  // namespace Config
-
-namespace Solvers::Linear {
-
-namespace Preconditioners {
-
-template< typename Matrix >
-class Preconditioner
-{
-public:
-   using MatrixPointer = std::shared_ptr< const Matrix >;
-   using VectorType = typename Matrix::VectorType;
-   virtual ~Preconditioner()
+   virtual ~DevicePreconditioner()
    {
       if( handle_ )
          tnlmi_preconditioner_destroy( handle_ );
    }
-   virtual void update( const MatrixPointer& matrix ) = 0;
-   void solve( const VectorType& b, VectorType& x ) const
-   {
-      if( ! handle_ )
-         throw std::logic_error( "Preconditioner::solve: update() was not called" );
-      detail::check( tnlmi_preconditioner_solve( handle_, b.getData(), x.getData(), nullptr ) );
-      detail::hip( hipDeviceSynchronize() );
-   }
    const tnlmi_preconditioner* handle() const { return handle_; }
 
 protected:
+   template< typename MatrixPointer >
    void build( tnlmi_precond_kind kind, const MatrixPointer& matrix )
    {
       if( handle_ )
          tnlmi_preconditioner_destroy( handle_ );
       handle_ = nullptr;
       const tnlmi_matrix m = matrix->abi();
-      detail::check( tnlmi_preconditioner_create( kind, &m, nullptr, &handle_ ) );
+      check( tnlmi_preconditioner_create( kind, &m, nullptr, &handle_ ) );
+   }
+   void apply( const void* b, void* x ) const
+   {
+      if( ! handle_ )
+         throw std::logic_error( "Preconditioner::solve: update() was not called" );
+      check( tnlmi_preconditioner_solve( handle_, (const double*) b, (double*) x, nullptr ) );
+      hip( hipDeviceSynchronize() );
    }
    tnlmi_preconditioner* handle_ = nullptr;
 };
 
-//! Preconditioners::Diagonal (Diagonal.hpp:12-43)
+//! The shared part of this device's solvers: the reference's LinearSolver
+//! (setup, setMatrix, setPreconditioner and the IterativeSolver settings),
+//! the solver state handed to the library and the result read back.
 template< typename Matrix >
-class Diagonal : public Preconditioner< Matrix >
+class DeviceSolver : public Solvers::Linear::LinearSolver< Matrix >
 {
-public:
-   void update( const typename Preconditioner< Matrix >::MatrixPointer& m ) override
+protected:
+   tnlmi_solver_state state() const
    {
-      this->build( TNLMI_PRECOND_DIAGONAL, m );
+      return { (int64_t) this->getMaxIterations(), (int64_t) this->getMinIterations(),
+               (double) this->getConvergenceResidue(), (double) this->getDivergenceResidue(), 0, 0.0, 0 };
+   }
+   bool finish( const tnlmi_solver_state& st )
+   {
+      this->resetIterations();
+      this->currentIteration = (typename Matrix::IndexType) st.iterations;
+      this->setResidue( st.residue );
+      return st.converged != 0;
+   }
+   const tnlmi_preconditioner* pre() const
+   {
+      if( ! this->preconditioner )
+         return nullptr;
+      const auto* p = dynamic_cast< const DevicePreconditioner* >( this->preconditioner.get() );
+      if( ! p )
+         throw Exceptions::NotImplementedError(
+            "Devices::MI355X solvers take the Diagonal or ILU0 preconditioner of Devices::MI355X" );
+      return p->handle();
+   }
+   tnlmi_matrix mat( const char* who ) const
+   {
+      if( ! this->matrix )
+         throw std::logic_error( std::string( "No matrix was set in " ) + who + ". Call setMatrix() before solve()." );
+      static_assert( std::is_same_v< typename Matrix::RealType, double >,
+                     "the solvers of Devices::MI355X take FP64 vectors (a FP32 matrix is accepted through its "
+                     "product's FP64 accumulation)" );
+      return this->matrix->abi();
    }
 };
+}  // namespace MI355XDetail
 
-//! Preconditioners::ILU0 (ILU0.hpp:12-139); the reference's GPU version throws
-//! NotImplementedError, this device factorises on the host and solves on the GPU
-template< typename Matrix >
-class ILU0 : public Preconditioner< Matrix >
+#define TNLMI_SPARSE_MATRIX_PARAMS                                                                                    \
+   typename Real_, typename Index_, typename MatrixType_, template< typename, typename, typename > class Segments_,   \
+      typename ComputeReal_, typename RealAllocator_, typename IndexAllocator_
+#define TNLMI_SPARSE_MATRIX                                                                                           \
+   Matrices::SparseMatrix< Real_, Devices::MI355X, Index_, MatrixType_, Segments_, ComputeReal_, RealAllocator_,      \
+                           IndexAllocator_ >
+
+namespace Solvers::Linear {
+
+namespace Preconditioners {
+
+//! Preconditioners::Diagonal (Diagonal.hpp:12-43): d_i = A(i, i), x = b / d
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class Diagonal< TNLMI_SPARSE_MATRIX > : public Preconditioner< TNLMI_SPARSE_MATRIX >,
+                                        public MI355XDetail::DevicePreconditioner
 {
+   using Base = Preconditioner< TNLMI_SPARSE_MATRIX >;
+
 public:
-   void update( const typename Preconditioner< Matrix >::MatrixPointer& m ) override
-   {
-      this->build( TNLMI_PRECOND_ILU0, m );
-   }
+   using typename Base::ConstVectorViewType;
+   using typename Base::MatrixPointer;
+   using typename Base::VectorViewType;
+   void update( const MatrixPointer& matrix ) override { this->build( TNLMI_PRECOND_DIAGONAL, matrix ); }
+   void solve( ConstVectorViewType b, VectorViewType x ) const override { this->apply( b.getData(), x.getData() ); }
+};
+
+//! Preconditioners::ILU0 (ILU0.hpp:12-139) through the reference's per-device
+//! ILU0_impl hook (ILU0.h:26-28); the reference's GPU version throws
+//! NotImplementedError, this device factorises on the host (the reference's
+//! IKJ order) and solves on the GPU
+template< typename Matrix, typename Real, typename Index >
+class ILU0_impl< Matrix, Real, Devices::MI355X, Index > : public Preconditioner< Matrix >,
+                                                          public MI355XDetail::DevicePreconditioner
+{
+   using Base = Preconditioner< Matrix >;
+
+public:
+   using typename Base::ConstVectorViewType;
+   using typename Base::MatrixPointer;
+   using typename Base::VectorViewType;
+   void update( const MatrixPointer& matrix ) override { this->build( TNLMI_PRECOND_ILU0, matrix ); }
+   void solve( ConstVectorViewType b, VectorViewType x ) const override { this->apply( b.getData(), x.getData() ); }
 };
 
 }  // namespace Preconditioners
 
-//! LinearSolver + IterativeSolver (LinearSolver.h:37-167, IterativeSolver.h:226-235)
-template< typename Matrix >
-class LinearSolver
-{
-public:
-   using MatrixPointer = std::shared_ptr< const Matrix >;
-   using PreconditionerPointer = std::shared_ptr< const Preconditioners::Preconditioner< Matrix > >;
-   using VectorType = typename Matrix::VectorType;
-   using RealType = typename Matrix::RealType;
-   virtual ~LinearSolver() = default;
-
-   void setMatrix( const MatrixPointer& m ) { matrix_ = m; }
-   void setPreconditioner( const PreconditionerPointer& p ) { preconditioner_ = p; }
-   virtual bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" )
-   {
-      if( p.checkParameter( prefix + "max-iterations" ) )
-         max_iterations_ = p.getParameter< long >( prefix + "max-iterations" );
-      if( p.checkParameter( prefix + "min-iterations" ) )
-         min_iterations_ = p.getParameter< long >( prefix + "min-iterations" );
-      if( p.checkParameter( prefix + "convergence-residue" ) )
-         convergence_residue_ = p.getParameter< double >( prefix + "convergence-residue" );
-      if( p.checkParameter( prefix + "divergence-residue" ) )
-         divergence_residue_ = p.getParameter< double >( prefix + "divergence-residue" );
-      return true;
-   }
-   void setMaxIterations( long n ) { max_iterations_ = n; }
-   void setMinIterations( long n ) { min_iterations_ = n; }
-   void setConvergenceResidue( double r ) { convergence_residue_ = r; }
-   void setDivergenceResidue( double r ) { divergence_residue_ = r; }
-   long getIterations() const { return iterations_; }
-   double getResidue() const { return residue_; }
-   virtual bool solve( const VectorType& b, VectorType& x ) = 0;
-
-protected:
-   tnlmi_solver_state state() const
-   {
-      return { max_iterations_, min_iterations_, convergence_residue_, divergence_residue_, 0, 0.0, 0 };
-   }
-   bool finish( const tnlmi_solver_state& st )
-   {
-      iterations_ = st.iterations;
-      residue_ = st.residue;
-      return st.converged != 0;
-   }
-   const tnlmi_preconditioner* pre() const { return preconditioner_ ? preconditioner_->handle() : nullptr; }
-   tnlmi_matrix mat( const char* who ) const
-   {
-      if( ! matrix_ )
-         throw std::logic_error( std::string( "No matrix was set in " ) + who + ". Call setMatrix() before solve()." );
-      return matrix_->abi();
-   }
-   MatrixPointer matrix_;
-   PreconditionerPointer preconditioner_;
-   long max_iterations_ = 1000000000, min_iterations_ = 0;
-   double convergence_residue_ = 1e-6, divergence_residue_ = std::numeric_limits< double >::max();
-   long iterations_ = 0;
-   double residue_ = 0;
-};
-
 //! CG (CG.hpp:10-108)
-template< typename Matrix >
-class CG : public LinearSolver< Matrix >
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class CG< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "CG" );
       tnlmi_solver_state st = this->state();
-      detail::check( tnlmi_cg_solve( &m, nullptr, this->pre(), b.getData(), x.getData(), &st, nullptr ) );
+      MI355XDetail::check( tnlmi_cg_solve( &m, nullptr, this->pre(), b.getData(), x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 };
 
 //! GMRES (GMRES.hpp:18-180): variant and restarting parameters via setup()
-template< typename Matrix >
-class GMRES : public LinearSolver< Matrix >
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class GMRES< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   bool setup( const Config::ParameterContainer& p, const String& prefix = "" ) override
    {
       if( p.checkParameter( prefix + "gmres-variant" ) ) {
-         const std::string v = p.getParameter< std::string >( prefix + "gmres-variant" );
+         const String v = p.getParameter< String >( prefix + "gmres-variant" );
          const std::map< std::string, tnlmi_gmres_variant > names{ { "CGS", TNLMI_GMRES_CGS },
                                                                    { "CGSR", TNLMI_GMRES_CGSR },
                                                                    { "MGS", TNLMI_GMRES_MGS },
@@ -991,15 +1273,25 @@ public:
       get( "gmres-restarting-max", rmax_ );
       get( "gmres-restarting-step-min", dmin_ );
       get( "gmres-restarting-step-max", dmax_ );
-      return LinearSolver< Matrix >::setup( p, prefix );
+      return Base::setup( p, prefix );
    }
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "GMRES" );
+      // GMRES.hpp:78-89
+      if( rmin_ <= 0 || rmax_ <= 0 || rmin_ > rmax_ ) {
+         std::cerr << "Wrong value for the GMRES restarting parameters: r_min = " << rmin_ << ", r_max = " << rmax_
+                   << '\n';
+         return false;
+      }
+      if( dmin_ < 0 || dmax_ < 0 || dmin_ > dmax_ ) {
+         std::cerr << "Wrong value for the GMRES restarting adjustment parameters: d_min = " << dmin_
+                   << ", d_max = " << dmax_ << '\n';
+         return false;
+      }
       tnlmi_solver_state st = this->state();
-      detail::check( tnlmi_gmres_solve( &m, nullptr, this->pre(), variant_, rmin_, rmax_, dmin_, dmax_, b.getData(),
-                                        x.getData(), &st, nullptr ) );
+      MI355XDetail::check( tnlmi_gmres_solve( &m, nullptr, this->pre(), variant_, rmin_, rmax_, dmin_, dmax_,
+                                                b.getData(), x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 
@@ -1009,23 +1301,26 @@ private:
 };
 
 //! BICGStab (BICGStab.hpp:12-127): bicgstab-exact-residue via setup()
-template< typename Matrix >
-class BICGStab : public LinearSolver< Matrix >
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class BICGStab< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   bool setup( const Config::ParameterContainer& p, const String& prefix = "" ) override
    {
       if( p.checkParameter( prefix + "bicgstab-exact-residue" ) )
          exact_residue_ = p.getParameter< bool >( prefix + "bicgstab-exact-residue" );
-      return LinearSolver< Matrix >::setup( p, prefix );
+      return Base::setup( p, prefix );
    }
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "BICGStab" );
       tnlmi_solver_state st = this->state();
-      detail::check( tnlmi_bicgstab_solve( &m, nullptr, this->pre(), exact_residue_ ? 1 : 0, b.getData(), x.getData(),
-                                           &st, nullptr ) );
+      MI355XDetail::check( tnlmi_bicgstab_solve( &m, nullptr, this->pre(), exact_residue_ ? 1 : 0, b.getData(),
+                                                   x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 
@@ -1034,40 +1329,47 @@ private:
 };
 
 //! TFQMR (TFQMR.hpp:12-121)
-template< typename Matrix >
-class TFQMR : public LinearSolver< Matrix >
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class TFQMR< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "TFQMR" );
       tnlmi_solver_state st = this->state();
-      detail::check( tnlmi_tfqmr_solve( &m, nullptr, this->pre(), b.getData(), x.getData(), &st, nullptr ) );
+      MI355XDetail::check(
+         tnlmi_tfqmr_solve( &m, nullptr, this->pre(), b.getData(), x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 };
 
 //! BICGStabL (BICGStabL.hpp:12-257): bicgstab-ell and bicgstab-exact-residue via setup()
-template< typename Matrix >
-class BICGStabL : public LinearSolver< Matrix >
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class BICGStabL< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   bool setup( const Config::ParameterContainer& p, const String& prefix = "" ) override
    {
       if( p.checkParameter( prefix + "bicgstab-ell" ) )
          ell_ = p.getParameter< int >( prefix + "bicgstab-ell" );
       if( p.checkParameter( prefix + "bicgstab-exact-residue" ) )
          exact_residue_ = p.getParameter< bool >( prefix + "bicgstab-exact-residue" );
-      return LinearSolver< Matrix >::setup( p, prefix );
+      return Base::setup( p, prefix );
    }
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "BICGStabL" );
       tnlmi_solver_state st = this->state();
-      detail::check( tnlmi_bicgstabl_solve( &m, nullptr, this->pre(), ell_, exact_residue_ ? 1 : 0, b.getData(),
-                                            x.getData(), &st, nullptr ) );
+      MI355XDetail::check( tnlmi_bicgstabl_solve( &m, nullptr, this->pre(), ell_, exact_residue_ ? 1 : 0,
+                                                    b.getData(), x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 
@@ -1077,27 +1379,30 @@ private:
 };
 
 //! IDRs (IDRs.hpp:12-386): idr-s and idr-residual-smoothing via setup() or the setters
-template< typename Matrix >
-class IDRs : public LinearSolver< Matrix >
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class IDRs< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   bool setup( const Config::ParameterContainer& p, const String& prefix = "" ) override
    {
       if( p.checkParameter( prefix + "idr-s" ) )
          s_ = p.getParameter< int >( prefix + "idr-s" );
       if( p.checkParameter( prefix + "idr-residual-smoothing" ) )
          smoothing_ = p.getParameter< bool >( prefix + "idr-residual-smoothing" );
-      return LinearSolver< Matrix >::setup( p, prefix );
+      return Base::setup( p, prefix );
    }
    void setShadowSpaceDimension( int s ) { s_ = s; }
    void setResidualSmoothing( bool smoothing ) { smoothing_ = smoothing; }
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "IDRs" );
       tnlmi_solver_state st = this->state();
-      detail::check(
-         tnlmi_idrs_solve( &m, nullptr, this->pre(), s_, smoothing_ ? 1 : 0, b.getData(), x.getData(), &st, nullptr ) );
+      MI355XDetail::check( tnlmi_idrs_solve( &m, nullptr, this->pre(), s_, smoothing_ ? 1 : 0, b.getData(),
+                                               x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 
@@ -1106,39 +1411,49 @@ private:
    bool smoothing_ = false;  // IDRs.h default
 };
 
-//! Jacobi (Jacobi.hpp:65-125)
-template< typename Matrix >
-class Jacobi : public LinearSolver< Matrix >
+//! Jacobi (Jacobi.hpp:65-125): jacobi-omega and residue-period via setup() or the setters
+template< TNLMI_SPARSE_MATRIX_PARAMS >
+class Jacobi< TNLMI_SPARSE_MATRIX > : public MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >
 {
+   using Base = MI355XDetail::DeviceSolver< TNLMI_SPARSE_MATRIX >;
+
 public:
-   void setOmega( double w ) { omega_ = w; }
-   void setResiduePeriod( long p ) { period_ = p; }
-   bool solve( const typename LinearSolver< Matrix >::VectorType& b,
-               typename LinearSolver< Matrix >::VectorType& x ) override
+   using typename Base::ConstVectorViewType;
+   using typename Base::VectorViewType;
+   using RealType = typename Base::RealType;
+   using IndexType = typename Base::IndexType;
+   bool setup( const Config::ParameterContainer& p, const String& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "jacobi-omega" ) )
+         setOmega( p.getParameter< double >( prefix + "jacobi-omega" ) );
+      if( omega_ <= 0.0 || omega_ > 2.0 )
+         std::cerr << "Warning: The Jacobi method parameter omega is out of interval (0,2). The value is " << omega_
+                   << " the method will not converge.\n";
+      if( p.checkParameter( prefix + "residue-period" ) )
+         setResiduePeriod( p.getParameter< int >( prefix + "residue-period" ) );
+      return Base::setup( p, prefix );
+   }
+   void setOmega( RealType omega ) { omega_ = omega; }
+   RealType getOmega() const { return omega_; }
+   void setResiduePeriod( IndexType period ) { period_ = period; }
+   IndexType getResiduePerid() const { return period_; }
+   bool solve( ConstVectorViewType b, VectorViewType x ) override
    {
       const tnlmi_matrix m = this->mat( "Jacobi" );
       tnlmi_solver_state st = this->state();
-      detail::check( tnlmi_jacobi_solve( &m, nullptr, omega_, period_, b.getData(), x.getData(), &st, nullptr ) );
+      MI355XDetail::check(
+         tnlmi_jacobi_solve( &m, nullptr, omega_, period_, b.getData(), x.getData(), &st, nullptr ) );
       return this->finish( st );
    }
 
 private:
-   double omega_ = 1.0;
-   long period_ = 4;
-};
-
-//! LinearResidueGetter (Utils/LinearResidueGetter.hpp:11-23)
-struct LinearResidueGetter
-{
-   template< typename Matrix, typename Vector >
-   static double getResidue( const Matrix& A, const Vector& x, const Vector& b, double = 0.0 )
-   {
-      const tnlmi_matrix m = A.abi();
-      double r = 0;
-      detail::check( tnlmi_true_residual( &m, x.getData(), b.getData(), &r, nullptr ) );
-      return r;
-   }
+   RealType omega_ = 1.0;  // Jacobi.h:121-123
+   IndexType period_ = 4;
 };
 
 }  // namespace Solvers::Linear
+
+#undef TNLMI_SPARSE_MATRIX
+#undef TNLMI_SPARSE_MATRIX_PARAMS
+
 }  // namespace TNL

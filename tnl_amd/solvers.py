@@ -130,6 +130,26 @@ class CG(IterativeSolver):
         return self._finish(st)
 
 
+class TrueResidualCG(IterativeSolver):
+    """CG's iteration (no or the Diagonal preconditioner) stopped on the true
+    residual ||b - A x|| / ||b|| -- the stopping rule of the mixed-precision
+    solvers -- instead of the reference's preconditioned residue
+    (tnlmi_cg_solve_true_residual).  column_codes: read a column-major
+    Ellpack / SlicedEllpack matrix through 8-bit column codes, as the mixed
+    solver reads its inner matrix."""
+
+    def __init__(self, column_codes=False):
+        super().__init__()
+        self.column_codes = bool(column_codes)
+
+    def solve(self, b, x):
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_cg_solve_true_residual(m, plan, pre, int(self.column_codes), _f64(b), _f64(x),
+                                                           ctypes.byref(st), _stream()))
+        return self._finish(st)
+
+
 class GMRES(IterativeSolver):
     def __init__(self):
         super().__init__()
