@@ -440,6 +440,22 @@ tnlmi_status tnlmi_gmres_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* pl
                                 tnlmi_gmres_variant variant, int32_t restarting_min, int32_t restarting_max,
                                 int32_t step_min, int32_t step_max, const double* d_b, double* d_x,
                                 tnlmi_solver_state* state, tnlmi_stream stream );
+/* Mixed-precision GMRES: tnlmi_gmres_solve's iteration (same variants,
+ * adaptive restarting, preconditioner M, stopping rule on the preconditioned
+ * residue) whose Arnoldi products use A_lo -- the same matrix with FP32 or
+ * BF16 values (FP64 vectors and accumulation) -- while every restart residual
+ * M^-1 (b - A x) is computed with A in FP64: iterative refinement with one
+ * GMRES cycle per correction.  The Krylov basis, the Hessenberg least squares
+ * (Givens rotations, back substitution) and x stay FP64 on the GPU.  When a
+ * cycle's estimate meets the rule, the FP64 residue of the updated x decides
+ * whether the solve stops.  A holds FP64 values.  Synchronous.  Reduces to
+ * GMRES.hpp:71-806 when A_lo = A; no reference counterpart otherwise (TNL has
+ * no BF16 type and no mixed-precision solver). */
+tnlmi_status tnlmi_gmres_mixed_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const tnlmi_spmv_plan* plan,
+                                      const tnlmi_preconditioner* M, tnlmi_gmres_variant variant,
+                                      int32_t restarting_min, int32_t restarting_max, int32_t step_min,
+                                      int32_t step_max, const double* d_b, double* d_x, tnlmi_solver_state* state,
+                                      tnlmi_stream stream );
 tnlmi_status tnlmi_jacobi_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, double omega,
                                  int64_t residue_period, const double* d_b, double* d_x, tnlmi_solver_state* state,
                                  tnlmi_stream stream );

@@ -28,6 +28,7 @@
 #include <TNL/Solvers/Linear/Preconditioners/ILU0.h>
 #include <TNL/Solvers/Linear/Utils/LinearResidueGetter.h>
 #include <TNL/Matrices/MatrixReader.h>
+#include <TNL/Matrices/SparseOperations.h>
 
 #include <algorithm>
 #include <chrono>
@@ -207,6 +208,37 @@ struct SpmvOp
    {
       spmv_impl< Mat< S > >( n, ncols, rp, c, v, x, y, a, b, be, en );
       return 0;
+   }
+};
+
+// copySparseMatrix( dst, src ) (SparseOperations.hpp:600-690) from a source of
+// format S built from the CSR arrays into a target of the dispatched format;
+// the target's storage arrays as fill_impl exports them
+template< template< class, class, class > class S >
+struct CopyOp
+{
+   template< template< class, class, class > class T >
+   struct Into
+   {
+      static int64_t run( const Mat< S >* src, int32_t* oc, double* ov, int64_t cap )
+      {
+         Mat< T > dst;
+         Matrices::copySparseMatrix( dst, *src );
+         const int64_t storage = dst.getValues().getSize();
+         if( storage <= cap )
+            for( int64_t i = 0; i < storage; i++ ) {
+               oc[ i ] = dst.getColumnIndexes()[ i ];
+               ov[ i ] = dst.getValues()[ i ];
+            }
+         return storage;
+      }
+   };
+   static int64_t run( int64_t n, int64_t ncols, const int32_t* rp, const int32_t* c, const double* v, int dst_format,
+                       int dst_col_major, int32_t* oc, double* ov, int64_t cap )
+   {
+      Mat< S > src;
+      build( src, n, ncols, rp, c, v );
+      return dispatch< CopyOp< S >::template Into >( dst_format, dst_col_major, &src, oc, ov, cap );
    }
 };
 
@@ -545,6 +577,17 @@ int64_t ref_fill( int format, int col_major, int64_t n, int64_t ncols, const int
                   const double* vals, int32_t* out_cols, double* out_vals, int64_t cap )
 {
    return dispatch< FillOp >( format, col_major, n, ncols, row_ptr, cols, vals, out_cols, out_vals, cap );
+}
+
+// copySparseMatrix from a matrix of format src_format (built from the CSR
+// arrays) into one of format dst_format: the target's storage size; its
+// column indexes and values when they fit cap
+int64_t ref_copy( int src_format, int src_col_major, int dst_format, int dst_col_major, int64_t n, int64_t ncols,
+                  const int32_t* row_ptr, const int32_t* cols, const double* vals, int32_t* out_cols,
+                  double* out_vals, int64_t cap )
+{
+   return dispatch< CopyOp >( src_format, src_col_major, n, ncols, row_ptr, cols, vals, dst_format, dst_col_major,
+                              out_cols, out_vals, cap );
 }
 
 int64_t ref_spmv( int format, int col_major, int64_t n, int64_t ncols, const int32_t* row_ptr, const int32_t* cols,

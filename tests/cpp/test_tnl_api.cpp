@@ -326,7 +326,12 @@ int main()
       EXPECT_EQ( y == z, true );
       EXPECT_EQ( y == w, true );
       EXPECT_EQ( Cm.getElement( 2, 3 ), -1.0 );
-      EXPECT_EQ( Am.getRowCapacity( 0 ), 2 );
+      // the chain's row lengths count each padded segment's slots (SparseOperations.hpp:
+      // 544-597 via row.getSize()): TNL's own Host chain gives row 0 a capacity of 55 too
+      // (3 in BiEllpack, then 55 in ChunkedEllpack, kept by the copy back to CSR)
+      EXPECT_EQ( Bm.getRowCapacity( 0 ), 3 );
+      EXPECT_EQ( Cm.getRowCapacity( 0 ), 55 );
+      EXPECT_EQ( Am.getRowCapacity( 0 ), 55 );
       std::cout << "copySparseMatrix CSR -> BiEllpack -> ChunkedEllpack -> CSR done\n";
    }
 
@@ -370,8 +375,10 @@ int main()
       Lm.load( path );
       EXPECT_EQ( Lm.getRows(), 5 );
       EXPECT_EQ( Lm.getElement( 2, 3 ), -1.0 );
-      EXPECT_EQ( BMatrix::getSerializationType().find( "BiEllpack< int, ColumnMajorOrder, 32 >" ) != std::string::npos,
-                 true );
+      const std::string type = BMatrix::getSerializationType();
+      if( type.find( "BiEllpack< int, ColumnMajorOrder, 32 >" ) == std::string::npos )
+         std::cerr << "serialization type: " << type << "\n";
+      EXPECT_EQ( type.find( "BiEllpack< int, ColumnMajorOrder, 32 >" ) != std::string::npos, true );
       Vector x( 5 ), y( 5 ), z( 5 );
       for( int i = 0; i < 5; i++ )
          x.setElement( i, 1.0 + i );

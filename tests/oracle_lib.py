@@ -282,7 +282,7 @@ class Reference:
         r.L = ctypes.CDLL(REF_SO)
         for name in ["ref_chunked_ellpack", "ref_adaptive_csr_blocks", "ref_fill", "ref_spmv", "ref_save",
                      "ref_load", "ref_spmv_symmetric", "ref_spmv_transposed", "ref_save_symmetric",
-                     "ref_add_elements", "ref_row_counts"]:
+                     "ref_add_elements", "ref_row_counts", "ref_copy"]:
             getattr(r.L, name).restype = ctypes.c_int64
         r.L.ref_last_error.restype = ctypes.c_char_p
         r.L.ref_spmv_csr_time.restype = ctypes.c_double
@@ -356,6 +356,19 @@ class Reference:
         oc, ov = np.zeros(cap, np.int32), np.zeros(cap)
         self.L.ref_fill(self.FORMATS[fmt], int(col_major), I64(n), I64(ncols), P(off), P(cols), P(vals), P(oc),
                         P(ov), I64(cap))
+        return oc, ov
+
+    def copy(self, src_fmt, dst_fmt, off, cols, vals, ncols, src_col_major=True, dst_col_major=True):
+        """TNL's copySparseMatrix (SparseOperations.hpp:600-690) from a matrix of
+        format src_fmt built from the CSR arrays into one of format dst_fmt: the
+        target's column indexes and values."""
+        off, cols, vals = i32(off), i32(cols), f64(vals)
+        n = len(off) - 1
+        args = (self.FORMATS[src_fmt], int(src_col_major), self.FORMATS[dst_fmt], int(dst_col_major), I64(n),
+                I64(ncols), P(off), P(cols), P(vals))
+        cap = int(self.L.ref_copy(*args, None, None, I64(0)))
+        oc, ov = np.zeros(cap, np.int32), np.zeros(cap)
+        self.L.ref_copy(*args, P(oc), P(ov), I64(cap))
         return oc, ov
 
     def add_elements(self, fmt, off, ncols, ops, symmetric=False, col_major=True):

@@ -45,7 +45,8 @@ def test_scanner_flags_divergent_barrier_loop():
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-@pytest.mark.parametrize("src", ["batched.hip", "blas1.hip", "ir.hip", "fused_cg.hip", "spai.hip"])
+@pytest.mark.parametrize("src", ["batched.hip", "blas1.hip", "ir.hip", "fused_cg.hip", "spai.hip", "krylov.hip",
+                                 "spmv_window.hip"])
 def test_kernels_have_uniform_barrier_loops(tmp_path, src):
     out = tmp_path / (src + ".s")
     cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",

@@ -1,8 +1,11 @@
 """copySparseMatrix on the GPU (tnlmi_matrix_row_lengths / tnlmi_matrix_copy,
 SparseMatrix.convert): converting a matrix to another format gives exactly the
-storage TNL builds for that format from the same elements (oracle/_ref fill:
-SparseMatrix::setRowCapacities + setElement), and a chain of conversions back
-to CSR gives the original arrays."""
+storage TNL's own copySparseMatrix builds from the same source (oracle/_ref
+ref_copy).  As in the reference, a row's length is its leading slots with a
+column below the column count over the whole segment (getSegmentSize), so
+the padding slots of a padded source format (column -1) are copied as slots
+of the target; a chain of conversions back to CSR keeps every row's entries
+in order, followed only by such padding slots."""
 import ctypes
 import os
 import sys
@@ -34,10 +37,15 @@ def test_convert_matches_tnl_storage(fmt, param, name, org):
     rng = np.random.default_rng(17)
     n, m = 700, 900
     off, cols, vals = random_csr(n, m, 40, rng, pad=0.3, long_rows=[(5, 600), (699, 300)])
-    A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, fmt=tnl.BIELLPACK, param=32)  # any source format
-    assert np.array_equal(A.rowLengths().cpu().numpy(), np.diff(off))
+    A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals, fmt=tnl.BIELLPACK, param=32)  # a padded source format
+    # every slot of a BiEllpack segment (entries and padding, column -1) is below the column count:
+    # the copy's row lengths are the segment sizes, TNL's getRowCapacity
+    seg_sizes = ref.row_counts("BiEllpack", off, cols, vals, m)[0]
+    assert np.array_equal(A.rowLengths().cpu().numpy(), seg_sizes)
+    assert np.all(seg_sizes >= np.diff(off))
     B = A.convert(fmt, param, org)
-    want_c, want_v = ref.fill(name, off, cols, vals, m, col_major=org == tnl.COLUMN_MAJOR)
+    want_c, want_v = ref.copy("BiEllpack", name, off, cols, vals, m, src_col_major=True,
+                              dst_col_major=org == tnl.COLUMN_MAJOR)
     assert np.array_equal(B.column_indexes.cpu().numpy()[:len(want_c)], want_c)
     assert np.array_equal(B.values.cpu().numpy()[:len(want_v)], want_v)
     x = rng.standard_normal(m)
@@ -49,18 +57,38 @@ def test_convert_matches_tnl_storage(fmt, param, name, org):
 
 
 def test_convert_chain_back_to_csr():
+    ref = OL.Reference.load()
     rng = np.random.default_rng(18)
     n, m = 1000, 1000
     off, cols, vals = random_csr(n, m, 30, rng, pad=0.2, long_rows=[(0, 900)])
     A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals)
     chain = A
-    for fmt, param, org in [(tnl.BIELLPACK, 32, tnl.ROW_MAJOR), (tnl.CHUNKED_ELLPACK, 0, tnl.COLUMN_MAJOR),
-                            (tnl.SLICED_ELLPACK, 32, tnl.ROW_MAJOR), (tnl.SORTED_CSR, 0, tnl.COLUMN_MAJOR),
-                            (tnl.ELLPACK, 0, tnl.COLUMN_MAJOR), (tnl.CSR, 0, tnl.COLUMN_MAJOR)]:
+    hops = [(tnl.BIELLPACK, 32, tnl.ROW_MAJOR), (tnl.CHUNKED_ELLPACK, 0, tnl.COLUMN_MAJOR),
+            (tnl.SLICED_ELLPACK, 32, tnl.ROW_MAJOR), (tnl.SORTED_CSR, 0, tnl.COLUMN_MAJOR),
+            (tnl.ELLPACK, 0, tnl.COLUMN_MAJOR), (tnl.CSR, 0, tnl.COLUMN_MAJOR)]
+    for i, (fmt, param, org) in enumerate(hops):
         chain = chain.convert(fmt, param, org)
-    assert np.array_equal(chain.segments.array("offsets"), off)
-    assert np.array_equal(chain.column_indexes.cpu().numpy()[:len(cols)], cols)
-    assert np.array_equal(chain.values.cpu().numpy()[:len(vals)], vals)
+        if i == 0 and ref is not None:  # the first hop, from CSR arrays, exactly as TNL's copySparseMatrix
+            want_c, want_v = ref.copy("CSR", "BiEllpack", off, cols, vals, m, dst_col_major=False)
+            assert np.array_equal(chain.column_indexes.cpu().numpy()[:len(want_c)], want_c)
+            assert np.array_equal(chain.values.cpu().numpy()[:len(want_v)], want_v)
+    # every row: its entries in their order, then only padding slots (column -1, value 0)
+    off2 = chain.segments.array("offsets")
+    c2, v2 = chain.column_indexes.cpu().numpy(), chain.values.cpu().numpy()
+    for r in range(n):
+        a, b = off[r], off[r + 1]
+        real = cols[a:b] >= 0
+        k = int(real.sum())
+        row_c, row_v = c2[off2[r]:off2[r + 1]], v2[off2[r]:off2[r + 1]]
+        assert len(row_c) >= b - a
+        assert np.array_equal(row_c[:k], cols[a:b][real]) and np.array_equal(row_v[:k], vals[a:b][real])
+        assert np.all(row_c[k:] == -1) and np.all(row_v[k:] == 0.0)
+    x = rng.standard_normal(m)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    chain.vectorProduct(torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    want = OL.Oracle().spmv(off, cols, vals, x)
+    assert np.abs(y.cpu().numpy() - want).max() <= 1e-12 * np.abs(want).max()
 
 
 def test_convert_bf16_and_errors():
@@ -69,8 +97,14 @@ def test_convert_bf16_and_errors():
     A = tnl.SparseMatrix.from_csr(200, 300, off, cols, vals, value_dtype=torch.bfloat16)
     B = A.convert(tnl.ELLPACK)
     C = B.convert(tnl.CSR)
-    assert np.array_equal(C.values[:len(vals)].float().cpu().numpy(),
-                          A.values[:len(vals)].float().cpu().numpy())
+    # every Ellpack row has the full width: the copy back to CSR keeps its padding slots, as TNL's
+    # copySparseMatrix does (the BF16 values are exact in FP64, so TNL's FP64 chain is the expectation)
+    ref = OL.Reference.load()
+    if ref is not None:
+        vb = A.values[:len(vals)].double().cpu().numpy()
+        want_c, want_v = ref.copy("Ellpack", "CSR", off, cols, vb, 300)
+        assert np.array_equal(C.column_indexes.cpu().numpy()[:len(want_c)], want_c)
+        assert np.array_equal(C.values[:len(want_v)].double().cpu().numpy(), want_v)
     # a target whose segments are shorter than the rows is refused, nothing written past them
     L = A.rowLengths()
     short = L.clone()

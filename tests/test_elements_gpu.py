@@ -205,8 +205,9 @@ def test_slots_past_the_capacity_are_walked(fmt, param, org):
         B = A.convert(tnl.CSR)
         C = B.convert(fmt, param=param, organization=org)
         for M in (B, C):
+            # getElement of a symmetric matrix reads (c, r) above the diagonal (SparseMatrixBase.hpp:238-262)
             got = np.array([[M.getElement(r, c) for c in range(n)] for r in range(n)])
-            assert np.array_equal(got, D), symmetric
+            assert np.array_equal(got, full), symmetric
         P = solvers.Diagonal(A)
         b = torch.tensor([4.0, 10.0, 12.0], dtype=torch.float64, device="cuda")
         z = torch.zeros(n, dtype=torch.float64, device="cuda")

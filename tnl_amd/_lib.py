@@ -144,6 +144,9 @@ _PROTOS = {
     "tnlmi_gmres_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int, ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_vp, c_vp,
                                          ctypes.POINTER(SolverState), c_vp]),
+    "tnlmi_gmres_mixed_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp,
+                                               ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, c_vp, c_vp, ctypes.POINTER(SolverState), c_vp]),
     "tnlmi_bicgstab_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int32, c_vp, c_vp,
                                             ctypes.POINTER(SolverState), c_vp]),
     "tnlmi_bicgstabl_solve": (ctypes.c_int, [ctypes.POINTER(Matrix), c_vp, c_vp, ctypes.c_int32, ctypes.c_int32, c_vp,

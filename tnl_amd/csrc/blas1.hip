@@ -59,11 +59,33 @@ struct SqF
 
 constexpr int kMaxCols = 32;
 
+// A wave's 64 lanes hold kMaxCols partial sums each; the wave totals of all
+// columns come out of one butterfly that halves the columns a lane carries at
+// every step (16 + 8 + 4 + 2 + 1 exchanges, then 1 for the last pair) instead
+// of a full 6-step reduction per column: lane l ends with column l >> 1.
+__device__ __forceinline__ double wave_sum_columns( double ( &acc )[ kMaxCols ] )
+{
+   const int lane = threadIdx.x & 63;
+#pragma unroll
+   for( int o = 32, m = kMaxCols / 2; o >= 2; o /= 2, m /= 2 ) {
+      const bool upper = lane & o;
+#pragma unroll
+      for( int j = 0; j < m; j++ ) {
+         // keep one half of the columns, send the other to the partner lane
+         const double keep = upper ? acc[ j + m ] : acc[ j ];
+         const double send = upper ? acc[ j ] : acc[ j + m ];
+         acc[ j ] = keep + __shfl_xor( send, o, 64 );
+      }
+   }
+   return acc[ 0 ] + __shfl_xor( acc[ 0 ], 1, 64 );
+}
+
 __global__ void __launch_bounds__( 256 ) multi_dot_pass1( int64_t n, int cols, const double* __restrict__ V,
                                                           int64_t ld, const double* __restrict__ w,
                                                           double* __restrict__ part )
 {
-   __shared__ double sm[ 4 ];
+   static_assert( kMaxCols == 32, "the butterfly of wave_sum_columns ends with one column per lane pair" );
+   __shared__ double sm[ 4 ][ kMaxCols ];
    double acc[ kMaxCols ];
 #pragma unroll
    for( int k = 0; k < kMaxCols; k++ )
@@ -76,22 +98,31 @@ __global__ void __launch_bounds__( 256 ) multi_dot_pass1( int64_t n, int cols, c
          if( k < cols )
             acc[ k ] += V[ i + k * ld ] * wi;
    }
-   for( int k = 0; k < cols; k++ ) {
-      const double v = block_sum< 256 >( acc[ k ], sm );
-      if( threadIdx.x == 0 )
-         part[ (int64_t) k * kRedBlocks + blockIdx.x ] = v;
+   // every column's wave total at once, then one barrier and a fixed-order
+   // sum of the four waves' totals
+   const double t = wave_sum_columns( acc );
+   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+   if( ( lane & 1 ) == 0 )
+      sm[ wv ][ lane >> 1 ] = t;
+   __syncthreads();
+   if( (int) threadIdx.x < cols ) {
+      const int k = threadIdx.x;
+      part[ (int64_t) k * kRedBlocks + blockIdx.x ] = ( ( sm[ 0 ][ k ] + sm[ 1 ][ k ] ) + sm[ 2 ][ k ] ) + sm[ 3 ][ k ];
    }
 }
 
+// out[k] = sum of the np partials of column k: wave k % 4 of the block sums
+// columns k = wave, wave + 4, ... (lanes stride the partials, then a wave
+// butterfly), so the columns proceed in parallel with no barrier
 __global__ void __launch_bounds__( 256 ) multi_dot_pass2( const double* part, int np, int cols, double* out )
 {
-   __shared__ double sm[ 4 ];
-   for( int k = 0; k < cols; k++ ) {
+   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+   for( int k = wv; k < cols; k += 4 ) {
       double a = 0;
-      for( int i = threadIdx.x; i < np; i += 256 )
+      for( int i = lane; i < np; i += 64 )
          a += part[ (int64_t) k * kRedBlocks + i ];
-      a = block_sum< 256 >( a, sm );
-      if( threadIdx.x == 0 )
+      a = group_sum< 64 >( a );
+      if( lane == 0 )
          out[ k ] = a;
    }
 }

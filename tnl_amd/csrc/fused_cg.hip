@@ -38,6 +38,7 @@
 #include "krylov.hpp"
 #include "segments.hpp"
 #include "spmv.hpp"
+#include "spmv_tile.hpp"
 
 namespace tnlmi {
 namespace {
@@ -177,6 +178,41 @@ __global__ void __launch_bounds__( kFT ) spmv_dot_ell( SegView v, const int32_t*
             acc += (double) y[ j ] * (double) p[ r0 + j ];
          }
    }
+   const double bs = block_sum< kFT >( acc, sm );
+   if( threadIdx.x < 64 )
+      part[ blk ] = bs;
+}
+
+// CSR with no row longer than kWholeRowMax (stencil matrices): the tiled CSR
+// product in its one-launch form (each wavefront streams its TILE elements,
+// every row is summed whole by its owner lane, left to right from 0) with the
+// (Ap, p) partial sums in its epilogue: each lane adds v * p[r] for the rows
+// it stores, one partial per workgroup.
+template< typename V, typename T, int TILE >
+__global__ void __launch_bounds__( kFT ) spmv_dot_csr( const int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ tile_rows, int64_t tiles,
+                                                       int64_t nnz, int64_t n, const int32_t* __restrict__ cols,
+                                                       const V* __restrict__ vals, const T* __restrict__ p,
+                                                       T* __restrict__ Ap, double* __restrict__ part,
+                                                       const CgDev* __restrict__ st, int32_t tail_max )
+{
+   static_assert( kFT == 256, "four tile wavefronts per workgroup" );
+   __shared__ T lds[ 4 ][ TILE + kWholeRowMax ];
+   __shared__ double sm[ kFT / 64 ];
+   if( ! st->active )
+      return;
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int64_t blk = xcd_contiguous( blockIdx.x, gridDim.x );
+   const int64_t t = blk * 4 + wave;
+   double acc = 0;
+   if( t < tiles )
+      csr_tile< V, T, T, true, TILE >(
+         off, n, cols, vals, p, t, tile_rows[ t ], tile_rows[ t + 1 ], nnz, lds[ wave ], lane,
+         [ & ]( int64_t r, T v ) {
+            Ap[ r ] = v;
+            acc += (double) v * (double) p[ r ];
+         },
+         []( int64_t, T ) {}, tail_max );
    const double bs = block_sum< kFT >( acc, sm );
    if( threadIdx.x < 64 )
       part[ blk ] = bs;
@@ -675,7 +711,8 @@ struct Stage
    const tnlmi_matrix* A = nullptr;
    const tnlmi_spmv_plan* plan = nullptr;
    SegView v{};
-   int kind = 0;  // 0: the format's own SpMV kernel + dot_part, 1: fused Ellpack/SlicedEllpack column-major
+   int kind = 0;  // 0: the format's own SpMV kernel + dot_part, 1: fused Ellpack/SlicedEllpack column-major,
+                  // 2: fused one-launch tiled CSR (no row longer than kWholeRowMax)
    int64_t grid = 0;                      // blocks of the fused SpMV kernel
    int rows_per_thread = 1;               // Ellpack / SlicedEllpack kernel
    std::shared_ptr< DeviceArray< double > > part;  // its per-block partial sums
@@ -764,6 +801,14 @@ Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_
       }
       return S;
    }
+   const tnlmi_segments& seg = *A->segments;
+   if( fmt == TNLMI_CSR && A->matrix_type != TNLMI_SYMMETRIC && seg.max_size <= kWholeRowMax && seg.tiles > 0
+       && ( seg.tile_nnz == kTileNnz || seg.tile_nnz == kTileNnzSmall ) ) {
+      S.kind = 2;
+      S.grid = ceil_div( seg.tiles, 4 );
+      S.part = std::make_shared< DeviceArray< double > >();
+      S.part->allocate( S.grid );
+   }
    return S;
 }
 
@@ -784,6 +829,18 @@ void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDe
       else
          hipLaunchKernelGGL( ( spmv_dot_ell_code< 1, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v, c, d,
                              S.ndict, vals, p, Ap, bp, st );
+   }
+   else if( S.kind == 2 ) {
+      const tnlmi_segments& seg = *S.A->segments;
+      const int32_t tail_max = std::max( 0, seg.max_size - 1 );
+      if( seg.tile_nnz == kTileNnzSmall )
+         hipLaunchKernelGGL( ( spmv_dot_csr< V, T, kTileNnzSmall > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s,
+                             seg.offsets.ptr, seg.tile_rows.ptr, seg.tiles, seg.elements, S.v.n, S.A->column_indexes,
+                             vals, p, Ap, bp, st, tail_max );
+      else
+         hipLaunchKernelGGL( ( spmv_dot_csr< V, T, kTileNnz > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s,
+                             seg.offsets.ptr, seg.tile_rows.ptr, seg.tiles, seg.elements, S.v.n, S.A->column_indexes,
+                             vals, p, Ap, bp, st, tail_max );
    }
    else if( S.kind == 1 ) {
       if( S.rows_per_thread == 4 )

@@ -31,6 +31,16 @@ struct tnlmi_preconditioner
    DeviceArray< int32_t > lp, lc, up, uc, lrows, urows;
    DeviceArray< double > lv, uv;
    std::vector< int64_t > llev, ulev;  // level boundaries into lrows / urows
+   DeviceArray< int64_t > dllev, dulev;  // the same on the device (read by the one-workgroup solves)
+   // launch plan of each solve: runs [l0, l1) of consecutive levels; narrow
+   // runs (every level at most kLvThreads * kLvRows rows) in one one-workgroup
+   // launch, a wide level alone in a launch of its own
+   struct Run
+   {
+      int64_t l0, l1;
+      bool narrow;
+   };
+   std::vector< Run > lruns, uruns;
    tnlmi_matrix M{};                   // explicit: x = M b
 };
 
@@ -123,6 +133,103 @@ __global__ void upper_level( const int32_t* __restrict__ rows, int64_t nrows, in
    for( int32_t g = up[ k ] + 1; g < up[ k + 1 ]; g++ )
       xi -= uv[ g ] * x[ uc[ g ] ];
    x[ i ] = xi / uii;
+}
+
+// Consecutive narrow levels in one launch of one workgroup: the levels in
+// order, a barrier after each, so a level's rows read the solution entries of
+// earlier levels that other threads of the workgroup wrote (stores and the
+// loads of x are device-coherent: no L1 copy of x is trusted).  While a level
+// is being solved, each thread already loads its rows of the next level (row
+// index, entry range, the first kLvPre entries, the right-hand side), so after
+// a barrier only the loads of the x entries a row needs stand between the
+// levels.  Summation order per row as lower_level / upper_level.
+constexpr int kLvThreads = 1024;
+constexpr int kLvRows = 2;  // rows per thread and level at most
+constexpr int kLvPre = 4;   // entries per row loaded ahead of their level
+
+struct LvRow
+{
+   int32_t i, g0, g1;  // row (-1: none), entries [g0, g1) (off-diagonal)
+   int32_t c[ kLvPre ];
+   double v[ kLvPre ];
+   double bi, dii;  // right-hand side (lower) / current x (upper); pivot (upper)
+};
+
+__device__ __forceinline__ double coherent_load( const double* p )
+{
+   return __hip_atomic_load( p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
+__device__ __forceinline__ void coherent_store( double* p, double v )
+{
+   __hip_atomic_store( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
+
+template< bool UPPER >
+__device__ __forceinline__ void lv_load( LvRow ( &R )[ kLvRows ], const int32_t* __restrict__ rows, int64_t p0,
+                                         int64_t p1, int64_t N, const int32_t* __restrict__ ptr,
+                                         const int32_t* __restrict__ col, const double* __restrict__ val,
+                                         const double* b, const double* x )
+{
+#pragma unroll
+   for( int r = 0; r < kLvRows; r++ ) {
+      const int64_t p = p0 + threadIdx.x + (int64_t) r * kLvThreads;
+      LvRow& w = R[ r ];
+      w.i = -1;
+      if( p < p1 ) {
+         w.i = rows[ p ];
+         const int64_t k = UPPER ? N - 1 - w.i : w.i;
+         const int32_t a = ptr[ k ], e = ptr[ k + 1 ];
+         if( UPPER ) {
+            w.dii = a < e ? val[ a ] : 0.0;  // a row without entries has no pivot
+            w.g0 = a + 1;
+            w.bi = coherent_load( x + w.i );
+         }
+         else {
+            w.dii = 1.0;
+            w.g0 = a;
+            w.bi = b[ w.i ];
+         }
+         w.g1 = e;
+#pragma unroll
+         for( int q = 0; q < kLvPre; q++ )
+            if( w.g0 + q < w.g1 ) {
+               w.c[ q ] = col[ w.g0 + q ];
+               w.v[ q ] = val[ w.g0 + q ];
+            }
+      }
+   }
+}
+
+template< bool UPPER >
+__global__ void __launch_bounds__( kLvThreads )
+   tri_levels( const int32_t* __restrict__ rows, const int64_t* __restrict__ bounds, int64_t l0, int64_t l1,
+               int64_t N, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+               const double* __restrict__ val, const double* b, double* x )
+{
+   LvRow cur[ kLvRows ], nxt[ kLvRows ];
+   lv_load< UPPER >( cur, rows, bounds[ l0 ], bounds[ l0 + 1 ], N, ptr, col, val, b, x );
+   for( int64_t l = l0; l < l1; l++ ) {  // uniform bounds: every thread meets every barrier
+      if( l + 1 < l1 )
+         lv_load< UPPER >( nxt, rows, bounds[ l + 1 ], bounds[ l + 2 ], N, ptr, col, val, b, x );
+#pragma unroll
+      for( int r = 0; r < kLvRows; r++ ) {
+         const LvRow& w = cur[ r ];
+         if( w.i < 0 )
+            continue;
+         double xi = w.bi;
+#pragma unroll
+         for( int q = 0; q < kLvPre; q++ )
+            if( w.g0 + q < w.g1 )
+               xi -= w.v[ q ] * coherent_load( x + w.c[ q ] );
+         for( int32_t g = w.g0 + kLvPre; g < w.g1; g++ )
+            xi -= val[ g ] * coherent_load( x + col[ g ] );
+         coherent_store( x + w.i, UPPER ? xi / w.dii : xi );
+      }
+      __syncthreads();
+#pragma unroll
+      for( int r = 0; r < kLvRows; r++ )
+         cur[ r ] = nxt[ r ];
+   }
 }
 
 // rows grouped by level; level(i) = 1 + max level of the rows it depends on
@@ -265,6 +372,23 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
    std::vector< int32_t > lo, uo;
    levels( N, lp, lc, false, lo, P.llev );
    levels( N, up, uc, true, uo, P.ulev );
+   auto plan_runs = []( const std::vector< int64_t >& bounds, std::vector< tnlmi_preconditioner::Run >& runs ) {
+      runs.clear();
+      const int64_t L = (int64_t) bounds.size() - 1;
+      for( int64_t l = 0; l < L; ) {
+         const bool narrow = bounds[ l + 1 ] - bounds[ l ] <= (int64_t) kLvThreads * kLvRows;
+         int64_t e = l + 1;
+         if( narrow )
+            while( e < L && bounds[ e + 1 ] - bounds[ e ] <= (int64_t) kLvThreads * kLvRows )
+               e++;
+         runs.push_back( { l, e, narrow } );
+         l = e;
+      }
+   };
+   plan_runs( P.llev, P.lruns );
+   plan_runs( P.ulev, P.uruns );
+   upload_vec( P.dllev, P.llev, s );
+   upload_vec( P.dulev, P.ulev, s );
    upload_vec( P.lp, lp, s );
    upload_vec( P.lc, lc, s );
    upload_vec( P.lv, lv, s );
@@ -294,17 +418,25 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
       return;
    }
    // ILU0.hpp:99-113: triangularSolveLower then triangularSolveUpper<reversed rows>
-   for( size_t l = 0; l + 1 < P->llev.size(); l++ ) {
-      const int64_t cnt = P->llev[ l + 1 ] - P->llev[ l ];
-      if( cnt )
-         hipLaunchKernelGGL( lower_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
-                             P->lrows.ptr + P->llev[ l ], cnt, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
+   for( const auto& R : P->lruns ) {
+      if( R.narrow ) {
+         hipLaunchKernelGGL( tri_levels< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, P->dllev.ptr,
+                             R.l0, R.l1, n, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
+         continue;
+      }
+      const int64_t cnt = P->llev[ R.l0 + 1 ] - P->llev[ R.l0 ];
+      hipLaunchKernelGGL( lower_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
+                          P->lrows.ptr + P->llev[ R.l0 ], cnt, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
    }
-   for( size_t l = 0; l + 1 < P->ulev.size(); l++ ) {
-      const int64_t cnt = P->ulev[ l + 1 ] - P->ulev[ l ];
-      if( cnt )
-         hipLaunchKernelGGL( upper_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
-                             P->urows.ptr + P->ulev[ l ], cnt, n, P->up.ptr, P->uc.ptr, P->uv.ptr, x );
+   for( const auto& R : P->uruns ) {
+      if( R.narrow ) {
+         hipLaunchKernelGGL( tri_levels< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, P->dulev.ptr,
+                             R.l0, R.l1, n, P->up.ptr, P->uc.ptr, P->uv.ptr, (const double*) x, x );
+         continue;
+      }
+      const int64_t cnt = P->ulev[ R.l0 + 1 ] - P->ulev[ R.l0 ];
+      hipLaunchKernelGGL( upper_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
+                          P->urows.ptr + P->ulev[ R.l0 ], cnt, n, P->up.ptr, P->uc.ptr, P->uv.ptr, x );
    }
 }
 
@@ -802,8 +934,15 @@ __global__ void cwy_add_y( double* __restrict__ x, const double* __restrict__ g,
       x[ j ] += g[ L.y + j ];
 }
 
+// GMRES.hpp:71-806.  inner == NULL: TNL's GMRES.  inner != NULL (the mixed-
+// precision variant): the Arnoldi products use *inner (the matrix with FP32 or
+// BF16 values) while every restart residual M^-1 (b - A x) uses A in FP64 --
+// iterative refinement with one GMRES cycle per correction; the orthogonal
+// basis, the Hessenberg least squares and x stay FP64.  When a cycle's
+// estimate meets the stopping rule, the FP64 residue of the updated x decides:
+// the solve stops only if that one meets it too.
 void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double* b, double* x, int variant, int rmin,
-                  int rmax, int dmin, int dmax, tnlmi_solver_state* st, hipStream_t s )
+                  int rmax, int dmin, int dmax, tnlmi_solver_state* st, hipStream_t s, const Operator* inner )
 {
    if( rmin <= 0 || rmax <= 0 || rmin > rmax || dmin < 0 || dmax < 0 || dmin > dmax ) {
       st->converged = 0;
@@ -826,13 +965,14 @@ void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double
    HostScalars hs;
    Stop S{ st };
    double* G = g.ptr;
+   const Operator& Ak = inner ? *inner : A;  // the Arnoldi operator
    auto matvec_prec = [ & ]( const double* v, double* out ) {  // preconditioned_matvec, :451-465
       if( pre ) {
-         A.apply( v, tmp.ptr, s );
+         Ak.apply( v, tmp.ptr, s );
          precond_apply( M, n, tmp.ptr, out, s );
       }
       else
-         A.apply( v, out, s );
+         Ak.apply( v, out, s );
    };
    auto compute_residue = [ & ]() {  // compute_residue, :433-449
       if( pre ) {
@@ -981,6 +1121,20 @@ void gmres_solve( const Operator& A, const tnlmi_preconditioner* M, const double
       }
       if( o < m ) {
          update( o );
+         if( inner ) {
+            // the estimate met the rule: the FP64 residue decides
+            compute_residue();
+            nrm2( n, r.ptr, G + L.sc + 1, rw, s );
+            hipLaunchKernelGGL( gm_residue_beta, dim3( 1 ), dim3( 1 ), 0, s, G, L );
+            const double* h = hs.read( G + L.sc, 4, s );
+            const double beta_old = beta;
+            beta = h[ 1 ];
+            S.res = h[ 3 ];
+            ++cycles;
+            beta_ratio = beta / beta_old;
+            if( S.check() )
+               continue;
+         }
          st->iterations = S.it;
          st->residue = S.res;
          st->converged = S.converged();
@@ -1145,7 +1299,31 @@ tnlmi_status tnlmi_gmres_solve( const tnlmi_matrix* A, const tnlmi_spmv_plan* pl
       check_state( st );
       const Operator op = make_op( A, plan );
       gmres_solve( op, M, d_b, d_x, variant, restarting_min, restarting_max, step_min, step_max, st,
-                   as_stream( stream ) );
+                   as_stream( stream ), nullptr );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+tnlmi_status tnlmi_gmres_mixed_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_lo, const tnlmi_spmv_plan* plan,
+                                      const tnlmi_preconditioner* M, tnlmi_gmres_variant variant,
+                                      int32_t restarting_min, int32_t restarting_max, int32_t step_min,
+                                      int32_t step_max, const double* d_b, double* d_x, tnlmi_solver_state* st,
+                                      tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      check_state( st );
+      if( ! A_lo || ! A_lo->segments || ! A || ! A->segments )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed GMRES: matrix is NULL" );
+      if( A_lo->rows != A->rows || A_lo->columns != A->columns )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed GMRES: the inner matrix has other dimensions" );
+      if( A_lo->value_type != TNLMI_F64 && A_lo->value_type != TNLMI_F32 && A_lo->value_type != TNLMI_BF16 )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed GMRES: the inner matrix holds FP64, FP32 or BF16 values" );
+      if( A->value_type != TNLMI_F64 )
+         fail( TNLMI_INVALID_ARGUMENT, "mixed GMRES: the matrix holds FP64 values" );
+      const Operator op = make_op( A, plan );
+      const Operator lo = make_op( A_lo, nullptr );
+      gmres_solve( op, M, d_b, d_x, variant, restarting_min, restarting_max, step_min, step_max, st,
+                   as_stream( stream ), &lo );
       TNLMI_HIP( hipGetLastError() );
    } );
 }

@@ -142,7 +142,8 @@ __global__ void __launch_bounds__( 256 ) gemm64( int64_t R, Term t0, Term t1, Te
 
 // dW[64 x 64] = X^T dY over R rows (and the column sums of dY): per block of
 // kWRows rows a partial product from LDS-staged tiles, summed over the blocks
-// in order by reduce_parts.
+// in a fixed order by reduce_partials (chunks of consecutive blocks, then the
+// chunk sums).
 __global__ void __launch_bounds__( 256 ) wgrad64( int64_t R, const uint16_t* __restrict__ X,
                                                   const uint16_t* __restrict__ dY, float* __restrict__ part,
                                                   float* __restrict__ bpart )
@@ -205,6 +206,27 @@ __global__ void reduce_parts( const float* __restrict__ part, int64_t nblk, int 
    for( int64_t b = 0; b < nblk; b++ )
       s += part[ b * width + e ];
    out[ e ] = s;
+}
+
+// Two-level form of reduce_parts for long partial lists: chunk c of at most
+// L consecutive blocks per column (in block order) by one lane each, then the
+// chunk sums in chunk order -- a fixed order, so deterministic, with
+// (columns / 64) x chunks waves in the first level instead of one thread
+// walking every block.
+constexpr int kRedChunks = 64;
+
+__global__ void __launch_bounds__( 64 ) reduce_parts_chunks( const float* __restrict__ part, int64_t nblk, int width,
+                                                             int64_t L, float* __restrict__ out )
+{
+   const int e = blockIdx.x * 64 + threadIdx.x;
+   if( e >= width )
+      return;
+   const int64_t c = blockIdx.y;
+   const int64_t b0 = c * L, b1 = std::min< int64_t >( nblk, b0 + L );
+   float s = 0.f;
+   for( int64_t b = b0; b < b1; b++ )
+      s += part[ b * width + e ];
+   out[ c * width + e ] = s;
 }
 
 // ------------------------------------------------------------ graph and IO
@@ -670,15 +692,27 @@ void forward( const tnlmi_spai& S, const Graph& g, const int32_t* off, const int
    TNLMI_HIP( hipGetLastError() );
 }
 
+// out[e] = sum of part[b * width + e] over the nblk blocks: chunks of
+// consecutive blocks (reduce_parts_chunks), then the chunk sums in order;
+// work holds kRedChunks * width floats
+void reduce_partials( const float* part, int64_t nblk, int width, float* out, float* work, hipStream_t s )
+{
+   const int64_t L = ceil_div( std::max< int64_t >( nblk, 1 ), kRedChunks );
+   const int64_t chunks = std::max< int64_t >( 1, ceil_div( nblk, L ) );
+   hipLaunchKernelGGL( reduce_parts_chunks, dim3( (unsigned) ceil_div( width, 64 ), (unsigned) chunks ), dim3( 64 ),
+                       0, s, part, nblk, width, L, work );
+   hipLaunchKernelGGL( reduce_parts, g1( width ), dim3( 256 ), 0, s, work, chunks, width, out );
+}
+
 void wgrad( int64_t R, const uint16_t* X, const uint16_t* dY, float* gW, float* gb, DeviceArray< float >& part,
-            DeviceArray< float >& bpart, hipStream_t s )
+            DeviceArray< float >& bpart, float* work, hipStream_t s )
 {
    const int64_t nb = ceil_div( R, kWRows );
    hipLaunchKernelGGL( wgrad64, dim3( (unsigned) nb ), dim3( 256 ), 0, s, R, X, dY, part.ptr,
                        gb ? bpart.ptr : (float*) nullptr );
-   hipLaunchKernelGGL( reduce_parts, g1( kH * kH ), dim3( 256 ), 0, s, part.ptr, nb, kH * kH, gW );
+   reduce_partials( part.ptr, nb, kH * kH, gW, work, s );
    if( gb )
-      hipLaunchKernelGGL( reduce_parts, dim3( 1 ), dim3( kH ), 0, s, bpart.ptr, nb, kH, gb );
+      reduce_partials( bpart.ptr, nb, kH, gb, work, s );
 }
 
 // probe loss on the device: returns L (host) and dL/dM per edge in dM
@@ -729,7 +763,7 @@ double loss_grad( tnlmi_spai& S, const tnlmi_matrix& A, const double* Z, int K, 
       TNLMI_HIP( hipMemcpyAsync( Mout, M.ptr, sizeof( double ) * E, hipMemcpyDeviceToDevice, s ) );
    float* gr = S.grads.ptr;
    const float* p = S.params.ptr;
-   DeviceArray< float > dE, dEn, dH, dG, part, bpart, dpart;
+   DeviceArray< float > dE, dEn, dH, dG, part, bpart, dpart, rwork;
    DeviceArray< uint16_t > dS, dT, dP, dQ;
    dE.allocate( E * kH );
    dEn.allocate( E * kH );
@@ -743,32 +777,33 @@ double loss_grad( tnlmi_spai& S, const tnlmi_matrix& A, const double* Z, int K, 
    part.allocate( nbE * kH * kH );
    bpart.allocate( nbE * kH );
    dpart.allocate( ceil_div( E, 256 ) * ( 5 * kH ) + 1 );
+   rwork.allocate( (int64_t) kRedChunks * kH * kH );  // chunk sums of reduce_partials (widest: kH * kH)
    // decoder
    const int64_t nb256 = ceil_div( E, 256 );
    hipLaunchKernelGGL( dec_bwd_k, dim3( (unsigned) nb256 ), dim3( 256 ), 0, s, E, dM.ptr, g.erow.ptr, g.diag.ptr,
                        a.E[ kL ].ptr, p + owo, dE.ptr, dpart.ptr );
-   hipLaunchKernelGGL( reduce_parts, dim3( 1 ), dim3( kH + 1 ), 0, s, dpart.ptr, nb256, kH + 1, gr + owo );
+   reduce_partials( dpart.ptr, nb256, kH + 1, gr + owo, rwork.ptr, s );
    TNLMI_HIP( hipMemsetAsync( dH.ptr, 0, sizeof( float ) * n * kH, s ) );
    const Term none{ nullptr, nullptr };
    for( int l = kL - 1; l >= 0; l-- ) {
       // node update: dT = dH * [H > 0]
       hipLaunchKernelGGL( mask_k, g1( n * kH ), dim3( 256 ), 0, s, n * kH, dH.ptr, a.H[ l + 1 ].ptr, dT.ptr );
-      wgrad( n, a.H[ l ].ptr, dT.ptr, gr + oW( l, mWh ), gr + obh( l ), part, bpart, s );
-      wgrad( n, a.G[ l + 1 ].ptr, dT.ptr, gr + oW( l, mWg ), nullptr, part, bpart, s );
+      wgrad( n, a.H[ l ].ptr, dT.ptr, gr + oW( l, mWh ), gr + obh( l ), part, bpart, rwork.ptr, s );
+      wgrad( n, a.G[ l + 1 ].ptr, dT.ptr, gr + oW( l, mWg ), nullptr, part, bpart, rwork.ptr, s );
       EpiArgs eg{};
       eg.yf = dG.ptr;
       gemm< 1, EPI_STORE >( n, Term{ dT.ptr, Wn( S, l, mWg ) }, none, none, eg, s );
       // edge update: dS = (dE + dG[row] / deg) * [E > 0]
       hipLaunchKernelGGL( edge_bwd_k, g1( E * kH ), dim3( 256 ), 0, s, E, dE.ptr, dG.ptr, g.erow.ptr, g.deg.ptr,
                           a.E[ l + 1 ].ptr, dS.ptr );
-      wgrad( E, a.E[ l ].ptr, dS.ptr, gr + oW( l, mWe ), gr + obe( l ), part, bpart, s );
+      wgrad( E, a.E[ l ].ptr, dS.ptr, gr + oW( l, mWe ), gr + obe( l ), part, bpart, rwork.ptr, s );
       EpiArgs ed{};
       ed.yf = dEn.ptr;
       gemm< 1, EPI_STORE >( E, Term{ dS.ptr, Wn( S, l, mWe ) }, none, none, ed, s );
       hipLaunchKernelGGL( scatter_k, g1( n * kH ), dim3( 256 ), 0, s, n, off, g.cptr.ptr, g.cperm.ptr, dS.ptr, dP.ptr,
                           dQ.ptr );
-      wgrad( n, a.H[ l ].ptr, dP.ptr, gr + oW( l, mWu ), nullptr, part, bpart, s );
-      wgrad( n, a.H[ l ].ptr, dQ.ptr, gr + oW( l, mWv ), nullptr, part, bpart, s );
+      wgrad( n, a.H[ l ].ptr, dP.ptr, gr + oW( l, mWu ), nullptr, part, bpart, rwork.ptr, s );
+      wgrad( n, a.H[ l ].ptr, dQ.ptr, gr + oW( l, mWv ), nullptr, part, bpart, rwork.ptr, s );
       EpiArgs eh{};
       eh.yf = dH.ptr;
       gemm< 3, EPI_STORE >( n, Term{ dT.ptr, Wn( S, l, mWh ) }, Term{ dP.ptr, Wn( S, l, mWu ) },
@@ -778,11 +813,11 @@ double loss_grad( tnlmi_spai& S, const tnlmi_matrix& A, const double* Z, int K, 
    // encoder
    hipLaunchKernelGGL( enc_bwd_k, dim3( (unsigned) nb256 ), dim3( 256 ), 0, s, E, g.xe.ptr, dE.ptr, a.E[ 0 ].ptr,
                        dpart.ptr );
-   hipLaunchKernelGGL( reduce_parts, g1( 5 * kH ), dim3( 256 ), 0, s, dpart.ptr, nb256, 5 * kH, gr + oAe );
+   reduce_partials( dpart.ptr, nb256, 5 * kH, gr + oAe, rwork.ptr, s );
    const int64_t nbn = ceil_div( n, 256 );
    hipLaunchKernelGGL( enc_bwd_k, dim3( (unsigned) nbn ), dim3( 256 ), 0, s, n, g.xn.ptr, dH.ptr, a.H[ 0 ].ptr,
                        dpart.ptr );
-   hipLaunchKernelGGL( reduce_parts, g1( 5 * kH ), dim3( 256 ), 0, s, dpart.ptr, nbn, 5 * kH, gr + oAn );
+   reduce_partials( dpart.ptr, nbn, 5 * kH, gr + oAn, rwork.ptr, s );
    TNLMI_HIP( hipGetLastError() );
    TNLMI_HIP( hipStreamSynchronize( s ) );
    return L;

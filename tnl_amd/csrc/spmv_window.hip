@@ -3,8 +3,9 @@
 //
 // Random column indexes make every nonzero an 8-byte gather into x; through
 // the L2 each one costs a 128-byte line and an L2 request.  The plan splits
-// the columns into windows of kWin = 8192 entries (64 KiB of x, one LDS
-// allocation) and the nonzeros into row blocks of at most `cap` (FP64 8160,
+// the columns into windows of `win` entries (8192 by default: 64 KiB of FP64
+// x, one LDS allocation; 10240, 16384 and 20480 are also built, up to the
+// 160 KiB of LDS a gfx950 workgroup can hold) and the nonzeros into row blocks of at most `cap` (FP64 8160,
 // FP32 16320): runs of whole consecutive rows, or one piece of a row longer
 // than cap (such a row is cut into pieces of cap nonzeros):
 //
@@ -28,6 +29,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "segments.hpp"
@@ -37,8 +39,7 @@
 namespace tnlmi {
 namespace {
 
-constexpr int kWinBits = 13;
-constexpr int kWin = 1 << kWinBits;  // x entries per window
+constexpr int kWinDefault = 8192;    // x entries per window
 constexpr int kP1 = 1024;            // phase-1 threads
 constexpr int kP2 = 512;             // phase-2 threads
 constexpr int kRowsMax = 1024;       // rows per row block at most
@@ -54,14 +55,14 @@ constexpr int cap_of()
 }
 
 // window of each nonzero (padding slots, column -1, go to an extra window K)
-__global__ void w_key( int64_t nnz, const int32_t* __restrict__ cols, int32_t K, int32_t* __restrict__ key,
-                       int32_t* __restrict__ ids )
+__global__ void w_key( int64_t nnz, const int32_t* __restrict__ cols, int32_t K, int32_t win,
+                       int32_t* __restrict__ key, int32_t* __restrict__ ids )
 {
    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    if( e >= nnz )
       return;
    const int32_t c = cols[ e ];
-   key[ e ] = c >= 0 ? ( c >> kWinBits ) : K;
+   key[ e ] = c >= 0 ? c / win : K;
    ids[ e ] = (int32_t) e;
 }
 
@@ -93,7 +94,7 @@ __device__ __forceinline__ int64_t block_of( int64_t B, const int64_t* __restric
 
 // the sorted arrays: value, local column, slot within the row block
 template< typename V >
-__global__ void w_fill( int64_t nnz, const int32_t* __restrict__ perm, const int32_t* __restrict__ cols,
+__global__ void w_fill( int64_t nnz, int32_t win, const int32_t* __restrict__ perm, const int32_t* __restrict__ cols,
                         const V* __restrict__ vals, int64_t B, const int64_t* __restrict__ E, V* __restrict__ pv,
                         uint16_t* __restrict__ pc, uint16_t* __restrict__ dest )
 {
@@ -103,7 +104,7 @@ __global__ void w_fill( int64_t nnz, const int32_t* __restrict__ perm, const int
    const int32_t e = perm[ i ];
    const int32_t c = cols[ e ];
    pv[ i ] = vals[ e ];
-   pc[ i ] = (uint16_t) ( c >= 0 ? ( c & ( kWin - 1 ) ) : 0 );
+   pc[ i ] = (uint16_t) ( c >= 0 ? c % win : 0 );
    dest[ i ] = (uint16_t) ( e - E[ block_of( B, E, e ) ] );
 }
 
@@ -130,15 +131,15 @@ __global__ void w_seg( int64_t B, int32_t K, const int64_t* __restrict__ E, cons
 }
 
 // phase 1: prod[i] = v[i] * x[window base + pc[i]] for the window's nonzeros
-template< typename V, typename C >
+template< typename V, typename C, int WIN >
 __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t* __restrict__ wstart,
                                                    const V* __restrict__ pv, const uint16_t* __restrict__ pc,
                                                    const C* __restrict__ x, C* __restrict__ prod )
 {
-   __shared__ C xw[ kWin ];
+   __shared__ C xw[ WIN ];
    const int64_t k = blockIdx.x;
-   const int64_t base = k * kWin;
-   for( int t = threadIdx.x; t < kWin; t += kP1 )
+   const int64_t base = k * WIN;
+   for( int t = threadIdx.x; t < WIN; t += kP1 )
       xw[ t ] = base + t < ncols ? x[ base + t ] : (C) 0;
    __syncthreads();
    const int64_t a = wstart[ k ], b = wstart[ k + 1 ];
@@ -163,17 +164,16 @@ __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t*
 
 // phase 2: row block b -> LDS in CSR order -> row sums -> y.  One workgroup
 // per row block.  The products of a block's segment (window k) are loaded by
-// a group of kGrp lanes together, kPer consecutive terms each: one load
-// instruction covers kGrp * kPer terms of a segment, so the lanes of a group
+// a group of kGrp lanes together, PER consecutive terms each: one load
+// instruction covers kGrp * PER terms of a segment, so the lanes of a group
 // touch consecutive cache lines (a lane per segment would touch a line of its
 // own in every load, and the L1 tag lookups, one per line, would bound the
-// kernel).  All rounds of a thread (kP2 / kGrp groups, R = kSegs / (kP2 /
-// kGrp) segments each) are issued before the first is stored.  (1024-thread
-// workgroups at 64 VGPRs, two per CU, measured no faster than 512.)
+// kernel).  All rounds of a thread (kP2 / kGrp groups, R = SEGS / (kP2 /
+// kGrp) segments each) are issued before the first is stored.  SEGS, the
+// segments (windows) per sweep of a block, is 512, or 256 for plans of at
+// most 256 windows.  (1024-thread workgroups at 64 VGPRs, two per CU,
+// measured no faster than 512.)
 constexpr int kGrp = 16;
-constexpr int kPer = 2;
-constexpr int kSpan = kGrp * kPer;  // terms of a segment per load round
-constexpr int kSegs = 512;          // segments (windows) per sweep of a block
 
 // per-block header: 8 int32 -- rows [r0, r1), nonzeros [e0, e1), the long
 // row a piece belongs to (-1: a run of whole rows), long-row list [l0, l1)
@@ -206,29 +206,30 @@ __device__ __forceinline__ C sum_row( const C* buf, int32_t q0, int32_t q1 )
    return sum;
 }
 
-// kPer terms at i (products and their 16-bit slots), loaded unconditionally:
-// prod and dest carry kSpan entries of padding, so a read past a segment's
+// PER terms at i (products and their 16-bit slots), loaded unconditionally:
+// prod and dest carry kPad entries of padding, so a read past a segment's
 // end stays in bounds (and what it reads is not stored)
-template< typename C >
-__device__ __forceinline__ void get_pair( const C* prod, const uint16_t* dest, int32_t i, C ( &v )[ kPer ],
-                                          uint32_t& d )
+template< typename C, int PER >
+__device__ __forceinline__ void get_pair( const C* prod, const uint16_t* dest, int32_t i, C ( &v )[ PER ],
+                                          uint32_t ( &d )[ PER / 2 ] )
 {
 #pragma unroll
-   for( int u = 0; u < kPer; u++ )
+   for( int u = 0; u < PER; u++ )
       v[ u ] = prod[ i + u ];
-   __builtin_memcpy( &d, dest + i, sizeof( d ) );
+   __builtin_memcpy( d, dest + i, sizeof( d ) );
 }
 
-template< typename C >
-__device__ __forceinline__ void put_pair( C* buf, const C ( &v )[ kPer ], uint32_t d, int32_t i, int32_t z )
+template< typename C, int PER >
+__device__ __forceinline__ void put_pair( C* buf, const C ( &v )[ PER ], const uint32_t ( &d )[ PER / 2 ], int32_t i,
+                                          int32_t z )
 {
 #pragma unroll
-   for( int u = 0; u < kPer; u++ )
+   for( int u = 0; u < PER; u++ )
       if( i + u < z )
-         buf[ ( d >> ( 16 * u ) ) & 0xffff ] = v[ u ];
+         buf[ ( d[ u / 2 ] >> ( 16 * ( u % 2 ) ) ) & 0xffff ] = v[ u ];
 }
 
-template< typename C, int CAP, bool BETA >
+template< typename C, int CAP, bool BETA, int SEGS, int PER >
 __global__ void __launch_bounds__( kP2 )
    w_phase2( int64_t B, int32_t K, const BlockHdr* __restrict__ hdr, const int32_t* __restrict__ seg,
              const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ off,
@@ -238,9 +239,13 @@ __global__ void __launch_bounds__( kP2 )
    __shared__ C buf[ CAP ];
    constexpr int P2 = kP2;
    __shared__ C red[ P2 / 64 ];
-   // kSegs segments per sweep: P2 / kGrp groups of R segments each
-   constexpr int R = kSegs * kGrp / P2;
+   // SEGS segments per sweep: P2 / kGrp groups of R segments each
+   constexpr int R = SEGS * kGrp / P2;
+   constexpr int kSpan = kGrp * PER;  // terms of a segment per load round
    static_assert( R >= 1 && R <= kGrp, "a group's lanes hold its segments' bounds" );
+   // a read of PER terms starts at most at a segment's last term (or, for an
+   // empty segment, its start, at most nnz): it ends inside the kPad padding
+   static_assert( PER % 2 == 0 && PER <= kPad, "slot words hold two slots; reads stay in the padding" );
    // consecutive row blocks on one XCD: their chunks of a window's products
    // (about 120 bytes each, 30 of slot indexes) share cache lines in its L2
    const int64_t b = xcd_contiguous( blockIdx.x, gridDim.x );
@@ -270,14 +275,14 @@ __global__ void __launch_bounds__( kP2 )
    const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
    const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
    // round i: the group's segment i, kSpan terms
-   C v[ R ][ kPer ];
-   uint32_t d[ R ];
+   C v[ R ][ PER ];
+   uint32_t d[ R ][ PER / 2 ];
 #pragma unroll
    for( int i = 0; i < R; i++ ) {
       // lanes past the segment's end read its last term again (no extra
       // cache line per request); what they read is not stored
       const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
-      get_pair( prod, dest, std::min( ai + kPer * gl, std::max( zi - 1, ai ) ), v[ i ], d[ i ] );
+      get_pair< C, PER >( prod, dest, std::min( ai + PER * gl, std::max( zi - 1, ai ) ), v[ i ], d[ i ] );
    }
    // clear (padding slots, column -1, have no product: they read as zero)
    const int32_t len = H.e1 - H.e0;
@@ -287,26 +292,26 @@ __global__ void __launch_bounds__( kP2 )
 #pragma unroll
    for( int i = 0; i < R; i++ ) {
       const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
-      put_pair( buf, v[ i ], d[ i ], ai + kPer * gl, zi );
+      put_pair< C, PER >( buf, v[ i ], d[ i ], ai + PER * gl, zi );
       // on demand: the rest of a segment longer than kSpan
-      for( int32_t i0 = ai + kSpan + kPer * gl; i0 < zi; i0 += kSpan ) {
-         C w[ kPer ];
-         uint32_t dw;
-         get_pair( prod, dest, i0, w, dw );
-         put_pair( buf, w, dw, i0, zi );
+      for( int32_t i0 = ai + kSpan + PER * gl; i0 < zi; i0 += kSpan ) {
+         C w[ PER ];
+         uint32_t dw[ PER / 2 ];
+         get_pair< C, PER >( prod, dest, i0, w, dw );
+         put_pair< C, PER >( buf, w, dw, i0, zi );
       }
    }
-   // on demand: windows past kSegs (segment k handled as segment k % kSegs)
-   for( int32_t k0 = kSegs; k0 < K; k0 += kSegs ) {
+   // on demand: windows past SEGS (segment k handled as segment k % SEGS)
+   for( int32_t k0 = SEGS; k0 < K; k0 += SEGS ) {
       const int32_t k = std::min( k0 + sx, K - 1 );
       const int32_t ak = seg[ b * K + k ], zk = k0 + sx < K ? seg[ ( b + 1 ) * K + k ] : ak;
       for( int i = 0; i < R; i++ ) {
          const int32_t ai = __shfl( ak, gb + i, 64 ), zi = __shfl( zk, gb + i, 64 );
-         for( int32_t i0 = ai + kPer * gl; i0 < zi; i0 += kSpan ) {
-            C w[ kPer ];
-            uint32_t dw;
-            get_pair( prod, dest, i0, w, dw );
-            put_pair( buf, w, dw, i0, zi );
+         for( int32_t i0 = ai + PER * gl; i0 < zi; i0 += kSpan ) {
+            C w[ PER ];
+            uint32_t dw[ PER / 2 ];
+            get_pair< C, PER >( prod, dest, i0, w, dw );
+            put_pair< C, PER >( buf, w, dw, i0, zi );
          }
       }
    }
@@ -373,7 +378,7 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    const int32_t* off = seg.offsets.ptr;
    const int32_t* cols = A.column_indexes;
    const V* vals = (const V*) A.values;
-   W.K = (int32_t) std::max< int64_t >( 1, ceil_div( A.columns, kWin ) );
+   W.K = (int32_t) std::max< int64_t >( 1, ceil_div( A.columns, W.win ) );
    // row blocks: runs of whole consecutive rows with at most cap nonzeros
    // together, or pieces of cap nonzeros of a longer row
    std::vector< int32_t > hoff( n + 1 );
@@ -446,7 +451,7 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
       a->allocate( std::max< int64_t >( 1, nnz ) );
    const dim3 b256( 256 );
    auto g = [ & ]( int64_t c ) { return dim3( (unsigned) std::max< int64_t >( 1, ceil_div( c, 256 ) ) ); };
-   hipLaunchKernelGGL( w_key, g( nnz ), b256, 0, s, nnz, cols, W.K, key.ptr, ids.ptr );
+   hipLaunchKernelGGL( w_key, g( nnz ), b256, 0, s, nnz, cols, W.K, W.win, key.ptr, ids.ptr );
    int bits = 1;
    while( ( 1 << bits ) <= W.K + 1 )
       bits++;
@@ -462,7 +467,7 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    W.pv.allocate( std::max< int64_t >( 1, nnz ) * (int64_t) sizeof( V ) );
    W.pc.allocate( std::max< int64_t >( 1, nnz ) );
    W.dest.allocate( nnz + kPad );
-   hipLaunchKernelGGL( ( w_fill< V > ), g( nnz ), b256, 0, s, nnz, perm.ptr, cols, vals, W.B, dE.ptr,
+   hipLaunchKernelGGL( ( w_fill< V > ), g( nnz ), b256, 0, s, nnz, W.win, perm.ptr, cols, vals, W.B, dE.ptr,
                        (V*) W.pv.ptr, W.pc.ptr, W.dest.ptr );
    W.seg.allocate( ( W.B + 1 ) * W.K );
    hipLaunchKernelGGL( w_seg, g( ( W.B + 1 ) * W.K ), b256, 0, s, W.B, W.K, dE.ptr, W.wstart.ptr, perm.ptr,
@@ -474,18 +479,45 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    TNLMI_HIP( hipStreamSynchronize( s ) );
 }
 
+template< typename C, int WIN >
+void phase1( const WindowPlan& W, const C* x, hipStream_t s )
+{
+   hipLaunchKernelGGL( ( w_phase1< C, C, WIN > ), dim3( (unsigned) W.K ), dim3( kP1 ), 0, s, W.columns, W.wstart.ptr,
+                       (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
+}
+
+template< typename C, int SEGS, int PER >
+void phase2( const WindowPlan& W, C* y, double alpha, double beta, int64_t begin, int64_t end, hipStream_t s )
+{
+   hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER >
+                                     : w_phase2< C, cap_of< C >(), true, SEGS, PER > ),
+                       dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
+                       (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y, (C) alpha,
+                       (C) beta, begin, end );
+}
+
 template< typename C >
 void run( const WindowPlan& W, const C* x, C* y, double alpha, double beta, int64_t begin, int64_t end,
           hipStream_t s )
 {
-   if( W.nnz > 0 )
-      hipLaunchKernelGGL( ( w_phase1< C, C > ), dim3( (unsigned) W.K ), dim3( kP1 ), 0, s, W.columns, W.wstart.ptr,
-                          (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
-   if( W.B > 0 )
-      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false > : w_phase2< C, cap_of< C >(), true > ),
-                          dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr,
-                          W.seg.ptr, (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y,
-                          (C) alpha, (C) beta, begin, end );
+   if( W.nnz > 0 ) {
+      switch( W.win ) {
+         case 8192: phase1< C, 8192 >( W, x, s ); break;
+         case 10240: phase1< C, 10240 >( W, x, s ); break;
+         case 16384: phase1< C, 16384 >( W, x, s ); break;
+         case 20480: phase1< C, 20480 >( W, x, s ); break;
+         default: fail( TNLMI_RUNTIME_ERROR, "windowed SpMV plan: unsupported window width" );
+      }
+   }
+   if( W.B > 0 ) {
+      const int key = W.segs * 10 + W.per;
+      switch( key ) {
+         case 5122: phase2< C, 512, 2 >( W, y, alpha, beta, begin, end, s ); break;
+         case 2562: phase2< C, 256, 2 >( W, y, alpha, beta, begin, end, s ); break;
+         case 2564: phase2< C, 256, 4 >( W, y, alpha, beta, begin, end, s ); break;
+         default: fail( TNLMI_RUNTIME_ERROR, "windowed SpMV plan: unsupported phase-2 shape" );
+      }
+   }
    if( W.H > 0 )
       hipLaunchKernelGGL( ( w_fixup< C > ), dim3( (unsigned) ceil_div( W.H, 256 ) ), dim3( 256 ), 0, s, W.H,
                           W.huge.ptr, (const C*) W.part.ptr, y, (C) alpha, (C) beta, begin, end );
@@ -499,6 +531,20 @@ void window_build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    W.columns = A.columns;
    W.nnz = A.segments->elements;
    W.value_type = A.value_type;
+   // window width and phase-2 shape: TNLMI_WINDOW_COLS (8192, 10240, 16384,
+   // 20480) and TNLMI_WINDOW_PER (2, 4) in the environment override them
+   W.win = kWinDefault;
+   if( const char* e = std::getenv( "TNLMI_WINDOW_COLS" ) ) {
+      const int v = std::atoi( e );
+      if( v == 8192 || v == 10240 || v == 16384 || v == 20480 )
+         W.win = v;
+   }
+   W.K = (int32_t) std::max< int64_t >( 1, ceil_div( A.columns, W.win ) );
+   W.segs = W.K <= 256 ? 256 : 512;
+   W.per = 2;
+   if( const char* e = std::getenv( "TNLMI_WINDOW_PER" ) )
+      if( std::atoi( e ) == 4 && W.segs == 256 )
+         W.per = 4;
    W.offs.allocate( A.rows + 1 );
    TNLMI_HIP( hipMemcpyAsync( W.offs.ptr, A.segments->offsets.ptr, sizeof( int32_t ) * ( A.rows + 1 ),
                               hipMemcpyDeviceToDevice, s ) );

@@ -9,6 +9,9 @@ struct WindowPlan
 {
    int64_t rows = 0, columns = 0, nnz = 0;
    tnlmi_dtype value_type = TNLMI_F64;
+   int32_t win = 8192;  // columns per window (x entries staged in LDS by phase 1)
+   int32_t segs = 512;  // phase 2: windows per sweep of a row block (256 or 512)
+   int32_t per = 2;     // phase 2: consecutive terms per lane and load
    int32_t K = 0;    // column windows
    int64_t B = 0;     // row blocks
    int64_t H = 0;     // rows longer than cap (cut into pieces)

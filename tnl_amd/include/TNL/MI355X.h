@@ -620,8 +620,10 @@ class SparseMatrix< Real, Devices::MI355X, Index, MatrixType, Segments, ComputeR
    // the device kernels take int32 offsets and column indexes (TNL's default
    // Index = int); 64-bit indices are not provided
    static_assert( std::is_same_v< Index, int >, "SparseMatrix on Devices::MI355X supports Index = int only" );
-   using SegTag = MI355XDetail::SegmentsFormat<
-      Segments< Devices::Host, Index, typename Allocators::Default< Devices::Host >::template Allocator< Index > > >;
+   // the segments type with this device (only named, never instantiated: its
+   // template arguments -- e.g. the default ElementsOrganization, ColumnMajorOrder
+   // on every device but Host (ElementsOrganization.h:21-32) -- select the format)
+   using SegTag = MI355XDetail::SegmentsFormat< Segments< Devices::MI355X, Index, IndexAllocator > >;
 
 public:
    using RealType = Real;

@@ -184,6 +184,35 @@ class GMRES(IterativeSolver):
         return self._finish(st)
 
 
+class MixedGMRES(GMRES):
+    """Mixed-precision GMRES (tnlmi_gmres_mixed_solve): GMRES's iteration and
+    options (variant, adaptive restarting, preconditioner, stopping rule) with
+    the Arnoldi products on the inner matrix -- the same matrix with FP32 or
+    BF16 values -- and every restart residual M^-1 (b - A x) in FP64 with the
+    matrix of setMatrix: iterative refinement with one GMRES cycle per
+    correction.  Basis, Hessenberg least squares and x stay FP64.  No
+    reference counterpart."""
+
+    def __init__(self):
+        super().__init__()
+        self.matrix_lo = None
+
+    def setInnerMatrix(self, matrix_lo):
+        self.matrix_lo = matrix_lo
+
+    def solve(self, b, x):
+        if self.matrix_lo is None:
+            raise _lib.LogicError(_lib.LOGIC_ERROR, "mixed GMRES: no inner matrix was set")
+        m, plan, pre = self._handles()
+        st = self._state()
+        _lib.check(_lib.lib().tnlmi_gmres_mixed_solve(m, ctypes.byref(self.matrix_lo._m), plan, pre,
+                                                      GMRES_VARIANTS[self.variant], self.restarting_min,
+                                                      self.restarting_max, self.restarting_step_min,
+                                                      self.restarting_step_max, _f64(b), _f64(x), ctypes.byref(st),
+                                                      _stream()))
+        return self._finish(st)
+
+
 class BICGStab(IterativeSolver):
     """Solvers::Linear::BICGStab (BICGStab.hpp:33-127); setup(exact_residue=...)
     is the bicgstab-exact-residue option (default false: the residue is
