@@ -185,6 +185,26 @@ tnlmi_status tnlmi_matrix_copy( const tnlmi_matrix* src, tnlmi_matrix* dst, cons
  * a position outside the matrix.  Synchronous. */
 tnlmi_status tnlmi_matrix_add_element( tnlmi_matrix* A, int64_t row, int64_t column, double value,
                                        double this_element_multiplicator, tnlmi_stream stream );
+/* A list of `count` element operations in one launch: operation i is
+ * addElement( d_rows[i], d_columns[i], d_values[i], d_multiplicators[i] )
+ * (d_multiplicators NULL: setElement, multiplicator 0), each exactly as
+ * tnlmi_matrix_add_element does it.  The operations of one row are applied in
+ * list order; rows are independent, so the matrix ends as after calling
+ * tnlmi_matrix_add_element for i = 0, 1, ... in turn.  Rows run in parallel
+ * (one thread per row, operations sorted by row on the device): the device
+ * arrays (int64 positions, FP64 values) are read once, with one host
+ * synchronisation for the whole list.  Errors: TNLMI_INVALID_ARGUMENT if a
+ * position lies outside the matrix (nothing is written); TNLMI_LOGIC_ERROR
+ * "The capacity of the sparse matrix row number r was exceeded." when an
+ * operation finds its row full -- that row stops there, other rows' operations
+ * are applied, and *failed (may be NULL; -1 otherwise) is the smallest list
+ * index of such an operation.  Synchronous.  No reference counterpart as one
+ * call (TNL assembles a matrix with addElement / setElement calls or with
+ * RowView::setElement inside forAllRows, SparseMatrix.hpp:336-470): it is the
+ * device-side assembly of many elements. */
+tnlmi_status tnlmi_matrix_add_elements( tnlmi_matrix* A, int64_t count, const int64_t* d_rows,
+                                        const int64_t* d_columns, const double* d_values,
+                                        const double* d_multiplicators, int64_t* failed, tnlmi_stream stream );
 /* Per-row counts over each row's getSegmentSize slots, into d_counts (rows
  * int32), synchronous (SparseMatrixBase.hpp:55-132; padding slots count as 0):
  *   0  getRowCapacity( r ): the segment size (Ellpack: the width)

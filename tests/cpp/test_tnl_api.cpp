@@ -46,10 +46,12 @@ static int failures = 0;
       }                                                                                                              \
    } while( 0 )
 
-template< template< class, class, class > class Segments >
+// IndexT = int or long: the reference's SpMV tests instantiate both index
+// types (src/UnitTests/Matrices/SparseMatrixTest_types.h)
+template< template< class, class, class > class Segments, typename IndexT = int >
 void vectorProductKATs( const char* name, bool csr = false )
 {
-   using Matrix = Matrices::SparseMatrix< double, Device, int, Matrices::GeneralMatrix, Segments >;
+   using Matrix = Matrices::SparseMatrix< double, Device, IndexT, Matrices::GeneralMatrix, Segments >;
    std::vector< Seg::LaunchConfiguration > configs{ Seg::LaunchConfiguration() };
    if( csr ) {
       for( int t : { 2, 4, 8, 16, 32, 64, 128, 256 } )
@@ -61,7 +63,7 @@ void vectorProductKATs( const char* name, bool csr = false )
       // zeroMatrix: capacities [1, 2, 1, 1], no entries
       {
          Matrix m( 4, 4 );
-         m.setRowCapacities( std::vector< int >{ 1, 2, 1, 1 } );
+         m.setRowCapacities( std::vector< IndexT >{ 1, 2, 1, 1 } );
          Vector in( 4, 1 ), out( 4, -1 );
          m.vectorProduct( in, out, cfg );
          for( int i = 0; i < 4; i++ )
@@ -107,14 +109,14 @@ void vectorProductKATs( const char* name, bool csr = false )
       {
          const int n = 1051;
          Matrix m( n, n );
-         std::vector< int > off( n + 1, 0 ), cols;
+         std::vector< IndexT > off( n + 1, 0 ), cols;
          std::vector< double > vals;
          for( int i = 0; i < n; i++ ) {
             for( int j = 0; j <= i; j++ ) {
                cols.push_back( j );
                vals.push_back( j + 1 );
             }
-            off[ i + 1 ] = (int) cols.size();
+            off[ i + 1 ] = (IndexT) cols.size();
          }
          m.setFromCSR( off, cols, vals );
          Vector in( n, 1.0 ), out( n, 0.0 );
@@ -134,7 +136,7 @@ void vectorProductKATs( const char* name, bool csr = false )
       EXPECT_EQ( out.getElement( 2 ), 2.0 * 8 + 3.0 );
       EXPECT_EQ( out.getElement( 3 ), 1.0 );
    }
-   std::cout << name << " KATs done\n";
+   std::cout << name << ( std::is_same_v< IndexT, int > ? "" : " (Index = long)" ) << " KATs done\n";
 }
 
 // The solver example written once for any device: the reference's code for
@@ -142,7 +144,7 @@ void vectorProductKATs( const char* name, bool csr = false )
 template< typename Solver, typename Matrix >
 int solveExample( const std::shared_ptr< Matrix >& A, const char* name, bool diag = false )
 {
-   using V = Containers::Vector< double, typename Matrix::DeviceType, int >;
+   using V = Containers::Vector< double, typename Matrix::DeviceType, typename Matrix::IndexType >;
    V x( 5, 1.0 ), b( 5 );
    A->vectorProduct( x, b );
    x = 0.0;
@@ -166,12 +168,12 @@ int solveExample( const std::shared_ptr< Matrix >& A, const char* name, bool dia
 }
 
 // IterativeLinearSolverExample.cpp's matrix, tridiag(-1, 2.5, -1), on any device
-template< typename D >
+template< typename D, typename IndexT = int >
 auto exampleMatrix()
 {
-   using M = Matrices::SparseMatrix< double, D, int >;
+   using M = Matrices::SparseMatrix< double, D, IndexT >;
    return std::make_shared< M >( 5, 5,
-                                 std::initializer_list< std::tuple< int, int, double > >{
+                                 std::initializer_list< std::tuple< IndexT, IndexT, double > >{
                                     { 0, 0, 2.5 }, { 0, 1, -1 }, { 1, 0, -1 }, { 1, 1, 2.5 }, { 1, 2, -1 },
                                     { 2, 1, -1 },  { 2, 2, 2.5 }, { 2, 3, -1 }, { 3, 2, -1 }, { 3, 3, 2.5 },
                                     { 3, 4, -1 },  { 4, 3, -1 },  { 4, 4, 2.5 } } );
@@ -187,6 +189,11 @@ int main()
    vectorProductKATs< Seg::BiEllpack >( "BiEllpack" );
    vectorProductKATs< Seg::ChunkedEllpack >( "ChunkedEllpack" );
    vectorProductKATs< Seg::SortedCSR >( "SortedCSR" );
+   vectorProductKATs< Seg::CSR, long >( "CSR", true );
+   vectorProductKATs< Seg::Ellpack, long >( "Ellpack" );
+   vectorProductKATs< Seg::SlicedEllpack, long >( "SlicedEllpack" );
+   vectorProductKATs< Seg::BiEllpack, long >( "BiEllpack" );
+   vectorProductKATs< Seg::ChunkedEllpack, long >( "ChunkedEllpack" );
 
    // SparseMatrixExample_addElement.cpp: diagonal i, then addElement( i, j, 1, 5 )
    // everywhere: 5 i + 1 on the diagonal, 1 elsewhere (inserted in column order)
@@ -264,6 +271,24 @@ int main()
                  solveExample< Solvers::Linear::CG< Matrix > >( G, "CG" ) );
       EXPECT_EQ( solveExample< Solvers::Linear::CG< HostMatrix > >( H, "CG + Diagonal", true ),
                  solveExample< Solvers::Linear::CG< Matrix > >( G, "CG + Diagonal", true ) );
+      // Index = long on both devices
+      using HostMatrixL = Matrices::SparseMatrix< double, Devices::Host, long >;
+      using MatrixL = Matrices::SparseMatrix< double, Device, long >;
+      auto HL = exampleMatrix< Devices::Host, long >();
+      auto GL = exampleMatrix< Device, long >();
+      EXPECT_EQ( solveExample< Solvers::Linear::CG< HostMatrixL > >( HL, "CG (Index = long)" ),
+                 solveExample< Solvers::Linear::CG< MatrixL > >( GL, "CG (Index = long)" ) );
+      EXPECT_EQ( solveExample< Solvers::Linear::GMRES< HostMatrixL > >( HL, "GMRES + Diagonal (Index = long)", true ),
+                 solveExample< Solvers::Linear::GMRES< MatrixL > >( GL, "GMRES + Diagonal (Index = long)", true ) );
+      {
+         Containers::Vector< long, Device, long > caps;
+         GL->getRowCapacities( caps );
+         const auto hc = caps.toHost();
+         EXPECT_EQ( hc[ 0 ], 2L );
+         EXPECT_EQ( hc[ 2 ], 3L );
+         EXPECT_EQ( GL->getNonzeroElementsCount(), 13L );
+         EXPECT_EQ( GL->getElement( 3, 4 ), -1.0 );
+      }
       // cross-device assignment of the reference's host matrix and vector
       Matrix D;
       D = *H;

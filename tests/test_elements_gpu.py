@@ -213,3 +213,73 @@ def test_slots_past_the_capacity_are_walked(fmt, param, org):
         z = torch.zeros(n, dtype=torch.float64, device="cuda")
         P.solve(b, z)
         assert np.array_equal(z.cpu().numpy(), b.cpu().numpy() / np.diag(D)), symmetric
+
+
+@pytest.mark.parametrize("fmt,param,name", FORMATS, ids=[f[2] for f in FORMATS])
+@pytest.mark.parametrize("symmetric", [False, True], ids=["general", "symmetric"])
+def test_add_elements_batch_matches_one_by_one(fmt, param, name, symmetric):
+    """addElements (one launch for the whole list, rows in parallel) leaves the
+    storage TNL's addElement calls in list order leave, and equals the one-by-one
+    GPU path; mixed setElement (multiplicator 0) and addElement operations."""
+    ref = OL.Reference.load()
+    rng = np.random.default_rng(fmt * 2 + symmetric + 100)
+    n = m = 300
+    caps = rng.integers(1, 9, n)
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum(caps)
+    # operations that never overflow a row: each stored row receives at most its capacity of distinct
+    # columns (a symmetric matrix stores row >= column; an operation names either (r, c) or (c, r))
+    ops = []
+    for r in range(n):
+        pool = np.arange(r + 1) if symmetric else np.arange(m)
+        cols = rng.choice(pool, size=min(int(caps[r]), len(pool)), replace=False)
+        for _ in range(int(rng.integers(0, 12))):
+            c = int(cols[rng.integers(len(cols))])
+            rr, cc = (c, r) if symmetric and rng.random() < 0.5 else (r, c)
+            ops.append((rr, cc, float(rng.standard_normal()), [0.0, 1.0, 0.5][rng.integers(3)]))
+    order = rng.permutation(len(ops))
+    ops = [ops[i] for i in order]
+    empty = (np.full(off[-1], -1, np.int32), np.zeros(off[-1]))
+    A = tnl.SparseMatrix.from_csr(n, m, off, *empty, fmt=fmt, param=param, symmetric=symmetric)
+    B = tnl.SparseMatrix.from_csr(n, m, off, *empty, fmt=fmt, param=param, symmetric=symmetric)
+    rows = [o[0] for o in ops]
+    cols = [o[1] for o in ops]
+    vals = [o[2] for o in ops]
+    mls = [o[3] for o in ops]
+    A.addElements(rows, cols, vals, mls)
+    apply(B, ops, -1)
+    assert np.array_equal(A.column_indexes.cpu().numpy(), B.column_indexes.cpu().numpy())
+    assert np.array_equal(A.values.cpu().numpy(), B.values.cpu().numpy())
+    if ref is not None:
+        rc, rv, rfailed, _ = ref.add_elements("CSR" if fmt == tnl.ADAPTIVE_CSR else name, off, m, ops,
+                                              symmetric=symmetric)
+        assert rfailed == -1
+        assert np.array_equal(A.column_indexes.cpu().numpy()[: len(rc)], rc)
+        assert np.array_equal(A.values.cpu().numpy()[: len(rv)], rv)
+    # setElement batch (multiplicator 0) over the same positions
+    A.setElementsBatch(rows, cols, [2.0 * v for v in vals])
+    B2 = tnl.SparseMatrix.from_csr(n, m, off, *empty, fmt=fmt, param=param, symmetric=symmetric)
+    apply(B2, ops, -1)
+    apply(B2, [(r, c, 2.0 * v, 0.0) for (r, c, v, _) in ops], -1)
+    assert np.array_equal(A.values.cpu().numpy(), B2.values.cpu().numpy())
+
+
+def test_add_elements_batch_errors():
+    n = 6
+    off = np.array([0, 1, 2, 3, 4, 5, 6], np.int32)
+    empty = (np.full(6, -1, np.int32), np.zeros(6))
+    A = tnl.SparseMatrix.from_csr(n, n, off, *empty)
+    # row 2 takes one column, then a larger one: its capacity is exceeded at list index 3;
+    # the other rows' operations are applied, row 2 keeps its first element
+    rows, cols, vals = [0, 2, 4, 2, 5], [0, 1, 4, 3, 5], [1.0, 2.0, 3.0, 4.0, 5.0]
+    with pytest.raises(tnl.LogicError, match="The capacity of the sparse matrix row number 2 was exceeded"):
+        A.addElements(rows, cols, vals)
+    got = {(r, c): A.getElement(r, c) for r, c in zip(rows, cols)}
+    assert got == {(0, 0): 1.0, (2, 1): 2.0, (4, 4): 3.0, (2, 3): 0.0, (5, 5): 5.0}
+    # a position outside the matrix: nothing is written
+    B = tnl.SparseMatrix.from_csr(n, n, off, *empty)
+    with pytest.raises(tnl.InvalidArgument):
+        B.addElements([0, 6], [0, 0], [1.0, 1.0])
+    assert B.getElement(0, 0) == 0.0
+    with pytest.raises(tnl.InvalidArgument):
+        B.addElements([0, 1], [0], [1.0, 1.0])

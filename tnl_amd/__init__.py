@@ -330,6 +330,36 @@ class SparseMatrix:
         _lib.check(_lib.lib().tnlmi_matrix_add_element(ctypes.byref(self._m), int(row), int(column), float(value),
                                                        float(thisElementMultiplicator), _stream(stream)))
 
+    def addElements(self, rows, columns, values, thisElementMultiplicators=None, stream=None):
+        """Many addElement operations in one launch (tnlmi_matrix_add_elements): operation i is
+        addElement(rows[i], columns[i], values[i], thisElementMultiplicators[i]); None for the
+        multiplicators gives setElement (multiplicator 0).  The matrix ends as after the operations
+        one by one in list order; LogicError when a row is full (that row stops, other rows' operations
+        are applied).  rows / columns / values / multiplicators: sequences, numpy arrays or tensors."""
+        torch = _torch()
+
+        def dev(a, dtype):
+            t = a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a))
+            return t.to(device="cuda", dtype=dtype).contiguous()
+        r, c, v = dev(rows, torch.int64), dev(columns, torch.int64), dev(values, torch.float64)
+        if not (r.numel() == c.numel() == v.numel()):
+            raise InvalidArgument(_lib.INVALID_ARGUMENT, "addElements: rows, columns and values differ in length")
+        m = None
+        if thisElementMultiplicators is not None:
+            m = dev(thisElementMultiplicators, torch.float64)
+            if m.numel() != r.numel():
+                raise InvalidArgument(_lib.INVALID_ARGUMENT, "addElements: multiplicators differ in length")
+        failed = ctypes.c_int64(-1)
+        _lib.check(_lib.lib().tnlmi_matrix_add_elements(
+            ctypes.byref(self._m), r.numel(), ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(c.data_ptr()),
+            ctypes.c_void_p(v.data_ptr()), ctypes.c_void_p(m.data_ptr()) if m is not None else None,
+            ctypes.byref(failed), _stream(stream)))
+
+    def setElementsBatch(self, rows, columns, values, stream=None):
+        """setElement for each (rows[i], columns[i], values[i]) in one launch (addElements with
+        multiplicator 0)."""
+        self.addElements(rows, columns, values, None, stream)
+
     def _row_counts(self, kind, stream=None):
         torch = _torch()
         out = torch.empty(max(1, self.rows), dtype=torch.int32, device="cuda")

@@ -183,6 +183,8 @@ _PROTOS = {
     "tnlmi_matrix_copy": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.POINTER(Matrix), c_vp, c_vp]),
     "tnlmi_matrix_add_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.c_double, ctypes.c_double, c_vp]),
+    "tnlmi_matrix_add_elements": (ctypes.c_int, [ctypes.POINTER(Matrix), c_i64, c_vp, c_vp, c_vp, c_vp,
+                                                 ctypes.POINTER(c_i64), c_vp]),
     "tnlmi_matrix_row_counts": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int32, c_vp, c_vp]),
     "tnlmi_matrix_get_element": (ctypes.c_int, [ctypes.POINTER(Matrix), ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.POINTER(ctypes.c_double), c_vp]),

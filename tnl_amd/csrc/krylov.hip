@@ -39,8 +39,14 @@ struct tnlmi_preconditioner
    {
       int64_t l0, l1;
       bool narrow;
+      bool ring;  // every dependency inside the run lies within kRing positions of its level's end
    };
    std::vector< Run > lruns, uruns;
+   // the factors in level-order positions (for the LDS-ring solve): per position p
+   // (row lrows[p] / urows[p]) its entries [pp[p], pp[p+1]): dependency position,
+   // dependency row, value; the upper factor's pivot per position
+   DeviceArray< int32_t > lpp, ldep, lcol, upp, udep, ucol;
+   DeviceArray< double > lval, uval, udiag;
    tnlmi_matrix M{};                   // explicit: x = M b
 };
 
@@ -137,8 +143,12 @@ __global__ void upper_level( const int32_t* __restrict__ rows, int64_t nrows, in
 
 // Consecutive narrow levels in one launch of one workgroup: the levels in
 // order, a barrier after each, so a level's rows read the solution entries of
-// earlier levels that other threads of the workgroup wrote (stores and the
-// loads of x are device-coherent: no L1 copy of x is trusted).  While a level
+// earlier levels that other threads of the workgroup wrote.  The barrier's
+// workgroup-scope release / acquire makes them visible: the workgroup runs on
+// one CU, whose L1 all its waves share, so the x loads and stores are
+// workgroup-scope accesses (device-scope ones would go past the XCD's L2 to
+// memory: 29.7 ms against 17.4 ms per ILU(0) GMRES iteration on the 1024^2
+// convection-diffusion matrix, measured).  While a level
 // is being solved, each thread already loads its rows of the next level (row
 // index, entry range, the first kLvPre entries, the right-hand side), so after
 // a barrier only the loads of the x entries a row needs stand between the
@@ -157,11 +167,11 @@ struct LvRow
 
 __device__ __forceinline__ double coherent_load( const double* p )
 {
-   return __hip_atomic_load( p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+   return __hip_atomic_load( p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
 }
 __device__ __forceinline__ void coherent_store( double* p, double v )
 {
-   __hip_atomic_store( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+   __hip_atomic_store( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
 }
 
 template< bool UPPER >
@@ -197,6 +207,95 @@ __device__ __forceinline__ void lv_load( LvRow ( &R )[ kLvRows ], const int32_t*
                w.v[ q ] = val[ w.g0 + q ];
             }
       }
+   }
+}
+
+// The LDS-ring form of tri_levels: the solution entries of the run's latest
+// kRing positions (in level order) stay in LDS, so a row reads the entries
+// it depends on from LDS (those before the run from global memory, written by
+// earlier launches) and the levels synchronise through LDS alone -- the
+// barrier waits for no global store.  Used for a run in which every
+// dependency inside the run lies within kRing positions of its level's end
+// (a stencil's: within two levels), checked when the factors are built.
+constexpr int kRing = 8192;
+
+struct RingRow
+{
+   int32_t p, i, g0, g1;  // position (-1: none), row, entries [g0, g1)
+   int32_t q[ kLvPre ], c[ kLvPre ];
+   double v[ kLvPre ];
+   double bi, dii;
+};
+
+template< bool UPPER >
+__device__ __forceinline__ void ring_load( RingRow ( &R )[ kLvRows ], const int32_t* __restrict__ rows, int64_t p0,
+                                           int64_t p1, const int32_t* __restrict__ pp,
+                                           const int32_t* __restrict__ dep, const int32_t* __restrict__ col,
+                                           const double* __restrict__ val, const double* __restrict__ diag,
+                                           const double* b, const double* x )
+{
+#pragma unroll
+   for( int r = 0; r < kLvRows; r++ ) {
+      const int64_t p = p0 + threadIdx.x + (int64_t) r * kLvThreads;
+      RingRow& w = R[ r ];
+      w.p = -1;
+      if( p < p1 ) {
+         w.p = (int32_t) p;
+         w.i = rows[ p ];
+         w.g0 = pp[ p ];
+         w.g1 = pp[ p + 1 ];
+         w.bi = UPPER ? x[ w.i ] : b[ w.i ];
+         w.dii = UPPER ? diag[ p ] : 1.0;
+#pragma unroll
+         for( int q = 0; q < kLvPre; q++ )
+            if( w.g0 + q < w.g1 ) {
+               w.q[ q ] = dep[ w.g0 + q ];
+               w.c[ q ] = col[ w.g0 + q ];
+               w.v[ q ] = val[ w.g0 + q ];
+            }
+      }
+   }
+}
+
+template< bool UPPER >
+__global__ void __launch_bounds__( kLvThreads )
+   tri_ring( const int32_t* __restrict__ rows, const int64_t* __restrict__ bounds, int64_t l0, int64_t l1,
+             const int32_t* __restrict__ pp, const int32_t* __restrict__ dep, const int32_t* __restrict__ col,
+             const double* __restrict__ val, const double* __restrict__ diag, const double* b, double* x )
+{
+   __shared__ double ring[ kRing ];
+   const int64_t P0 = bounds[ l0 ];  // the run's first position: dependencies before it are in x
+   RingRow cur[ kLvRows ], nxt[ kLvRows ];
+   ring_load< UPPER >( cur, rows, bounds[ l0 ], bounds[ l0 + 1 ], pp, dep, col, val, diag, b, x );
+   auto xdep = [ & ]( int32_t q, int32_t c ) -> double { return q >= P0 ? ring[ q & ( kRing - 1 ) ] : x[ c ]; };
+   for( int64_t l = l0; l < l1; l++ ) {  // uniform bounds: every thread meets every barrier
+      if( l + 1 < l1 )
+         ring_load< UPPER >( nxt, rows, bounds[ l + 1 ], bounds[ l + 2 ], pp, dep, col, val, diag, b, x );
+#pragma unroll
+      for( int r = 0; r < kLvRows; r++ ) {
+         const RingRow& w = cur[ r ];
+         if( w.p < 0 )
+            continue;
+         double xi = w.bi;
+#pragma unroll
+         for( int q = 0; q < kLvPre; q++ )
+            if( w.g0 + q < w.g1 )
+               xi -= w.v[ q ] * xdep( w.q[ q ], w.c[ q ] );
+         for( int32_t g = w.g0 + kLvPre; g < w.g1; g++ )
+            xi -= val[ g ] * xdep( dep[ g ], col[ g ] );
+         if( UPPER )
+            xi = xi / w.dii;
+         ring[ w.p & ( kRing - 1 ) ] = xi;
+         x[ w.i ] = xi;
+      }
+      // LDS alone orders the levels: the global stores of x are read back by
+      // no later level of this launch
+      __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
+#pragma unroll
+      for( int r = 0; r < kLvRows; r++ )
+         cur[ r ] = nxt[ r ];
    }
 }
 
@@ -387,6 +486,57 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
    };
    plan_runs( P.llev, P.lruns );
    plan_runs( P.ulev, P.uruns );
+   // the factors in level-order positions, and which narrow runs may keep
+   // their recent solution entries in the LDS ring
+   auto positions = [ & ]( bool upper, const std::vector< int32_t >& order, const std::vector< int64_t >& bounds,
+                           std::vector< tnlmi_preconditioner::Run >& runs, DeviceArray< int32_t >& dpp,
+                           DeviceArray< int32_t >& ddep, DeviceArray< int32_t >& dcol, DeviceArray< double >& dval,
+                           DeviceArray< double >* ddiag ) {
+      const std::vector< int32_t >& ptr = upper ? up : lp;
+      const std::vector< int32_t >& cc = upper ? uc : lc;
+      const std::vector< double >& vv = upper ? uv : lv;
+      std::vector< int32_t > pos( N );
+      for( int64_t p = 0; p < N; p++ )
+         pos[ order[ p ] ] = (int32_t) p;
+      std::vector< int32_t > pp( N + 1, 0 ), dep, col;
+      std::vector< double > val, diag;
+      for( int64_t p = 0; p < N; p++ ) {
+         const int64_t i = order[ p ];
+         const int64_t k = upper ? N - 1 - i : i;
+         int32_t g = ptr[ k ];
+         if( upper ) {
+            diag.push_back( g < ptr[ k + 1 ] ? vv[ g ] : 0.0 );  // a row without entries has no pivot
+            g++;
+         }
+         for( ; g < ptr[ k + 1 ]; g++ ) {
+            dep.push_back( pos[ cc[ g ] ] );
+            col.push_back( cc[ g ] );
+            val.push_back( vv[ g ] );
+         }
+         pp[ p + 1 ] = (int32_t) dep.size();
+      }
+      for( auto& R : runs ) {
+         R.ring = R.narrow;
+         if( ! R.narrow )
+            continue;
+         const int64_t P0 = bounds[ R.l0 ];
+         for( int64_t l = R.l0; l < R.l1 && R.ring; l++ )
+            for( int64_t p = bounds[ l ]; p < bounds[ l + 1 ] && R.ring; p++ )
+               for( int32_t g = pp[ p ]; g < pp[ p + 1 ]; g++ )
+                  if( dep[ g ] >= P0 && bounds[ l + 1 ] - dep[ g ] > kRing ) {
+                     R.ring = false;
+                     break;
+                  }
+      }
+      upload_vec( dpp, pp, s );
+      upload_vec( ddep, dep, s );
+      upload_vec( dcol, col, s );
+      upload_vec( dval, val, s );
+      if( ddiag )
+         upload_vec( *ddiag, diag, s );
+   };
+   positions( false, lo, P.llev, P.lruns, P.lpp, P.ldep, P.lcol, P.lval, nullptr );
+   positions( true, uo, P.ulev, P.uruns, P.upp, P.udep, P.ucol, P.uval, &P.udiag );
    upload_vec( P.dllev, P.llev, s );
    upload_vec( P.dulev, P.ulev, s );
    upload_vec( P.lp, lp, s );
@@ -419,6 +569,11 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
    }
    // ILU0.hpp:99-113: triangularSolveLower then triangularSolveUpper<reversed rows>
    for( const auto& R : P->lruns ) {
+      if( R.ring ) {
+         hipLaunchKernelGGL( tri_ring< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, P->dllev.ptr, R.l0,
+                             R.l1, P->lpp.ptr, P->ldep.ptr, P->lcol.ptr, P->lval.ptr, (const double*) nullptr, b, x );
+         continue;
+      }
       if( R.narrow ) {
          hipLaunchKernelGGL( tri_levels< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, P->dllev.ptr,
                              R.l0, R.l1, n, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
@@ -429,6 +584,12 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
                           P->lrows.ptr + P->llev[ R.l0 ], cnt, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
    }
    for( const auto& R : P->uruns ) {
+      if( R.ring ) {
+         hipLaunchKernelGGL( tri_ring< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, P->dulev.ptr, R.l0,
+                             R.l1, P->upp.ptr, P->udep.ptr, P->ucol.ptr, P->uval.ptr, P->udiag.ptr,
+                             (const double*) x, x );
+         continue;
+      }
       if( R.narrow ) {
          hipLaunchKernelGGL( tri_levels< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, P->dulev.ptr,
                              R.l0, R.l1, n, P->up.ptr, P->uc.ptr, P->uv.ptr, (const double*) x, x );

@@ -42,8 +42,8 @@ namespace {
 constexpr int kWinDefault = 8192;    // x entries per window
 constexpr int kP1 = 1024;            // phase-1 threads
 constexpr int kP2 = 512;             // phase-2 threads
-constexpr int kRowsMax = 1024;       // rows per row block at most
-constexpr int kLongMax = 8;          // rows longer than kLongRow per row block at most (one per wave)
+// a row block has at most 2 kP2 rows (two per thread) and kP2 / 64 rows longer
+// than kLongRow (one per wave)
 constexpr int kLongRow = 256;        // longer rows are summed by a whole wave
 constexpr int kPad = 32;             // padding entries of prod and dest (phase-2 reads past a segment)
 
@@ -131,7 +131,10 @@ __global__ void w_seg( int64_t B, int32_t K, const int64_t* __restrict__ E, cons
 }
 
 // phase 1: prod[i] = v[i] * x[window base + pc[i]] for the window's nonzeros
-template< typename V, typename C, int WIN >
+// (U nonzeros per thread and batch; a software-pipelined form that requests
+// the next batch before the current one's products are stored measured
+// slower: 272 against 235 us per power-law SpMV)
+template< typename V, typename C, int WIN, int U >
 __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t* __restrict__ wstart,
                                                    const V* __restrict__ pv, const uint16_t* __restrict__ pc,
                                                    const C* __restrict__ x, C* __restrict__ prod )
@@ -145,7 +148,6 @@ __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t*
    const int64_t a = wstart[ k ], b = wstart[ k + 1 ];
    // plain (cached) loads and stores: the nontemporal forms measured slower
    // here, the stored products in particular (phase 2 reads them next)
-   constexpr int U = 4;
    for( int64_t i0 = a + threadIdx.x; i0 < b; i0 += U * kP1 ) {
       C p[ U ];
 #pragma unroll
@@ -221,15 +223,15 @@ __device__ __forceinline__ void get_pair( const C* prod, const uint16_t* dest, i
 
 template< typename C, int PER >
 __device__ __forceinline__ void put_pair( C* buf, const C ( &v )[ PER ], const uint32_t ( &d )[ PER / 2 ], int32_t i,
-                                          int32_t z )
+                                          int32_t z, int32_t a = INT32_MIN )
 {
 #pragma unroll
    for( int u = 0; u < PER; u++ )
-      if( i + u < z )
+      if( i + u < z && i + u >= a )
          buf[ ( d[ u / 2 ] >> ( 16 * ( u % 2 ) ) ) & 0xffff ] = v[ u ];
 }
 
-template< typename C, int CAP, bool BETA, int SEGS, int PER >
+template< typename C, int CAP, bool BETA, int SEGS, int PER, bool ALIGN >
 __global__ void __launch_bounds__( kP2 )
    w_phase2( int64_t B, int32_t K, const BlockHdr* __restrict__ hdr, const int32_t* __restrict__ seg,
              const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ off,
@@ -261,9 +263,9 @@ __global__ void __launch_bounds__( kP2 )
    const int32_t kt = sx < K ? sx : K - 1;
    const int32_t a = seg[ b * K + kt ];
    const int32_t z = sx < K ? seg[ ( b + 1 ) * K + kt ] : a;
-   // the rows of this thread (a block has at most kRowsMax) and of its wave's
+   // the rows of this thread (a block has at most 2 P2) and of its wave's
    // long row (at most one per wave); unconditional loads at clamped indexes
-   constexpr int J = kRowsMax / P2;
+   constexpr int J = 2;
    int32_t ob[ J ], oe[ J ];
    const int32_t last = std::max( H.r0, H.r1 - 1 );
 #pragma unroll
@@ -274,7 +276,11 @@ __global__ void __launch_bounds__( kP2 )
    }
    const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
    const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
-   // round i: the group's segment i, kSpan terms
+   // round i: the group's segment i, kSpan terms.  ALIGN: a group's reads
+   // start at the segment's first term rounded down to a multiple of PER, so
+   // every lane's PER products (8 PER bytes) and slots (2 PER bytes) are
+   // naturally aligned; a term before the segment is read but not stored.
+   constexpr int32_t kAl = ALIGN ? ~( PER - 1 ) : ~0;
    C v[ R ][ PER ];
    uint32_t d[ R ][ PER / 2 ];
 #pragma unroll
@@ -282,7 +288,8 @@ __global__ void __launch_bounds__( kP2 )
       // lanes past the segment's end read its last term again (no extra
       // cache line per request); what they read is not stored
       const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
-      get_pair< C, PER >( prod, dest, std::min( ai + PER * gl, std::max( zi - 1, ai ) ), v[ i ], d[ i ] );
+      get_pair< C, PER >( prod, dest, std::min( ( ai & kAl ) + PER * gl, std::max( zi - 1, ai ) & kAl ), v[ i ],
+                          d[ i ] );
    }
    // clear (padding slots, column -1, have no product: they read as zero)
    const int32_t len = H.e1 - H.e0;
@@ -292,9 +299,10 @@ __global__ void __launch_bounds__( kP2 )
 #pragma unroll
    for( int i = 0; i < R; i++ ) {
       const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
-      put_pair< C, PER >( buf, v[ i ], d[ i ], ai + PER * gl, zi );
+      const int32_t a0 = ai & kAl;
+      put_pair< C, PER >( buf, v[ i ], d[ i ], a0 + PER * gl, zi, ai );
       // on demand: the rest of a segment longer than kSpan
-      for( int32_t i0 = ai + kSpan + PER * gl; i0 < zi; i0 += kSpan ) {
+      for( int32_t i0 = a0 + kSpan + PER * gl; i0 < zi; i0 += kSpan ) {
          C w[ PER ];
          uint32_t dw[ PER / 2 ];
          get_pair< C, PER >( prod, dest, i0, w, dw );
@@ -307,11 +315,11 @@ __global__ void __launch_bounds__( kP2 )
       const int32_t ak = seg[ b * K + k ], zk = k0 + sx < K ? seg[ ( b + 1 ) * K + k ] : ak;
       for( int i = 0; i < R; i++ ) {
          const int32_t ai = __shfl( ak, gb + i, 64 ), zi = __shfl( zk, gb + i, 64 );
-         for( int32_t i0 = ai + PER * gl; i0 < zi; i0 += kSpan ) {
+         for( int32_t i0 = ( ai & kAl ) + PER * gl; i0 < zi; i0 += kSpan ) {
             C w[ PER ];
             uint32_t dw[ PER / 2 ];
             get_pair< C, PER >( prod, dest, i0, w, dw );
-            put_pair< C, PER >( buf, w, dw, i0, zi );
+            put_pair< C, PER >( buf, w, dw, i0, zi, ai );
          }
       }
    }
@@ -401,11 +409,11 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
          r++;
          continue;
       }
-      // at most cap nonzeros, kRowsMax rows and kLongMax rows longer than kLongRow
+      // at most cap nonzeros, 2 kP2 rows and kP2 / 64 rows longer than kLongRow
       int64_t e = r, longs = 0;
-      while( e < n && hoff[ e + 1 ] - start <= cap && e - r < kRowsMax ) {
+      while( e < n && hoff[ e + 1 ] - start <= cap && e - r < 2 * kP2 ) {
          const bool lg = hoff[ e + 1 ] - hoff[ e ] > kLongRow;
-         if( lg && longs == kLongMax )
+         if( lg && longs == kP2 / 64 )
             break;
          longs += lg;
          e++;
@@ -482,18 +490,29 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
 template< typename C, int WIN >
 void phase1( const WindowPlan& W, const C* x, hipStream_t s )
 {
-   hipLaunchKernelGGL( ( w_phase1< C, C, WIN > ), dim3( (unsigned) W.K ), dim3( kP1 ), 0, s, W.columns, W.wstart.ptr,
-                       (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
+   if( W.unroll == 8 )
+      hipLaunchKernelGGL( ( w_phase1< C, C, WIN, 8 > ), dim3( (unsigned) W.K ), dim3( kP1 ), 0, s, W.columns,
+                          W.wstart.ptr, (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
+   else
+      hipLaunchKernelGGL( ( w_phase1< C, C, WIN, 4 > ), dim3( (unsigned) W.K ), dim3( kP1 ), 0, s, W.columns,
+                          W.wstart.ptr, (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
 }
 
 template< typename C, int SEGS, int PER >
 void phase2( const WindowPlan& W, C* y, double alpha, double beta, int64_t begin, int64_t end, hipStream_t s )
 {
-   hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER >
-                                     : w_phase2< C, cap_of< C >(), true, SEGS, PER > ),
-                       dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
-                       (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y, (C) alpha,
-                       (C) beta, begin, end );
+   if( W.align )
+      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, true >
+                                        : w_phase2< C, cap_of< C >(), true, SEGS, PER, true > ),
+                          dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
+                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y, (C) alpha,
+                          (C) beta, begin, end );
+   else
+      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, false >
+                                        : w_phase2< C, cap_of< C >(), true, SEGS, PER, false > ),
+                          dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
+                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y, (C) alpha,
+                          (C) beta, begin, end );
 }
 
 template< typename C >
@@ -531,8 +550,9 @@ void window_build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    W.columns = A.columns;
    W.nnz = A.segments->elements;
    W.value_type = A.value_type;
-   // window width and phase-2 shape: TNLMI_WINDOW_COLS (8192, 10240, 16384,
-   // 20480) and TNLMI_WINDOW_PER (2, 4) in the environment override them
+   // window width and kernel shapes: TNLMI_WINDOW_COLS (8192, 10240, 16384,
+   // 20480), TNLMI_WINDOW_PER (phase 2: 2, 4) and TNLMI_WINDOW_UNROLL (phase 1:
+   // 4, 8) in the environment override them
    W.win = kWinDefault;
    if( const char* e = std::getenv( "TNLMI_WINDOW_COLS" ) ) {
       const int v = std::atoi( e );
@@ -545,6 +565,15 @@ void window_build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    if( const char* e = std::getenv( "TNLMI_WINDOW_PER" ) )
       if( std::atoi( e ) == 4 && W.segs == 256 )
          W.per = 4;
+   W.unroll = 4;
+   if( const char* e = std::getenv( "TNLMI_WINDOW_UNROLL" ) )
+      if( std::atoi( e ) == 8 )
+         W.unroll = 8;
+   // TNLMI_WINDOW_ALIGN=0: phase-2 reads start at each segment's first term
+   // instead of the aligned position before it
+   W.align = true;
+   if( const char* e = std::getenv( "TNLMI_WINDOW_ALIGN" ) )
+      W.align = e[ 0 ] != '0';
    W.offs.allocate( A.rows + 1 );
    TNLMI_HIP( hipMemcpyAsync( W.offs.ptr, A.segments->offsets.ptr, sizeof( int32_t ) * ( A.rows + 1 ),
                               hipMemcpyDeviceToDevice, s ) );

@@ -12,6 +12,8 @@ struct WindowPlan
    int32_t win = 8192;  // columns per window (x entries staged in LDS by phase 1)
    int32_t segs = 512;  // phase 2: windows per sweep of a row block (256 or 512)
    int32_t per = 2;     // phase 2: consecutive terms per lane and load
+   int32_t unroll = 4;  // phase 1: nonzeros per thread and batch
+   bool align = true;    // phase 2 reads products and slots from aligned positions
    int32_t K = 0;    // column windows
    int64_t B = 0;     // row blocks
    int64_t H = 0;     // rows longer than cap (cut into pieces)

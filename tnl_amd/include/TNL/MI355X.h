@@ -31,6 +31,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <initializer_list>
@@ -617,9 +618,39 @@ template< typename Real, typename Index, typename MatrixType, template< typename
           typename ComputeReal, typename RealAllocator, typename IndexAllocator >
 class SparseMatrix< Real, Devices::MI355X, Index, MatrixType, Segments, ComputeReal, RealAllocator, IndexAllocator >
 {
-   // the device kernels take int32 offsets and column indexes (TNL's default
-   // Index = int); 64-bit indices are not provided
-   static_assert( std::is_same_v< Index, int >, "SparseMatrix on Devices::MI355X supports Index = int only" );
+   // Index = int or long.  The API's row, column and capacity values are of
+   // type Index; the device arrays (offsets, column indexes) are 32-bit, as the
+   // kernels read them: a matrix whose rows, columns or storage do not fit
+   // int32 raises std::overflow_error, as the reference's Ellpack does for
+   // its own int32 overflow (Ellpack.hpp:91-95).
+   static_assert( std::is_same_v< Index, int > || std::is_same_v< Index, long >
+                     || std::is_same_v< Index, long long >,
+                  "SparseMatrix on Devices::MI355X supports Index = int or long" );
+   using DeviceIndexVector = Containers::Vector< int32_t, Devices::MI355X, int32_t >;
+   // host index values -> the 32-bit device array (std::overflow_error if one does not fit)
+   static DeviceIndexVector toDevice32( const std::vector< Index >& v, const char* what )
+   {
+      DeviceIndexVector d;
+      if constexpr( std::is_same_v< Index, int32_t > )
+         d = v;
+      else {
+         std::vector< int32_t > w( v.size() );
+         for( std::size_t i = 0; i < v.size(); i++ ) {
+            if( v[ i ] < INT32_MIN || v[ i ] > INT32_MAX )
+               throw std::overflow_error( std::string( what ) + ": the value " + std::to_string( v[ i ] )
+                                          + " does not fit the 32-bit device indexes" );
+            w[ i ] = (int32_t) v[ i ];
+         }
+         d = w;
+      }
+      return d;
+   }
+   static void checkDimension( Index n, const char* what )
+   {
+      if( n > (Index) INT32_MAX )
+         throw std::overflow_error( std::string( what ) + " " + std::to_string( n )
+                                    + " does not fit the 32-bit device indexes" );
+   }
    // the segments type with this device (only named, never instantiated: its
    // template arguments -- e.g. the default ElementsOrganization, ColumnMajorOrder
    // on every device but Host (ElementsOrganization.h:21-32) -- select the format)
@@ -721,7 +752,7 @@ public:
       release();
       rows_ = B.getRows();
       columns_ = B.getColumns();
-      RowCapacitiesType len( std::max< Index >( 1, rows_ ) );
+      DeviceIndexVector len( std::max< int32_t >( 1, (int32_t) rows_ ) );
       MI355XDetail::check( tnlmi_matrix_row_lengths( &src, len.getData(), nullptr ) );
       MI355XDetail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param,
                                                     len.getData(), rows_, nullptr, &segments_ ) );
@@ -735,6 +766,8 @@ public:
    {
       if( rows < 0 || columns < 0 )
          throw std::invalid_argument( "setDimensions: negative dimensions" );
+      checkDimension( rows, "setDimensions: rows" );
+      checkDimension( columns, "setDimensions: columns" );
       rows_ = rows;
       columns_ = columns;
       setRowCapacities( std::vector< Index >( rows, 0 ) );
@@ -919,7 +952,7 @@ public:
    //! the values and column indexes on the device (an extension of this
    //! device: writing through them changes the matrix)
    Real* getValuesData() const { return values_; }
-   Index* getColumnIndexesData() const { return columnIndexes_; }
+   int32_t* getColumnIndexesData() const { return columnIndexes_; }
 
    //! getSerializationType (SparseMatrixBase.hpp:46-53)
    static std::string getSerializationType()
@@ -979,16 +1012,29 @@ private:
    template< typename V >
    void rowCounts( int kind, V& out ) const
    {
-      RowCapacitiesType c( rows_ );
+      DeviceIndexVector c( std::max< int32_t >( 1, (int32_t) rows_ ) );
       const tnlmi_matrix m = abi();
       MI355XDetail::check( tnlmi_matrix_row_counts( &m, kind, c.getData(), nullptr ) );
-      if constexpr( std::is_same_v< typename V::DeviceType, Devices::MI355X > )
-         out = c.getConstView();
-      else {
+      using OutIndex = std::remove_const_t< typename V::RealType >;
+      if constexpr( std::is_same_v< typename V::DeviceType, Devices::MI355X >
+                    && std::is_same_v< OutIndex, int32_t > ) {
          out.setSize( rows_ );
+         if( rows_ > 0 )
+            MI355XDetail::hip( hipMemcpy( out.getData(), c.getData(), sizeof( int32_t ) * rows_,
+                                          hipMemcpyDeviceToDevice ) );
+      }
+      else {
          const auto h = c.toHost();
+         std::vector< OutIndex > w( (std::size_t) rows_ );
          for( Index r = 0; r < rows_; r++ )
-            out[ r ] = h[ r ];
+            w[ r ] = (OutIndex) h[ r ];
+         if constexpr( std::is_same_v< typename V::DeviceType, Devices::MI355X > )
+            out = w;
+         else {
+            out.setSize( rows_ );
+            for( Index r = 0; r < rows_; r++ )
+               out[ r ] = w[ r ];
+         }
       }
    }
    // the rows' segment sizes on the host (getRowCapacity)
@@ -1005,7 +1051,7 @@ private:
    {
       const std::size_t storage = (std::size_t) std::max< int64_t >( 1, tnlmi_segments_storage_size( segments_ ) );
       values_ = RealAllocator().allocate( storage );
-      columnIndexes_ = IndexAllocator().allocate( storage );
+      columnIndexes_ = Allocators::MI355X< int32_t >().allocate( storage );  // 32-bit on the device
       storage_ = storage;
    }
    void release()
@@ -1015,7 +1061,7 @@ private:
       if( values_ )
          RealAllocator().deallocate( values_, storage_ );
       if( columnIndexes_ )
-         IndexAllocator().deallocate( columnIndexes_, storage_ );
+         Allocators::MI355X< int32_t >().deallocate( columnIndexes_, storage_ );
       segments_ = nullptr;
       values_ = nullptr;
       columnIndexes_ = nullptr;
@@ -1025,11 +1071,10 @@ private:
                   const std::vector< Real >& vals )
    {
       release();
-      RowCapacitiesType d_caps, d_off, d_cols;
+      DeviceIndexVector d_caps = toDevice32( caps, "row capacity" );
+      DeviceIndexVector d_off = toDevice32( off, "storage offset" );
+      DeviceIndexVector d_cols = toDevice32( cols, "column index" );
       ValuesVectorType d_vals;
-      d_caps = caps;
-      d_off = off;
-      d_cols = cols;
       d_vals = vals;
       MI355XDetail::check( tnlmi_segments_create( SegTag::format, SegTag::organization, SegTag::param,
                                                     d_caps.getData(), rows_, nullptr, &segments_ ) );
@@ -1044,7 +1089,7 @@ private:
    Index rows_ = 0, columns_ = 0;
    tnlmi_segments* segments_ = nullptr;
    Real* values_ = nullptr;
-   Index* columnIndexes_ = nullptr;
+   int32_t* columnIndexes_ = nullptr;
    std::size_t storage_ = 0;
    std::vector< Index > hoff_{ 0 };  // prefix sums of the row capacities
 };
