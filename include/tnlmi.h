@@ -363,6 +363,17 @@ tnlmi_status tnlmi_spmv_plan_execute( const tnlmi_spmv_plan* plan, tnlmi_dtype x
  * ExpressionTemplates.h:372-384, :442-460 (the C++ layer reads the scalar). */
 tnlmi_status tnlmi_dot( int64_t n, const double* d_a, const double* d_b, double* d_result, tnlmi_stream stream );
 tnlmi_status tnlmi_nrm2( int64_t n, const double* d_a, double* d_result, tnlmi_stream stream );
+/* tnlmi_dot without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels, grid and
+ * summation order as tnlmi_dot (bit-identical result); the first pass's
+ * partial sums go to a static device array of the library (1024 FP64 =
+ * 8 KiB per GPU, held until the process ends) instead of a workspace the
+ * library allocates on first use.  Any n >= 0 fits (the first pass has at
+ * most 1024 workgroups).  Calls that overlap on one GPU share that array and
+ * race: serialise them (one stream, or events). */
+tnlmi_status tnlmi_dot_noalloc( int64_t n, const double* d_a, const double* d_b, double* d_result,
+                                tnlmi_stream stream );
 /* y = alpha*x + beta*y (Containers/detail/VectorAssignment.h expressions) */
 tnlmi_status tnlmi_axpby( int64_t n, double alpha, const double* d_x, double beta, double* d_y,
                           tnlmi_stream stream );

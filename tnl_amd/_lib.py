@@ -133,6 +133,7 @@ _PROTOS = {
                                                ctypes.c_double, c_i64, c_i64, c_vp]),
     "tnlmi_dot": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "tnlmi_nrm2": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp]),
+    "tnlmi_dot_noalloc": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "tnlmi_axpby": (ctypes.c_int, [c_i64, ctypes.c_double, c_vp, ctypes.c_double, c_vp, c_vp]),
     "tnlmi_preconditioner_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Matrix), c_vp, ctypes.POINTER(c_vp)]),
     "tnlmi_preconditioner_destroy": (None, [c_vp]),

@@ -2,6 +2,7 @@
 // plus their C ABI (TNL's Containers/Expressions BLAS-1 on the new device).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "blas1.hpp"
 
@@ -251,6 +252,40 @@ void fill( int64_t n, double v, double* y, hipStream_t s )
    hipLaunchKernelGGL( fill_k, vgrid( n ), dim3( 256 ), 0, s, n, v, y );
 }
 
+// ---------------------------------------------- allocation-free (a, b)
+// The first pass's partial sums of tnlmi_dot_noalloc live in this file-scope
+// device array: kRedBlocks FP64 (8 KiB), one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in host
+// code the name is a stand-in, not a device address): it passes the address
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device and kept in scratch_dev.
+static __device__ __align__( 256 ) double scratch[ kRedBlocks ];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+double* scratch_dev[ kMaxDevices ] = {};
+std::mutex scratch_mutex;
+
+double* scratch_ptr()
+{
+   int dev = 0;
+   TNLMI_HIP( hipGetDevice( &dev ) );
+   if( dev < 0 || dev >= kMaxDevices )
+      fail( TNLMI_RUNTIME_ERROR, "dot_noalloc: device ordinal out of range" );
+   std::lock_guard< std::mutex > lock( scratch_mutex );
+   if( ! scratch_dev[ dev ] ) {
+      void* p = nullptr;
+      TNLMI_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ) );
+      if( ! p )
+         fail( TNLMI_RUNTIME_ERROR, "dot_noalloc: no device address for the scratch array" );
+      scratch_dev[ dev ] = (double*) p;
+   }
+   return scratch_dev[ dev ];
+}
+
+}  // namespace
+
 }  // namespace tnlmi
 
 using namespace tnlmi;
@@ -266,6 +301,31 @@ tnlmi_status tnlmi_dot( int64_t n, const double* d_a, const double* d_b, double*
          fail( TNLMI_INVALID_ARGUMENT, "dot: invalid argument" );
       static thread_local RedWork w;
       dot( n, d_a, d_b, d_result, w, as_stream( stream ) );
+      TNLMI_HIP( hipGetLastError() );
+   } );
+}
+
+// tnlmi_dot without any device allocation: the same two kernels, the same
+// fixed grid and order (bit-identical results), with the partial sums in the
+// library's static device array instead of a workspace it allocates.  The
+// first pass's grid is at most kRedBlocks workgroups whatever n is, so every
+// n fits the array; a grid beyond it is refused all the same.  Calls that
+// overlap on one GPU share the array and race.
+tnlmi_status tnlmi_dot_noalloc( int64_t n, const double* d_a, const double* d_b, double* d_result,
+                                tnlmi_stream stream )
+{
+   return guarded( [ & ] {
+      if( n < 0 || ( n && ( ! d_a || ! d_b ) ) || ! d_result )
+         fail( TNLMI_INVALID_ARGUMENT, "dot_noalloc: invalid argument" );
+      const int g = red_grid( n );
+      if( g > kRedBlocks )
+         fail( TNLMI_INVALID_ARGUMENT, "dot_noalloc: more partial sums than the static scratch array holds" );
+      double* part = scratch_ptr();
+      const hipStream_t s = as_stream( stream );
+      // every partial that pass 2 reads (the first g) is written by pass 1:
+      // nothing of an earlier call's contents is read
+      hipLaunchKernelGGL( red_pass1< DotF >, dim3( g ), dim3( 256 ), 0, s, n, DotF{ d_a, d_b }, part );
+      hipLaunchKernelGGL( red_pass2, dim3( 1 ), dim3( 256 ), 0, s, part, g, d_result, 0 );
       TNLMI_HIP( hipGetLastError() );
    } );
 }
