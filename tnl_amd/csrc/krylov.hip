@@ -43,9 +43,9 @@ struct tnlmi_preconditioner
    };
    std::vector< Run > lruns, uruns;
    // the factors in level-order positions (for the LDS-ring solve): per position p
-   // (row lrows[p] / urows[p]) its entries [pp[p], pp[p+1]): dependency position,
-   // dependency row, value; the upper factor's pivot per position
-   DeviceArray< int32_t > lpp, ldep, lcol, upp, udep, ucol;
+   // (row lrows[p] / urows[p]) its entries [pp[p], pp[p+1]): dependency position
+   // and value; the upper factor's pivot per position
+   DeviceArray< int32_t > lpp, ldep, upp, udep;
    DeviceArray< double > lval, uval, udiag;
    tnlmi_matrix M{};                   // explicit: x = M b
 };
@@ -219,83 +219,92 @@ __device__ __forceinline__ void lv_load( LvRow ( &R )[ kLvRows ], const int32_t*
 // (a stencil's: within two levels), checked when the factors are built.
 constexpr int kRing = 8192;
 
+constexpr int kRingPre = 3;  // entries per row loaded ahead (a 7-point stencil's triangle has 3)
+
 struct RingRow
 {
    int32_t p, i, g0, g1;  // position (-1: none), row, entries [g0, g1)
-   int32_t q[ kLvPre ], c[ kLvPre ];
-   double v[ kLvPre ];
+   int32_t q[ kRingPre ];  // their dependency positions
+   double v[ kRingPre ];
    double bi, dii;
 };
 
 template< bool UPPER >
-__device__ __forceinline__ void ring_load( RingRow ( &R )[ kLvRows ], const int32_t* __restrict__ rows, int64_t p0,
-                                           int64_t p1, const int32_t* __restrict__ pp,
-                                           const int32_t* __restrict__ dep, const int32_t* __restrict__ col,
+__device__ __forceinline__ void ring_load( RingRow& w, const int32_t* __restrict__ rows, int64_t p0, int64_t p1,
+                                           const int32_t* __restrict__ pp, const int32_t* __restrict__ dep,
                                            const double* __restrict__ val, const double* __restrict__ diag,
                                            const double* b, const double* x )
 {
+   const int64_t p = p0 + threadIdx.x;
+   w.p = -1;
+   if( p < p1 ) {
+      w.p = (int32_t) p;
+      w.i = rows[ p ];
+      w.g0 = pp[ p ];
+      w.g1 = pp[ p + 1 ];
+      w.bi = UPPER ? x[ w.i ] : b[ w.i ];
+      w.dii = UPPER ? diag[ p ] : 1.0;
 #pragma unroll
-   for( int r = 0; r < kLvRows; r++ ) {
-      const int64_t p = p0 + threadIdx.x + (int64_t) r * kLvThreads;
-      RingRow& w = R[ r ];
-      w.p = -1;
-      if( p < p1 ) {
-         w.p = (int32_t) p;
-         w.i = rows[ p ];
-         w.g0 = pp[ p ];
-         w.g1 = pp[ p + 1 ];
-         w.bi = UPPER ? x[ w.i ] : b[ w.i ];
-         w.dii = UPPER ? diag[ p ] : 1.0;
-#pragma unroll
-         for( int q = 0; q < kLvPre; q++ )
-            if( w.g0 + q < w.g1 ) {
-               w.q[ q ] = dep[ w.g0 + q ];
-               w.c[ q ] = col[ w.g0 + q ];
-               w.v[ q ] = val[ w.g0 + q ];
-            }
-      }
+      for( int q = 0; q < kRingPre; q++ )
+         if( w.g0 + q < w.g1 ) {
+            w.q[ q ] = dep[ w.g0 + q ];
+            w.v[ q ] = val[ w.g0 + q ];
+         }
    }
 }
+
+// A level's rows are requested kRingDepth levels ahead (one row per thread:
+// ring runs have levels of at most kLvThreads rows), so their loads have
+// kRingDepth levels' time to arrive; the stages rotate through registers by
+// unrolling the level loop kRingDepth times.
+constexpr int kRingDepth = 4;
 
 template< bool UPPER >
 __global__ void __launch_bounds__( kLvThreads )
    tri_ring( const int32_t* __restrict__ rows, const int64_t* __restrict__ bounds, int64_t l0, int64_t l1,
-             const int32_t* __restrict__ pp, const int32_t* __restrict__ dep, const int32_t* __restrict__ col,
-             const double* __restrict__ val, const double* __restrict__ diag, const double* b, double* x )
+             const int32_t* __restrict__ pp, const int32_t* __restrict__ dep, const double* __restrict__ val,
+             const double* __restrict__ diag, const double* b, double* x )
 {
    __shared__ double ring[ kRing ];
    const int64_t P0 = bounds[ l0 ];  // the run's first position: dependencies before it are in x
-   RingRow cur[ kLvRows ], nxt[ kLvRows ];
-   ring_load< UPPER >( cur, rows, bounds[ l0 ], bounds[ l0 + 1 ], pp, dep, col, val, diag, b, x );
-   auto xdep = [ & ]( int32_t q, int32_t c ) -> double { return q >= P0 ? ring[ q & ( kRing - 1 ) ] : x[ c ]; };
-   for( int64_t l = l0; l < l1; l++ ) {  // uniform bounds: every thread meets every barrier
-      if( l + 1 < l1 )
-         ring_load< UPPER >( nxt, rows, bounds[ l + 1 ], bounds[ l + 2 ], pp, dep, col, val, diag, b, x );
+   RingRow st[ kRingDepth ];
+   auto load = [ & ]( RingRow& R, int64_t l ) {
+      if( l < l1 )
+         ring_load< UPPER >( R, rows, bounds[ l ], bounds[ l + 1 ], pp, dep, val, diag, b, x );
+      else
+         R.p = -1;
+   };
 #pragma unroll
-      for( int r = 0; r < kLvRows; r++ ) {
-         const RingRow& w = cur[ r ];
-         if( w.p < 0 )
-            continue;
-         double xi = w.bi;
+   for( int d = 0; d < kRingDepth; d++ )
+      load( st[ d ], l0 + d );
+   // a dependency before the run: its row's entry of x, written by an earlier launch
+   auto xdep = [ & ]( int32_t q ) -> double { return q >= P0 ? ring[ q & ( kRing - 1 ) ] : x[ rows[ q ] ]; };
+   for( int64_t l = l0; l < l1; l += kRingDepth ) {  // uniform bounds: every thread meets every barrier
 #pragma unroll
-         for( int q = 0; q < kLvPre; q++ )
-            if( w.g0 + q < w.g1 )
-               xi -= w.v[ q ] * xdep( w.q[ q ], w.c[ q ] );
-         for( int32_t g = w.g0 + kLvPre; g < w.g1; g++ )
-            xi -= val[ g ] * xdep( dep[ g ], col[ g ] );
-         if( UPPER )
-            xi = xi / w.dii;
-         ring[ w.p & ( kRing - 1 ) ] = xi;
-         x[ w.i ] = xi;
+      for( int d = 0; d < kRingDepth; d++ ) {
+         if( l + d < l1 ) {  // uniform
+            const RingRow w = st[ d ];
+            load( st[ d ], l + d + kRingDepth );  // the stage's next level, kRingDepth ahead
+            if( w.p >= 0 ) {
+               double xi = w.bi;
+#pragma unroll
+               for( int q = 0; q < kRingPre; q++ )
+                  if( w.g0 + q < w.g1 )
+                     xi -= w.v[ q ] * xdep( w.q[ q ] );
+               for( int32_t g = w.g0 + kRingPre; g < w.g1; g++ )
+                  xi -= val[ g ] * xdep( dep[ g ] );
+               if( UPPER )
+                  xi = xi / w.dii;
+               ring[ w.p & ( kRing - 1 ) ] = xi;
+               x[ w.i ] = xi;
+            }
+            // LDS alone orders the levels: the global stores of x are read
+            // back by no later level of this launch
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
+         }
       }
-      // LDS alone orders the levels: the global stores of x are read back by
-      // no later level of this launch
-      __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
-      __builtin_amdgcn_s_barrier();
-      __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
-#pragma unroll
-      for( int r = 0; r < kLvRows; r++ )
-         cur[ r ] = nxt[ r ];
    }
 }
 
@@ -490,7 +499,7 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
    // their recent solution entries in the LDS ring
    auto positions = [ & ]( bool upper, const std::vector< int32_t >& order, const std::vector< int64_t >& bounds,
                            std::vector< tnlmi_preconditioner::Run >& runs, DeviceArray< int32_t >& dpp,
-                           DeviceArray< int32_t >& ddep, DeviceArray< int32_t >& dcol, DeviceArray< double >& dval,
+                           DeviceArray< int32_t >& ddep, DeviceArray< double >& dval,
                            DeviceArray< double >* ddiag ) {
       const std::vector< int32_t >& ptr = upper ? up : lp;
       const std::vector< int32_t >& cc = upper ? uc : lc;
@@ -498,7 +507,7 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
       std::vector< int32_t > pos( N );
       for( int64_t p = 0; p < N; p++ )
          pos[ order[ p ] ] = (int32_t) p;
-      std::vector< int32_t > pp( N + 1, 0 ), dep, col;
+      std::vector< int32_t > pp( N + 1, 0 ), dep;
       std::vector< double > val, diag;
       for( int64_t p = 0; p < N; p++ ) {
          const int64_t i = order[ p ];
@@ -510,7 +519,6 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
          }
          for( ; g < ptr[ k + 1 ]; g++ ) {
             dep.push_back( pos[ cc[ g ] ] );
-            col.push_back( cc[ g ] );
             val.push_back( vv[ g ] );
          }
          pp[ p + 1 ] = (int32_t) dep.size();
@@ -519,6 +527,8 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
          R.ring = R.narrow;
          if( ! R.narrow )
             continue;
+         for( int64_t l = R.l0; l < R.l1 && R.ring; l++ )
+            R.ring = bounds[ l + 1 ] - bounds[ l ] <= kLvThreads;  // one row per thread
          const int64_t P0 = bounds[ R.l0 ];
          for( int64_t l = R.l0; l < R.l1 && R.ring; l++ )
             for( int64_t p = bounds[ l ]; p < bounds[ l + 1 ] && R.ring; p++ )
@@ -530,13 +540,12 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
       }
       upload_vec( dpp, pp, s );
       upload_vec( ddep, dep, s );
-      upload_vec( dcol, col, s );
       upload_vec( dval, val, s );
       if( ddiag )
          upload_vec( *ddiag, diag, s );
    };
-   positions( false, lo, P.llev, P.lruns, P.lpp, P.ldep, P.lcol, P.lval, nullptr );
-   positions( true, uo, P.ulev, P.uruns, P.upp, P.udep, P.ucol, P.uval, &P.udiag );
+   positions( false, lo, P.llev, P.lruns, P.lpp, P.ldep, P.lval, nullptr );
+   positions( true, uo, P.ulev, P.uruns, P.upp, P.udep, P.uval, &P.udiag );
    upload_vec( P.dllev, P.llev, s );
    upload_vec( P.dulev, P.ulev, s );
    upload_vec( P.lp, lp, s );
@@ -571,7 +580,7 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
    for( const auto& R : P->lruns ) {
       if( R.ring ) {
          hipLaunchKernelGGL( tri_ring< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, P->dllev.ptr, R.l0,
-                             R.l1, P->lpp.ptr, P->ldep.ptr, P->lcol.ptr, P->lval.ptr, (const double*) nullptr, b, x );
+                             R.l1, P->lpp.ptr, P->ldep.ptr, P->lval.ptr, (const double*) nullptr, b, x );
          continue;
       }
       if( R.narrow ) {
@@ -586,8 +595,7 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
    for( const auto& R : P->uruns ) {
       if( R.ring ) {
          hipLaunchKernelGGL( tri_ring< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, P->dulev.ptr, R.l0,
-                             R.l1, P->upp.ptr, P->udep.ptr, P->ucol.ptr, P->uval.ptr, P->udiag.ptr,
-                             (const double*) x, x );
+                             R.l1, P->upp.ptr, P->udep.ptr, P->uval.ptr, P->udiag.ptr, (const double*) x, x );
          continue;
       }
       if( R.narrow ) {
