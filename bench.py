@@ -677,7 +677,7 @@ def wl_gmres_convdiff(args):
     log(f"convection-diffusion {m}^2: {n} rows, {nnz} nnz, Peclet {pe:.1f}")
     res = {}
 
-    def run(label, make_pre, max_its):
+    def run(label, make_pre, max_its, progress=None):
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -685,7 +685,7 @@ def wl_gmres_convdiff(args):
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t0
         ok, its, r = spai.gmres_solve(A, b, x, pre, tolerance=tol, restart=30, max_iterations=max_its,
-                                      variant=args.gmres_variant)
+                                      variant=args.gmres_variant, log=progress)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         rec = {"seconds": round(t, 4), "setup_seconds": round(t_setup, 4), "converged": ok, "iterations": its,
@@ -705,7 +705,7 @@ def wl_gmres_convdiff(args):
     def learned():
         return solvers.Explicit(net.inverse(A)), {}
     run("learned_spai", learned, args.max_iterations)
-    run("ilu0", lambda: (solvers.ILU0(A), {}), args.ilu_max_iterations)
+    run("ilu0", lambda: (solvers.ILU0(A), {}), args.ilu_max_iterations, progress=lambda m: log(f"[ilu0] {m}"))
 
     # the nonsymmetric alternative on the same system: BiCGStab (BICGStab.hpp),
     # restarted from its own x with a tighter target until the true residual
@@ -850,9 +850,9 @@ def main():
     p.add_argument("--side", type=int, default=None, help="grid side (cg-poisson 256, gmres-convdiff 1024)")
     p.add_argument("--delta", type=float, default=0.1, help="cg-poisson: reliable-update threshold")
     p.add_argument("--systems", type=int, default=4096, help="batched-cg system count")
-    p.add_argument("--loop-sample", type=int, default=256, help="batched-cg: systems timed in the per-system loop")
+    p.add_argument("--loop-sample", type=int, default=4096, help="batched-cg: systems timed in the per-system loop")
     p.add_argument("--max-iterations", type=int, default=100_000, help="gmres-convdiff: GMRES iteration cap")
-    p.add_argument("--ilu-max-iterations", type=int, default=3000, help="gmres-convdiff: cap for ILU(0)")
+    p.add_argument("--ilu-max-iterations", type=int, default=20_000, help="gmres-convdiff: cap for ILU(0)")
     p.add_argument("--solve-seed", type=int, default=7)
     p.add_argument("--gmres-variant", default="CGSR", choices=["CGS", "CGSR", "MGS", "MGSR"])
     p.add_argument("--spai-instances", type=int, default=2000)

@@ -452,6 +452,69 @@ def test_mixed_cg_column_codes_identical(monkeypatch, fmt, param):
     assert np.array_equal(out[0][0], out[1][0]) and out[0][1:] == out[1][1:]
 
 
+def _cg_solve(A, b, inner=None, its=3000):
+    if inner is None:
+        s = S.CG()
+        s.setMatrix(A)
+        s.setPreconditioner(S.Diagonal(A))
+    else:
+        s = S.ReliableMixedCG()
+        s.setMatrix(A)
+        s.setInnerMatrix(inner)
+    s.setConvergenceResidue(1e-10)
+    s.setMaxIterations(its)
+    x = torch.zeros(A.rows, dtype=torch.float64, device="cuda")
+    assert s.solve(dev(b), x)
+    return x.cpu().numpy(), s.getIterations()
+
+
+@pytest.mark.parametrize("vt", [torch.float64, torch.float32, torch.bfloat16], ids=["f64", "f32", "bf16"])
+def test_cg_csr_stage_copy_matches_ellpack(orc, monkeypatch, vt):
+    # a CSR matrix whose rows are short and about equally long runs the fused
+    # Ellpack kernels on a column-major copy of its slots made by the solve:
+    # the result equals the same matrix stored as Ellpack bit for bit; with
+    # the copy switched off (the one-launch CSR kernel) it agrees to the
+    # solver's tolerance
+    off, cols, vals = tnl.poisson3d(19, 18, 16)  # 5,472 rows: not a multiple of 64
+    n = len(off) - 1
+    b = np.random.default_rng(31).standard_normal(n)
+    mats = {}
+    for fmt in (tnl.CSR, tnl.ELLPACK):
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals, fmt=fmt)
+        lo = None if vt == torch.float64 else tnl.SparseMatrix(A.rows, A.columns, A.segments, A.values.to(vt),
+                                                                A.column_indexes)
+        mats[fmt] = (A, lo)
+    x_csr, it_csr = _cg_solve(*mats[tnl.CSR][:1], b, inner=mats[tnl.CSR][1])
+    x_ell, it_ell = _cg_solve(*mats[tnl.ELLPACK][:1], b, inner=mats[tnl.ELLPACK][1])
+    assert it_csr == it_ell and np.array_equal(x_csr, x_ell)
+    monkeypatch.setenv("TNLMI_CG_CSR_COPY", "0")
+    x_one, it_one = _cg_solve(*mats[tnl.CSR][:1], b, inner=mats[tnl.CSR][1])
+    assert abs(it_one - it_csr) <= 2
+    np.testing.assert_allclose(x_one, x_csr, rtol=1e-7, atol=1e-9)
+    assert orc.true_residual(off, cols, vals, x_one, b) <= 1e-9
+
+
+def test_cg_csr_uneven_rows_one_launch_kernel(orc):
+    # rows of 1..~20 entries: an Ellpack copy would be much larger than the
+    # CSR arrays, so the solve keeps the one-launch CSR kernel
+    import scipy.sparse as sp
+    n = 4000
+    R = sp.random(n, n, density=3.0 / n, random_state=np.random.RandomState(5), format="csr")
+    R.data[:] = -np.abs(R.data)
+    M = (R + R.T).tocsr()
+    M = (M + sp.diags(np.asarray(abs(M).sum(axis=1)).ravel() + 1.0)).tocsr()
+    M.sort_indices()
+    lens = np.diff(M.indptr)
+    assert lens.max() <= 32 and lens.max() * n > 1.2 * M.nnz
+    off, cols, vals = M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64)
+    A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+    b = np.random.default_rng(32).standard_normal(n)
+    x, its = _cg_solve(A, b)
+    xo, so = orc.cg(off, cols, vals, b, precond=1, tol=1e-10)
+    assert abs(its - so.iterations) <= 1
+    np.testing.assert_allclose(x, xo, rtol=1e-7, atol=1e-9)
+
+
 def test_mixed_cg_many_offsets_keeps_int32_columns(orc):
     # more than 255 distinct offsets column - row: no codes, the int32 kernel
     import scipy.sparse as sp

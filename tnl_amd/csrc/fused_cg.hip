@@ -716,6 +716,13 @@ struct Stage
    int64_t grid = 0;                      // blocks of the fused SpMV kernel
    int rows_per_thread = 1;               // Ellpack / SlicedEllpack kernel
    std::shared_ptr< DeviceArray< double > > part;  // its per-block partial sums
+   // the slots kind 1 reads: the matrix's own, or (a CSR matrix with short
+   // rows) the stage's column-major Ellpack copy of them
+   const int32_t* cols = nullptr;
+   const void* vals = nullptr;
+   int64_t storage = 0;
+   std::shared_ptr< DeviceArray< int32_t > > ecols;
+   std::shared_ptr< DeviceArray< char > > evals;
    // kind 1 with 8-bit column codes (the inner matrix of the mixed solver)
    std::shared_ptr< DeviceArray< uint8_t > > codes;
    std::shared_ptr< DeviceArray< int32_t > > dict;
@@ -730,8 +737,7 @@ void make_codes( Stage& S, hipStream_t s )
    const char* env = std::getenv( "TNLMI_COLUMN_CODES" );
    if( env && std::string( env ) == "0" )
       return;
-   const tnlmi_segments& seg = *S.A->segments;
-   const int64_t storage = seg.storage;
+   const int64_t storage = S.storage;
    DeviceArray< unsigned long long > table;
    DeviceArray< int32_t > flags;  // count, overflow
    table.allocate( kHash );
@@ -739,7 +745,7 @@ void make_codes( Stage& S, hipStream_t s )
    TNLMI_HIP( hipMemsetAsync( table.ptr, 0xff, sizeof( unsigned long long ) * kHash, s ) );
    TNLMI_HIP( hipMemsetAsync( flags.ptr, 0, sizeof( int32_t ) * 2, s ) );
    const unsigned grid = (unsigned) std::min< int64_t >( 4096, std::max< int64_t >( 1, ceil_div( S.v.n, kFT ) ) );
-   hipLaunchKernelGGL( code_collect, dim3( grid ), dim3( kFT ), 0, s, S.v, S.A->column_indexes, table.ptr, flags.ptr,
+   hipLaunchKernelGGL( code_collect, dim3( grid ), dim3( kFT ), 0, s, S.v, S.cols, table.ptr, flags.ptr,
                        flags.ptr + 1 );
    TNLMI_HIP( hipGetLastError() );
    std::vector< unsigned long long > ht( kHash );
@@ -771,11 +777,101 @@ void make_codes( Stage& S, hipStream_t s )
       TNLMI_HIP( hipMemcpyAsync( S.dict->ptr, keys.data(), sizeof( int32_t ) * keys.size(), hipMemcpyHostToDevice,
                                  s ) );
    TNLMI_HIP( hipMemsetAsync( S.codes->ptr, kPadCode, storage + 16, s ) );
-   hipLaunchKernelGGL( code_encode, dim3( grid ), dim3( kFT ), 0, s, S.v, S.A->column_indexes, table.ptr, dcode_of.ptr,
+   hipLaunchKernelGGL( code_encode, dim3( grid ), dim3( kFT ), 0, s, S.v, S.cols, table.ptr, dcode_of.ptr,
                        S.codes->ptr );
    TNLMI_HIP( hipGetLastError() );
    TNLMI_HIP( hipStreamSynchronize( s ) );
    S.ndict = (int32_t) keys.size();
+}
+
+// kind 1 launch shape over S.v (column-major Ellpack / SlicedEllpack)
+void ell_shape( Stage& S, hipStream_t s, bool codes )
+{
+   S.kind = 1;
+   // 16-byte loads of the column indexes for 4-byte values, of the values for FP64
+   S.rows_per_thread = S.A->value_type == TNLMI_F64 ? 2 : 4;
+   if( S.v.format == TNLMI_SLICED_ELLPACK && S.v.slice_size % S.rows_per_thread != 0 )
+      S.rows_per_thread = 1;
+   S.grid = ceil_div( ceil_div( S.v.n, S.rows_per_thread ), kFT );
+   S.part = std::make_shared< DeviceArray< double > >();
+   S.part->allocate( S.grid );
+   if( codes )
+      make_codes( S, s );
+}
+
+// CSR rows as column-major Ellpack slots: slot j of row r at j * stride + r
+// (padding: column -1, value 0); rows [n, stride) are all padding
+template< typename V >
+__global__ void csr_to_ell( int64_t n, int64_t stride, int32_t w, const int32_t* __restrict__ off,
+                            const int32_t* __restrict__ cols, const V* __restrict__ vals, int32_t* __restrict__ ec,
+                            V* __restrict__ ev )
+{
+   const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( r >= stride )
+      return;
+   int32_t b = 0, e = 0;
+   if( r < n ) {
+      b = off[ r ];
+      e = off[ r + 1 ];
+   }
+   for( int32_t j = 0; j < w; j++ ) {
+      const bool in = b + j < e;
+      ec[ j * stride + r ] = in ? cols[ b + j ] : -1;
+      ev[ j * stride + r ] = in ? vals[ b + j ] : V{};
+   }
+}
+
+// A CSR matrix whose rows are all short (at most kWholeRowMax entries) and
+// about equally long (the Ellpack copy at most 1/8 larger than the CSR
+// arrays): the stage keeps a column-major Ellpack copy of its slots, made
+// here on every solve, and runs the fused Ellpack kernels on it.  Every row
+// keeps its entries in CSR order, so the products and sums are those of the
+// Ellpack kernels on the same matrix stored as Ellpack.  TNLMI_CG_CSR_COPY=0
+// in the environment keeps the one-launch CSR kernel instead.
+bool csr_ell_copy( Stage& S, hipStream_t s )
+{
+   const char* env = std::getenv( "TNLMI_CG_CSR_COPY" );
+   if( env && std::string( env ) == "0" )
+      return false;
+   const tnlmi_matrix& A = *S.A;
+   const tnlmi_segments& seg = *A.segments;
+   const int64_t n = S.v.n;
+   const int32_t w = std::max( 1, seg.max_size );
+   const int64_t stride = ceil_div( std::max< int64_t >( n, 1 ), 64 ) * 64;
+   if( seg.max_size > kWholeRowMax || n == 0 || stride * w > seg.elements + seg.elements / 8 )
+      return false;
+   const size_t vs = A.value_type == TNLMI_F64 ? 8 : A.value_type == TNLMI_F32 ? 4 : 2;
+   S.ecols = std::make_shared< DeviceArray< int32_t > >();
+   S.evals = std::make_shared< DeviceArray< char > >();
+   S.ecols->allocate( stride * w );
+   S.evals->allocate( stride * w * (int64_t) vs );
+   const dim3 grid( (unsigned) ceil_div( stride, 256 ) ), block( 256 );
+   switch( A.value_type ) {
+      case TNLMI_F64:
+         hipLaunchKernelGGL( csr_to_ell< double >, grid, block, 0, s, n, stride, w, seg.offsets.ptr, A.column_indexes,
+                             (const double*) A.values, S.ecols->ptr, (double*) S.evals->ptr );
+         break;
+      case TNLMI_F32:
+         hipLaunchKernelGGL( csr_to_ell< float >, grid, block, 0, s, n, stride, w, seg.offsets.ptr, A.column_indexes,
+                             (const float*) A.values, S.ecols->ptr, (float*) S.evals->ptr );
+         break;
+      default:
+         hipLaunchKernelGGL( csr_to_ell< bf16 >, grid, block, 0, s, n, stride, w, seg.offsets.ptr, A.column_indexes,
+                             (const bf16*) A.values, S.ecols->ptr, (bf16*) S.evals->ptr );
+         break;
+   }
+   TNLMI_HIP( hipGetLastError() );
+   SegView v{};
+   v.format = TNLMI_ELLPACK;
+   v.col_major = 1;
+   v.n = n;
+   v.ell_width = w;
+   v.ell_aligned = (int32_t) stride;
+   S.v = v;
+   S.cols = S.ecols->ptr;
+   S.vals = S.evals->ptr;
+   S.storage = stride * w;
+   return true;
 }
 
 Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_t s, bool codes = false )
@@ -784,24 +880,21 @@ Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_
    S.A = A;
    S.plan = plan;
    S.v = make_view( *A->segments );
+   S.cols = A->column_indexes;
+   S.vals = A->values;
+   S.storage = A->segments->storage;
    if( plan )
       return S;
    const int fmt = S.v.format;
    if( ( fmt == TNLMI_ELLPACK || fmt == TNLMI_SLICED_ELLPACK ) && S.v.col_major && A->matrix_type != TNLMI_SYMMETRIC ) {
-      S.kind = 1;
-      // 16-byte loads of the column indexes for 4-byte values, of the values for FP64
-      S.rows_per_thread = A->value_type == TNLMI_F64 ? 2 : 4;
-      if( fmt == TNLMI_SLICED_ELLPACK && S.v.slice_size % S.rows_per_thread != 0 )
-         S.rows_per_thread = 1;
-      S.grid = ceil_div( ceil_div( S.v.n, S.rows_per_thread ), kFT );
-      S.part = std::make_shared< DeviceArray< double > >();
-      S.part->allocate( S.grid );
-      if( codes ) {
-         make_codes( S, s );
-      }
+      ell_shape( S, s, codes );
       return S;
    }
    const tnlmi_segments& seg = *A->segments;
+   if( fmt == TNLMI_CSR && A->matrix_type != TNLMI_SYMMETRIC && csr_ell_copy( S, s ) ) {
+      ell_shape( S, s, codes );
+      return S;
+   }
    if( fmt == TNLMI_CSR && A->matrix_type != TNLMI_SYMMETRIC && seg.max_size <= kWholeRowMax && seg.tiles > 0
        && ( seg.tile_nnz == kTileNnz || seg.tile_nnz == kTileNnzSmall ) ) {
       S.kind = 2;
@@ -815,7 +908,7 @@ Stage make_stage( const tnlmi_matrix* A, const tnlmi_spmv_plan* plan, hipStream_
 template< typename V, typename T >
 void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDev* st, hipStream_t s )
 {
-   const V* vals = (const V*) S.A->values;
+   const V* vals = (const V*) S.vals;
    double* bp = S.part->ptr;
    if( S.kind == 1 && S.codes ) {
       const uint8_t* c = S.codes->ptr;
@@ -845,13 +938,13 @@ void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDe
    else if( S.kind == 1 ) {
       if( S.rows_per_thread == 4 )
          hipLaunchKernelGGL( ( spmv_dot_ell< 4, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
-                             S.A->column_indexes, vals, p, Ap, bp, st );
+                             S.cols, vals, p, Ap, bp, st );
       else if( S.rows_per_thread == 2 )
          hipLaunchKernelGGL( ( spmv_dot_ell< 2, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
-                             S.A->column_indexes, vals, p, Ap, bp, st );
+                             S.cols, vals, p, Ap, bp, st );
       else
          hipLaunchKernelGGL( ( spmv_dot_ell< 1, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
-                             S.A->column_indexes, vals, p, Ap, bp, st );
+                             S.cols, vals, p, Ap, bp, st );
    }
    hipLaunchKernelGGL( fold_parts, dim3( kFB ), dim3( kFT ), 0, s, bp, S.grid, part, st );
 }

@@ -131,14 +131,15 @@ def train(net, sizes=(256,), instances=2000, steps_per_instance=1, probes=4, lr=
 
 
 def gmres_solve(A, b, x, preconditioner, tolerance=1e-10, restart=30, max_iterations=100_000, variant="MGSR",
-                stall_cycles=20, true_residual_check=True):
+                stall_cycles=20, true_residual_check=True, log=None):
     """Left-preconditioned GMRES(restart) until the FP64 true residual
     ||b - A x|| / ||b|| <= tolerance (LinearResidueGetter.hpp:11-23).  GMRES
     stops on its preconditioned residue, so the solve is resumed from x with
     a proportionally tighter residue target while the true residual is above
     the tolerance.  Stops early (returns False) when the true residual falls
     by less than 1 % over stall_cycles restart cycles (or over one
-    resumed solve).
+    resumed solve).  log, when given, is called with a progress line after
+    every stall_cycles restart cycles.
     Returns (converged, iterations, true residual)."""
     s = solvers.GMRES()
     s.setMatrix(A)
@@ -154,6 +155,8 @@ def gmres_solve(A, b, x, preconditioner, tolerance=1e-10, restart=30, max_iterat
         ok = s.solve(b, x)
         its += s.getIterations()
         r = solvers.true_residual(A, x, b)
+        if log is not None:
+            log(f"gmres: {its} iterations, true residual {r:.3e}")
         if r <= tolerance:
             return True, its, r
         if not true_residual_check or not (r < 0.99 * prev):  # stalled (or NaN)
