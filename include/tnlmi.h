@@ -326,7 +326,7 @@ tnlmi_status tnlmi_spmv_transposed( const tnlmi_matrix* A, tnlmi_dtype x_type, c
  * window's products are formed with x staged in LDS and summed per row block
  * in LDS; every row of at most 256 nonzeros is summed left to right in
  * column order from zero, bit for bit as the host CSR vectorProduct, longer
- * rows in a fixed order (rows longer than a row block's 8160 FP64 / 16320
+ * rows in a fixed order (rows longer than a row block's 10000 FP64 / 20000
  * FP32 nonzeros as pieces whose partial sums are added in order).  FP64 or
  * FP32 values, vectors and compute type alike.  Built once (synchronous); the plan
  * owns its copy (the matrix may be freed), so value changes of the matrix

@@ -402,11 +402,11 @@ def test_windowed_plan_fp32_and_errors(orc):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
 def test_windowed_plan_rows_longer_than_a_block(orc, dtype):
-    # a row block holds 8160 FP64 / 16320 FP32 nonzeros: longer rows are cut
+    # a row block holds 10000 FP64 / 20000 FP32 nonzeros: longer rows are cut
     # into pieces whose partial sums are added in order; empty rows around them
     rng = np.random.default_rng(7)
     m = 70000
-    longs = [(0, 9000), (5, 8160), (6, 8161), (9, 40000), (40, 16321), (77, 20000)]
+    longs = [(0, 12000), (5, 10000), (6, 10001), (9, 45000), (40, 20001), (77, 25000)]
     off, cols, vals = random_csr(80, m, 12, rng, long_rows=longs)
     sizes = np.diff(off).copy()
     sizes[[1, 2, 78, 79]] = 0  # empty rows after the first long row and after the last one

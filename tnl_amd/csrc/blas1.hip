@@ -59,16 +59,18 @@ struct SqF
 };
 
 constexpr int kMaxCols = 32;
+constexpr int kDotGroup = 8;  // columns of V per workgroup of the multi-dot's first pass
 
-// A wave's 64 lanes hold kMaxCols partial sums each; the wave totals of all
-// columns come out of one butterfly that halves the columns a lane carries at
-// every step (16 + 8 + 4 + 2 + 1 exchanges, then 1 for the last pair) instead
-// of a full 6-step reduction per column: lane l ends with column l >> 1.
-__device__ __forceinline__ double wave_sum_columns( double ( &acc )[ kMaxCols ] )
+// A wave's 64 lanes hold the partial sums of kDotGroup columns each; three
+// halving steps (4 + 2 + 1 exchanges) leave lane l with column l >> 3 summed
+// over the 8 lanes that share bits 5..3 of l, and three more exchanges sum
+// those 8 lanes.
+__device__ __forceinline__ double wave_sum_group( double ( &acc )[ kDotGroup ] )
 {
+   static_assert( kDotGroup == 8, "three halving steps" );
    const int lane = threadIdx.x & 63;
 #pragma unroll
-   for( int o = 32, m = kMaxCols / 2; o >= 2; o /= 2, m /= 2 ) {
+   for( int o = 32, m = kDotGroup / 2; m >= 1; o /= 2, m /= 2 ) {
       const bool upper = lane & o;
 #pragma unroll
       for( int j = 0; j < m; j++ ) {
@@ -78,54 +80,65 @@ __device__ __forceinline__ double wave_sum_columns( double ( &acc )[ kMaxCols ] 
          acc[ j ] = keep + __shfl_xor( send, o, 64 );
       }
    }
-   return acc[ 0 ] + __shfl_xor( acc[ 0 ], 1, 64 );
+   double t = acc[ 0 ];
+   t += __shfl_xor( t, 4, 64 );
+   t += __shfl_xor( t, 2, 64 );
+   t += __shfl_xor( t, 1, 64 );
+   return t;
 }
 
+// part[k * kRedBlocks + b] = sum over the rows of workgroup column b of
+// V[i, k] w[i], for the kDotGroup columns k of workgroup row blockIdx.y:
+// eight accumulators per thread keep the loads of several rows in flight at
+// full occupancy (w is read once per group of eight columns)
 __global__ void __launch_bounds__( 256 ) multi_dot_pass1( int64_t n, int cols, const double* __restrict__ V,
                                                           int64_t ld, const double* __restrict__ w,
                                                           double* __restrict__ part )
 {
-   static_assert( kMaxCols == 32, "the butterfly of wave_sum_columns ends with one column per lane pair" );
-   __shared__ double sm[ 4 ][ kMaxCols ];
-   double acc[ kMaxCols ];
+   __shared__ double sm[ 4 ][ kDotGroup ];
+   const int c0 = blockIdx.y * kDotGroup;
+   const int nc = min( kDotGroup, cols - c0 );
+   const double* __restrict__ Vc = V + (int64_t) c0 * ld;
+   double acc[ kDotGroup ];
 #pragma unroll
-   for( int k = 0; k < kMaxCols; k++ )
+   for( int k = 0; k < kDotGroup; k++ )
       acc[ k ] = 0;
    const int64_t stride = (int64_t) gridDim.x * 256;
    for( int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += stride ) {
       const double wi = w[ i ];
 #pragma unroll
-      for( int k = 0; k < kMaxCols; k++ )
-         if( k < cols )
-            acc[ k ] += V[ i + k * ld ] * wi;
+      for( int k = 0; k < kDotGroup; k++ )
+         if( k < nc )
+            acc[ k ] += Vc[ i + k * ld ] * wi;
    }
    // every column's wave total at once, then one barrier and a fixed-order
    // sum of the four waves' totals
-   const double t = wave_sum_columns( acc );
+   const double t = wave_sum_group( acc );
    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-   if( ( lane & 1 ) == 0 )
-      sm[ wv ][ lane >> 1 ] = t;
+   if( ( lane & 7 ) == 0 )
+      sm[ wv ][ lane >> 3 ] = t;
    __syncthreads();
-   if( (int) threadIdx.x < cols ) {
+   if( (int) threadIdx.x < nc ) {
       const int k = threadIdx.x;
-      part[ (int64_t) k * kRedBlocks + blockIdx.x ] = ( ( sm[ 0 ][ k ] + sm[ 1 ][ k ] ) + sm[ 2 ][ k ] ) + sm[ 3 ][ k ];
+      part[ (int64_t) ( c0 + k ) * kRedBlocks + blockIdx.x ] =
+         ( ( sm[ 0 ][ k ] + sm[ 1 ][ k ] ) + sm[ 2 ][ k ] ) + sm[ 3 ][ k ];
    }
 }
 
-// out[k] = sum of the np partials of column k: wave k % 4 of the block sums
-// columns k = wave, wave + 4, ... (lanes stride the partials, then a wave
-// butterfly), so the columns proceed in parallel with no barrier
+// out[k] = sum of the np partials of column k: one wavefront per column
+// (lanes stride the partials, then a wave butterfly), four per workgroup
 __global__ void __launch_bounds__( 256 ) multi_dot_pass2( const double* part, int np, int cols, double* out )
 {
-   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-   for( int k = wv; k < cols; k += 4 ) {
-      double a = 0;
-      for( int i = lane; i < np; i += 64 )
-         a += part[ (int64_t) k * kRedBlocks + i ];
-      a = group_sum< 64 >( a );
-      if( lane == 0 )
-         out[ k ] = a;
-   }
+   const int lane = threadIdx.x & 63;
+   const int k = blockIdx.x * 4 + ( threadIdx.x >> 6 );
+   if( k >= cols )
+      return;  // the whole wavefront
+   double a = 0;
+   for( int i = lane; i < np; i += 64 )
+      a += part[ (int64_t) k * kRedBlocks + i ];
+   a = group_sum< 64 >( a );
+   if( lane == 0 )
+      out[ k ] = a;
 }
 
 __global__ void axpby_k( int64_t n, double alpha, const double* x, double beta, double* y )
@@ -206,10 +219,14 @@ void multi_dot( int64_t n, int cols, const double* V, int64_t ld, const double* 
 {
    if( cols > kMaxCols )
       fail( TNLMI_NOT_IMPLEMENTED, "multi_dot: at most 32 columns" );
+   if( cols <= 0 )
+      return;
    rw.ensure( kMaxCols );
    const int g = red_grid( n );
-   hipLaunchKernelGGL( multi_dot_pass1, dim3( g ), dim3( 256 ), 0, s, n, cols, V, ld, w, rw.slot( 0 ) );
-   hipLaunchKernelGGL( multi_dot_pass2, dim3( 1 ), dim3( 256 ), 0, s, rw.slot( 0 ), g, cols, out );
+   hipLaunchKernelGGL( multi_dot_pass1, dim3( g, (unsigned) ceil_div( cols, kDotGroup ) ), dim3( 256 ), 0, s, n,
+                       cols, V, ld, w, rw.slot( 0 ) );
+   hipLaunchKernelGGL( multi_dot_pass2, dim3( (unsigned) ceil_div( cols, 4 ) ), dim3( 256 ), 0, s, rw.slot( 0 ), g,
+                       cols, out );
 }
 
 void axpby( int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s )

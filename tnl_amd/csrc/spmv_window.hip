@@ -5,8 +5,8 @@
 // the L2 each one costs a 128-byte line and an L2 request.  The plan splits
 // the columns into windows of `win` entries (8192 by default: 64 KiB of FP64
 // x, one LDS allocation; 10240, 16384 and 20480 are also built, up to the
-// 160 KiB of LDS a gfx950 workgroup can hold) and the nonzeros into row blocks of at most `cap` (FP64 8160,
-// FP32 16320): runs of whole consecutive rows, or one piece of a row longer
+// 160 KiB of LDS a gfx950 workgroup can hold) and the nonzeros into row blocks of at most `cap` (FP64 10000,
+// FP32 20000): runs of whole consecutive rows, or one piece of a row longer
 // than cap (such a row is cut into pieces of cap nonzeros):
 //
 //   phase 1  one workgroup per window: x[window] -> LDS, then stream the
@@ -47,11 +47,14 @@ constexpr int kP2 = 512;             // phase-2 threads
 constexpr int kLongRow = 256;        // longer rows are summed by a whole wave
 constexpr int kPad = 32;             // padding entries of prod and dest (phase-2 reads past a segment)
 
-// row-block capacity: the phase-2 LDS buffer within 64 KiB
+// row-block capacity: the phase-2 LDS buffer of 80,000 bytes, so two phase-2
+// workgroups share a CU's 160 KiB (measured against 8160 FP64 nonzeros in
+// 64 KiB: 467-480 against 481-496 us per power-law SpMV, the chunks of a
+// window's products in a block being 1.2 times longer)
 template< typename C >
 constexpr int cap_of()
 {
-   return sizeof( C ) == 8 ? 8160 : 16320;
+   return sizeof( C ) == 8 ? 10000 : 20000;
 }
 
 // window of each nonzero (padding slots, column -1, go to an extra window K)
@@ -235,7 +238,8 @@ template< typename C, int CAP, bool BETA, int SEGS, int PER, bool ALIGN >
 __global__ void __launch_bounds__( kP2 )
    w_phase2( int64_t B, int32_t K, const BlockHdr* __restrict__ hdr, const int32_t* __restrict__ seg,
              const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ off,
-             const int32_t* __restrict__ lrows, C* __restrict__ part, C* __restrict__ y, C alpha, C beta,
+             const int32_t* __restrict__ lrows, const uint16_t* __restrict__ rorder, C* __restrict__ part,
+             C* __restrict__ y, C alpha, C beta,
              int64_t begin, int64_t end )
 {
    __shared__ C buf[ CAP ];
@@ -263,16 +267,19 @@ __global__ void __launch_bounds__( kP2 )
    const int32_t kt = sx < K ? sx : K - 1;
    const int32_t a = seg[ b * K + kt ];
    const int32_t z = sx < K ? seg[ ( b + 1 ) * K + kt ] : a;
-   // the rows of this thread (a block has at most 2 P2) and of its wave's
-   // long row (at most one per wave); unconditional loads at clamped indexes
+   // the rows of this thread: ranks t and t + P2 of the block's rows ordered
+   // by length, longest first (a block has at most 2 P2 rows), so the rows
+   // of a wave, each summed serially by its lane, are about equally long;
+   // and its wave's long row (at most one per wave).  Unconditional loads at
+   // clamped indexes.
    constexpr int J = 2;
-   int32_t ob[ J ], oe[ J ];
-   const int32_t last = std::max( H.r0, H.r1 - 1 );
+   const int32_t nrows = H.r1 - H.r0;
+   int32_t rr[ J ], ob[ J ], oe[ J ];
 #pragma unroll
    for( int j = 0; j < J; j++ ) {
-      const int32_t r = std::min( H.r0 + t + j * P2, last );
-      ob[ j ] = off[ r ];
-      oe[ j ] = off[ r + 1 ];
+      rr[ j ] = H.r0 + rorder[ H.r0 + std::min( t + j * P2, std::max( nrows - 1, 0 ) ) ];
+      ob[ j ] = off[ rr[ j ] ];
+      oe[ j ] = off[ rr[ j ] + 1 ];
    }
    const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
    const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
@@ -343,11 +350,9 @@ __global__ void __launch_bounds__( kP2 )
    // rows of at most kLongRow nonzeros: one thread each, summed from zero
    // left to right
 #pragma unroll
-   for( int j = 0; j < J; j++ ) {
-      const int32_t r = H.r0 + t + j * P2;
-      if( r < H.r1 && oe[ j ] - ob[ j ] <= kLongRow )
-         keep( r, sum_row( buf, ob[ j ] - e0, oe[ j ] - e0 ) );
-   }
+   for( int j = 0; j < J; j++ )
+      if( t + j * P2 < nrows && oe[ j ] - ob[ j ] <= kLongRow )
+         keep( rr[ j ], sum_row( buf, ob[ j ] - e0, oe[ j ] - e0 ) );
    // the long row of this wave (if any): lane l sums terms l, l + 64, ... and
    // an xor butterfly adds the 64 parts
    if( H.l0 + wid < H.l1 ) {
@@ -452,6 +457,25 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    if( lrows.empty() )
       lrows.assign( 3, 0 );  // phase 2 reads one entry at a clamped index
    upload( W.lrows, lrows );
+   // each run's rows by length, longest first (ties: row order); a piece
+   // block's row keeps index 0
+   {
+      std::vector< uint16_t > order( std::max< int64_t >( 1, n ), 0 );
+      std::vector< uint16_t > idx;
+      for( int64_t i = 0; i < W.B; i++ ) {
+         if( piece[ i ] >= 0 )
+            continue;
+         const int32_t r0 = rb[ i ], cnt = rb[ i + 1 ] - r0;
+         idx.resize( cnt );
+         for( int32_t q = 0; q < cnt; q++ )
+            idx[ q ] = (uint16_t) q;
+         std::stable_sort( idx.begin(), idx.end(), [ & ]( uint16_t a, uint16_t b ) {
+            return hoff[ r0 + a + 1 ] - hoff[ r0 + a ] > hoff[ r0 + b + 1 ] - hoff[ r0 + b ];
+         } );
+         std::copy( idx.begin(), idx.end(), order.begin() + r0 );
+      }
+      upload( W.rorder, order );
+   }
    W.part.allocate( std::max< int64_t >( 1, W.B ) * (int64_t) sizeof( V ) );
    // sort the nonzeros by window (stable: CSR order inside a window)
    DeviceArray< int32_t > key, skey, ids, perm;
@@ -505,13 +529,13 @@ void phase2( const WindowPlan& W, C* y, double alpha, double beta, int64_t begin
       hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, true >
                                         : w_phase2< C, cap_of< C >(), true, SEGS, PER, true > ),
                           dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
-                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y, (C) alpha,
+                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
                           (C) beta, begin, end );
    else
       hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, false >
                                         : w_phase2< C, cap_of< C >(), true, SEGS, PER, false > ),
                           dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
-                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, (C*) W.part.ptr, y, (C) alpha,
+                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
                           (C) beta, begin, end );
 }
 

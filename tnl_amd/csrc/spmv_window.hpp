@@ -21,6 +21,7 @@ struct WindowPlan
    DeviceArray< int64_t > wstart;  // K + 2: first sorted nonzero of each window (window K: padding slots)
    DeviceArray< int32_t > seg;     // (B + 1) x K: first nonzero of row block b in window k
    DeviceArray< int32_t > offs;    // rows + 1: the CSR offsets
+   DeviceArray< uint16_t > rorder; // rows: each run's rows by length, longest first (index inside the run)
    DeviceArray< int32_t > hdr;     // 8 B: rows, nonzeros, piece and long-row range of each row block
    DeviceArray< int32_t > lrows;   // 3 per row longer than the one-thread limit: row, first and end slot
    DeviceArray< int32_t > huge;    // 3 H: (row, first piece block, pieces) of each row longer than cap
