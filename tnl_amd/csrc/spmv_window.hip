@@ -265,24 +265,20 @@ __global__ void __launch_bounds__( kP2 )
    // (i < R) of the group's lanes
    const int32_t sx = wid * ( 64 / kGrp ) * R + ( lane / kGrp ) * R + std::min( gl, R - 1 );
    const int32_t kt = sx < K ? sx : K - 1;
+   // both bounds loaded unconditionally (a load under the condition would
+   // wait for the first before it is issued)
    const int32_t a = seg[ b * K + kt ];
-   const int32_t z = sx < K ? seg[ ( b + 1 ) * K + kt ] : a;
+   const int32_t zk = seg[ ( b + 1 ) * K + kt ];
+   const int32_t z = sx < K ? zk : a;
    // the rows of this thread: ranks t and t + P2 of the block's rows ordered
    // by length, longest first (a block has at most 2 P2 rows), so the rows
    // of a wave, each summed serially by its lane, are about equally long;
    // and its wave's long row (at most one per wave).  Unconditional loads at
-   // clamped indexes.
+   // clamped indexes, requested after the products (no product request
+   // waits on them; the offsets arrive while the products are scattered).
    constexpr int J = 2;
    const int32_t nrows = H.r1 - H.r0;
    int32_t rr[ J ], ob[ J ], oe[ J ];
-#pragma unroll
-   for( int j = 0; j < J; j++ ) {
-      rr[ j ] = H.r0 + rorder[ H.r0 + std::min( t + j * P2, std::max( nrows - 1, 0 ) ) ];
-      ob[ j ] = off[ rr[ j ] ];
-      oe[ j ] = off[ rr[ j ] + 1 ];
-   }
-   const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
-   const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
    // round i: the group's segment i, kSpan terms.  ALIGN: a group's reads
    // start at the segment's first term rounded down to a multiple of PER, so
    // every lane's PER products (8 PER bytes) and slots (2 PER bytes) are
@@ -298,6 +294,16 @@ __global__ void __launch_bounds__( kP2 )
       get_pair< C, PER >( prod, dest, std::min( ( ai & kAl ) + PER * gl, std::max( zi - 1, ai ) & kAl ), v[ i ],
                           d[ i ] );
    }
+#pragma unroll
+   for( int j = 0; j < J; j++ )
+      rr[ j ] = H.r0 + rorder[ H.r0 + std::min( t + j * P2, std::max( nrows - 1, 0 ) ) ];
+#pragma unroll
+   for( int j = 0; j < J; j++ ) {
+      ob[ j ] = off[ rr[ j ] ];
+      oe[ j ] = off[ rr[ j ] + 1 ];
+   }
+   const int32_t li = std::min( H.l0 + wid, std::max( H.l1 - 1, 0 ) );
+   const int32_t lr = lrows[ 3 * li ], lq0 = lrows[ 3 * li + 1 ], lq1 = lrows[ 3 * li + 2 ];
    // clear (padding slots, column -1, have no product: they read as zero)
    const int32_t len = H.e1 - H.e0;
    for( int32_t q = t; q < len; q += P2 )
