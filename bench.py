@@ -661,6 +661,7 @@ def wl_gmres_convdiff(args):
     from tnl_amd import spai
 
     tol = 1e-10
+    todo = set(args.solvers.split(","))
     # training: seeded side x side instances, one Adam step each, on the GPU
     net = spai.Network(seed=args.spai_seed)
     t0 = time.perf_counter()
@@ -700,12 +701,15 @@ def wl_gmres_convdiff(args):
     w.setMatrix(A)
     w.setMaxIterations(5)
     w.solve(b, torch.zeros(n, dtype=torch.float64, device="cuda"))
-    run("jacobi", lambda: (solvers.Diagonal(A), {}), args.max_iterations)
+    if "jacobi" in todo:
+        run("jacobi", lambda: (solvers.Diagonal(A), {}), args.max_iterations)
 
     def learned():
         return solvers.Explicit(net.inverse(A)), {}
-    run("learned_spai", learned, args.max_iterations)
-    run("ilu0", lambda: (solvers.ILU0(A), {}), args.ilu_max_iterations, progress=lambda m: log(f"[ilu0] {m}"))
+    if "learned_spai" in todo:
+        run("learned_spai", learned, args.max_iterations)
+    if "ilu0" in todo:
+        run("ilu0", lambda: (solvers.ILU0(A), {}), args.ilu_max_iterations, progress=lambda m: log(f"[ilu0] {m}"))
 
     # the nonsymmetric alternative on the same system: BiCGStab (BICGStab.hpp),
     # restarted from its own x with a tighter target until the true residual
@@ -742,14 +746,17 @@ def wl_gmres_convdiff(args):
         log(f"{label}: {rec}")
         res[label] = rec
 
-    run_bicgstab("bicgstab_jacobi", lambda: (solvers.Diagonal(A), {}))
-    run_bicgstab("bicgstab_learned_spai", learned)
+    if "bicgstab" in todo:
+        run_bicgstab("bicgstab_jacobi", lambda: (solvers.Diagonal(A), {}))
+        run_bicgstab("bicgstab_learned_spai", learned)
     # probe loss of the two inverses on the solve matrix
     Z = torch.from_numpy(np.random.default_rng(5).integers(0, 2, (4, n)) * 2.0 - 1.0).cuda()
     L_learned, _ = net.loss_grad(A, Z)
     L_jacobi, _ = spai.Network(seed=args.spai_seed).loss_grad(A, Z)
-    t_j, t_l = res["jacobi"]["seconds"], res["learned_spai"]["seconds"]
-    ok = res["jacobi"]["converged"] and res["learned_spai"]["converged"]
+    both = "jacobi" in res and "learned_spai" in res
+    t_j = res["jacobi"]["seconds"] if both else 0.0
+    t_l = res["learned_spai"]["seconds"] if both else 0.0
+    ok = both and res["jacobi"]["converged"] and res["learned_spai"]["converged"]
     speedup = t_j / t_l if ok else 0.0
     nb = spmv_bytes(n, nnz)
     out = base_record(args, round(speedup, 4), "x (Jacobi-GMRES(30) time / learned-SPAI-GMRES(30) time, to 1e-10)",
@@ -759,16 +766,16 @@ def wl_gmres_convdiff(args):
                        "rows": n, "nnz": nnz, "peclet": pe, "results": res,
                        "training": {"instances": args.spai_instances, "side": args.spai_side,
                                     "probes": args.spai_probes, "lr": args.spai_lr, "seconds": round(t_train, 2),
-                                    "loss_first_50": float(np.mean(hist[:50])),
-                                    "loss_last_50": float(np.mean(hist[-50:]))},
+                                    "loss_first_50": float(np.mean(hist[:50])) if len(hist) else None,
+                                    "loss_last_50": float(np.mean(hist[-50:])) if len(hist) else None},
                        "probe_loss_on_solve_matrix": {"jacobi": L_jacobi, "learned": L_learned},
                        "gmres_variant": args.gmres_variant,
                        "note": "left-preconditioned GMRES(30); the solve time of the learned preconditioner includes "
                                "the network's inference on the solve matrix; ILU(0) capped at --ilu-max-iterations; "
                                "bicgstab_* are BiCGStab on the same system to the same true residual (beside the "
                                "GMRES comparison the value is quoted on)"})
-    its_l = res["learned_spai"]["iterations"]
-    out["roofline"] = roofline(2 * nb * its_l / (t_l - res["learned_spai"]["setup_seconds"]) / 1e9)
+    its_l = res["learned_spai"]["iterations"] if both else 0
+    out["roofline"] = roofline(2 * nb * its_l / (t_l - res["learned_spai"]["setup_seconds"]) / 1e9 if both else 0.0)
     out["roofline"]["kernel"] = "SpMV with A and with M per GMRES iteration (orthogonalisation not counted)"
     out["cpu_baseline"] = None
     return out
@@ -855,6 +862,8 @@ def main():
     p.add_argument("--ilu-max-iterations", type=int, default=20_000, help="gmres-convdiff: cap for ILU(0)")
     p.add_argument("--solve-seed", type=int, default=7)
     p.add_argument("--gmres-variant", default="CGSR", choices=["CGS", "CGSR", "MGS", "MGSR"])
+    p.add_argument("--solvers", default="jacobi,learned_spai,ilu0,bicgstab",
+                   help="gmres-convdiff: the solves to run (value needs jacobi and learned_spai)")
     p.add_argument("--spai-instances", type=int, default=2000)
     p.add_argument("--spai-side", type=int, default=256)
     p.add_argument("--spai-probes", type=int, default=4)

@@ -242,8 +242,10 @@ def _block_csr(blocks):
 
 def test_ilu0_apply_narrow_and_wide_levels(orc):
     """The triangular solves run consecutive levels of at most 2,048 rows in one
-    one-workgroup launch and wider levels one launch each; both, in any order,
-    match the oracle's ILU(0) solve."""
+    one-workgroup launch (at most 16,384 levels per launch; levels of at most
+    1,024 rows with at most 3 entries per row through the LDS ring) and wider
+    levels one launch each; all of them, in any order, match the oracle's ILU(0)
+    solve."""
     rng = np.random.default_rng(31)
     # 5,000 independent rows (one wide level 0), then a 60 x 60 Laplacian (narrow levels)
     d = 5000
@@ -261,8 +263,16 @@ def test_ilu0_apply_narrow_and_wide_levels(orc):
     roff[1:] = np.cumsum([len(c) for c in rows])
     rcols = np.concatenate(rows)
     rvals = np.where(rcols == np.repeat(np.arange(m), np.diff(roff)), 20.0, rng.standard_normal(len(rcols)) * 0.5)
+    # a tridiagonal matrix of 40,000 rows: 40,000 levels of one row each, so the
+    # run is cut into three launches
+    t = 40000
+    toff = np.zeros(t + 1, np.int32)
+    toff[1:] = np.cumsum(np.minimum(np.arange(t) + 2, 3) - (np.arange(t) == t - 1))
+    tcols = np.concatenate([np.arange(max(r - 1, 0), min(r + 2, t), dtype=np.int32) for r in range(t)])
+    tvals = np.where(tcols == np.repeat(np.arange(t), np.diff(toff)), 2.5, -1.0 + 0.1 * rng.random(len(tcols)))
+    tri = (toff, tcols, tvals)
     for blocks in ([diag, lap], [lap, diag], [(roff, rcols, rvals)], [diag, (roff, rcols, rvals), lap],
-                   [tnl.laplace2d(300, 300)]):
+                   [tnl.laplace2d(300, 300)], [tri], [diag, tri]):
         off, cols, vals = _block_csr(blocks)
         n = len(off) - 1
         b = rng.standard_normal(n)

@@ -39,7 +39,7 @@ struct tnlmi_preconditioner
    {
       int64_t l0, l1;
       bool narrow;
-      bool ring;  // every dependency inside the run lies within kRing positions of its level's end
+      bool ring;  // every dependency of the run lies within kRing positions of its level's end
    };
    std::vector< Run > lruns, uruns;
    // the factors in level-order positions (for the LDS-ring solve): per position p
@@ -47,6 +47,11 @@ struct tnlmi_preconditioner
    // and value; the upper factor's pivot per position
    DeviceArray< int32_t > lpp, ldep, upp, udep;
    DeviceArray< double > lval, uval, udiag;
+   // the same per position in fixed-size records for the LDS-ring solve (row
+   // and its dependency positions, -1: none; their values and the pivot), and
+   // the right-hand side of a solve gathered into position order before it
+   DeviceArray< int32_t > lridx, uridx;
+   DeviceArray< double > lrval, urval, bpos;
    tnlmi_matrix M{};                   // explicit: x = M b
 };
 
@@ -210,100 +215,126 @@ __device__ __forceinline__ void lv_load( LvRow ( &R )[ kLvRows ], const int32_t*
    }
 }
 
-// The LDS-ring form of tri_levels: the solution entries of the run's latest
-// kRing positions (in level order) stay in LDS, so a row reads the entries
-// it depends on from LDS (those before the run from global memory, written by
-// earlier launches) and the levels synchronise through LDS alone -- the
-// barrier waits for no global store.  Used for a run in which every
-// dependency inside the run lies within kRing positions of its level's end
-// (a stencil's: within two levels), checked when the factors are built.
+// The LDS-ring form of tri_levels: the solution entries of the latest kRing
+// positions (in level order) stay in LDS -- those of the positions before
+// the run, solved by earlier launches, are copied in first -- so a row reads
+// the entries it depends on from LDS and the levels synchronise through LDS
+// alone: the barrier waits for no global store.  Used for a run in which
+// every dependency lies within kRing positions of its level's end (a
+// stencil's: within two levels), checked when the factors are built.
 constexpr int kRing = 8192;
 
 constexpr int kRingPre = 3;  // entries per row loaded ahead (a 7-point stencil's triangle has 3)
 
-struct RingRow
+// per-position records of the ring solve (16 and 32 bytes, naturally
+// aligned); a ring run's rows have at most kRingPre entries besides the pivot
+struct alignas( 16 ) RingIdx
 {
-   int32_t p, i, g0, g1;  // position (-1: none), row, entries [g0, g1)
-   int32_t q[ kRingPre ];  // their dependency positions
-   double v[ kRingPre ];
-   double bi, dii;
+   int32_t i, q[ kRingPre ];  // row; positions of its entries' dependencies (-1: none)
+};
+struct alignas( 16 ) RingVal
+{
+   double v[ kRingPre ], dii;  // the entries' values; the upper factor's pivot
 };
 
-template< bool UPPER >
-__device__ __forceinline__ void ring_load( RingRow& w, const int32_t* __restrict__ rows, int64_t p0, int64_t p1,
-                                           const int32_t* __restrict__ pp, const int32_t* __restrict__ dep,
-                                           const double* __restrict__ val, const double* __restrict__ diag,
-                                           const double* b, const double* x )
+struct RingRow
+{
+   int32_t p;  // position (-1: none)
+   RingIdx r;
+   RingVal v;
+   double bi;
+};
+
+// the row of this thread in the level [p0, p1), read at a clamped position:
+// every load is issued whatever the level's size and none depends on another,
+// so the wait for a stage counts exactly the loads issued after it (a
+// conditional or dependent load would make the wave wait for all of them:
+// loads complete in order)
+__device__ __forceinline__ void ring_load( RingRow& w, int64_t p0, int64_t p1, const RingIdx* __restrict__ ridx,
+                                           const RingVal* __restrict__ rval, const double* __restrict__ bpos )
 {
    const int64_t p = p0 + threadIdx.x;
-   w.p = -1;
-   if( p < p1 ) {
-      w.p = (int32_t) p;
-      w.i = rows[ p ];
-      w.g0 = pp[ p ];
-      w.g1 = pp[ p + 1 ];
-      w.bi = UPPER ? x[ w.i ] : b[ w.i ];
-      w.dii = UPPER ? diag[ p ] : 1.0;
-#pragma unroll
-      for( int q = 0; q < kRingPre; q++ )
-         if( w.g0 + q < w.g1 ) {
-            w.q[ q ] = dep[ w.g0 + q ];
-            w.v[ q ] = val[ w.g0 + q ];
-         }
-   }
+   const int64_t pc = p < p1 ? p : p1 - 1;
+   w.p = p < p1 ? (int32_t) p : -1;
+   w.r = ridx[ pc ];
+   w.v = rval[ pc ];
+   w.bi = bpos[ pc ];
 }
 
 // A level's rows are requested kRingDepth levels ahead (one row per thread:
 // ring runs have levels of at most kLvThreads rows), so their loads have
 // kRingDepth levels' time to arrive; the stages rotate through registers by
-// unrolling the level loop kRingDepth times.
-constexpr int kRingDepth = 4;
+// unrolling the level loop kRingDepth times, and a stage is reloaded after
+// its level is solved (no copy of it waits for its loads).
+constexpr int kRingDepth = 6;
+// levels of a run at most (their bounds are staged in LDS by the ring solve)
+constexpr int kRunLevels = 16384;
+
+// bpos[p] = src[rows[p]]: a solve's right-hand side in position order
+__global__ void gather_pos( int64_t n, const int32_t* __restrict__ rows, const double* __restrict__ src,
+                            double* __restrict__ bpos )
+{
+   const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( p < n )
+      bpos[ p ] = src[ rows[ p ] ];
+}
 
 template< bool UPPER >
 __global__ void __launch_bounds__( kLvThreads )
    tri_ring( const int32_t* __restrict__ rows, const int64_t* __restrict__ bounds, int64_t l0, int64_t l1,
-             const int32_t* __restrict__ pp, const int32_t* __restrict__ dep, const double* __restrict__ val,
-             const double* __restrict__ diag, const double* b, double* x )
+             const RingIdx* __restrict__ ridx, const RingVal* __restrict__ rval, const double* __restrict__ bpos,
+             double* x )
 {
    __shared__ double ring[ kRing ];
+   __shared__ int32_t lb[ kRunLevels + 1 ];  // the run's level bounds (a stage's addresses wait for no scalar load)
    const int64_t P0 = bounds[ l0 ];  // the run's first position: dependencies before it are in x
+   for( int64_t j = threadIdx.x; j <= l1 - l0; j += kLvThreads )
+      lb[ j ] = (int32_t) bounds[ l0 + j ];
+   __syncthreads();
    RingRow st[ kRingDepth ];
+   // levels past the run load the run's last level again (p = -1: nothing to solve)
    auto load = [ & ]( RingRow& R, int64_t l ) {
-      if( l < l1 )
-         ring_load< UPPER >( R, rows, bounds[ l ], bounds[ l + 1 ], pp, dep, val, diag, b, x );
-      else
+      const int32_t j = (int32_t) ( ( l < l1 ? l : l1 - 1 ) - l0 );
+      ring_load( R, lb[ j ], lb[ j + 1 ], ridx, rval, bpos );
+      if( l >= l1 )
          R.p = -1;
    };
 #pragma unroll
    for( int d = 0; d < kRingDepth; d++ )
       load( st[ d ], l0 + d );
-   // a dependency before the run: its row's entry of x, written by an earlier launch
-   auto xdep = [ & ]( int32_t q ) -> double { return q >= P0 ? ring[ q & ( kRing - 1 ) ] : x[ rows[ q ] ]; };
-   for( int64_t l = l0; l < l1; l += kRingDepth ) {  // uniform bounds: every thread meets every barrier
+   // the kRing positions before the run into the ring: their rows were
+   // solved by earlier launches, and every dependency of the run lies within
+   // kRing positions of its level's end (checked when the factors are built),
+   // so every dependency is read from LDS (a read that may go to x instead
+   // would be a flat load, whose wait is a wait for every load in flight)
+   for( int64_t q = P0 - kRing + threadIdx.x; q < P0; q += kLvThreads )
+      if( q >= 0 )
+         ring[ q & ( kRing - 1 ) ] = x[ rows[ q ] ];
+   __syncthreads();
+   auto xdep = [ & ]( int32_t q ) -> double { return ring[ q & ( kRing - 1 ) ]; };
+   // every thread runs every stage and meets every barrier (the last group of
+   // stages may run past the run with nothing to solve)
+   for( int64_t l = l0; l < l1; l += kRingDepth ) {
 #pragma unroll
       for( int d = 0; d < kRingDepth; d++ ) {
-         if( l + d < l1 ) {  // uniform
-            const RingRow w = st[ d ];
-            load( st[ d ], l + d + kRingDepth );  // the stage's next level, kRingDepth ahead
-            if( w.p >= 0 ) {
-               double xi = w.bi;
+         const RingRow& w = st[ d ];
+         if( w.p >= 0 ) {
+            double xi = w.bi;
 #pragma unroll
-               for( int q = 0; q < kRingPre; q++ )
-                  if( w.g0 + q < w.g1 )
-                     xi -= w.v[ q ] * xdep( w.q[ q ] );
-               for( int32_t g = w.g0 + kRingPre; g < w.g1; g++ )
-                  xi -= val[ g ] * xdep( dep[ g ] );
-               if( UPPER )
-                  xi = xi / w.dii;
-               ring[ w.p & ( kRing - 1 ) ] = xi;
-               x[ w.i ] = xi;
-            }
-            // LDS alone orders the levels: the global stores of x are read
-            // back by no later level of this launch
-            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
-            __builtin_amdgcn_s_barrier();
-            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
+            for( int q = 0; q < kRingPre; q++ )
+               if( w.r.q[ q ] >= 0 )
+                  xi -= w.v.v[ q ] * xdep( w.r.q[ q ] );
+            if( UPPER )
+               xi = xi / w.v.dii;
+            ring[ w.p & ( kRing - 1 ) ] = xi;
+            x[ w.r.i ] = xi;
          }
+         load( st[ d ], l + d + kRingDepth );  // the stage's next level, kRingDepth ahead
+         // LDS alone orders the levels: the global stores of x are read
+         // back by no later level of this launch
+         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
+         __builtin_amdgcn_s_barrier();
+         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
       }
    }
 }
@@ -487,7 +518,7 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
          const bool narrow = bounds[ l + 1 ] - bounds[ l ] <= (int64_t) kLvThreads * kLvRows;
          int64_t e = l + 1;
          if( narrow )
-            while( e < L && bounds[ e + 1 ] - bounds[ e ] <= (int64_t) kLvThreads * kLvRows )
+            while( e < L && e - l < kRunLevels && bounds[ e + 1 ] - bounds[ e ] <= (int64_t) kLvThreads * kLvRows )
                e++;
          runs.push_back( { l, e, narrow } );
          l = e;
@@ -500,7 +531,8 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
    auto positions = [ & ]( bool upper, const std::vector< int32_t >& order, const std::vector< int64_t >& bounds,
                            std::vector< tnlmi_preconditioner::Run >& runs, DeviceArray< int32_t >& dpp,
                            DeviceArray< int32_t >& ddep, DeviceArray< double >& dval,
-                           DeviceArray< double >* ddiag ) {
+                           DeviceArray< double >* ddiag, DeviceArray< int32_t >& dridx,
+                           DeviceArray< double >& drval ) {
       const std::vector< int32_t >& ptr = upper ? up : lp;
       const std::vector< int32_t >& cc = upper ? uc : lc;
       const std::vector< double >& vv = upper ? uv : lv;
@@ -529,23 +561,41 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
             continue;
          for( int64_t l = R.l0; l < R.l1 && R.ring; l++ )
             R.ring = bounds[ l + 1 ] - bounds[ l ] <= kLvThreads;  // one row per thread
-         const int64_t P0 = bounds[ R.l0 ];
          for( int64_t l = R.l0; l < R.l1 && R.ring; l++ )
-            for( int64_t p = bounds[ l ]; p < bounds[ l + 1 ] && R.ring; p++ )
+            for( int64_t p = bounds[ l ]; p < bounds[ l + 1 ] && R.ring; p++ ) {
+               if( pp[ p + 1 ] - pp[ p ] > kRingPre )
+                  R.ring = false;
                for( int32_t g = pp[ p ]; g < pp[ p + 1 ]; g++ )
-                  if( dep[ g ] >= P0 && bounds[ l + 1 ] - dep[ g ] > kRing ) {
+                  if( bounds[ l + 1 ] - dep[ g ] > kRing ) {
                      R.ring = false;
                      break;
                   }
+            }
+      }
+      // the ring solve's records: row, the entries (dependency position,
+      // value; -1 / 0 past the row's end) and the pivot (1 for the lower factor)
+      std::vector< int32_t > ridx( 4 * std::max< int64_t >( N, 1 ), -1 );
+      std::vector< double > rval( 4 * std::max< int64_t >( N, 1 ), 0.0 );
+      for( int64_t p = 0; p < N; p++ ) {
+         ridx[ 4 * p ] = order[ p ];
+         for( int q = 0; q < kRingPre; q++ )
+            if( pp[ p ] + q < pp[ p + 1 ] ) {
+               ridx[ 4 * p + 1 + q ] = dep[ pp[ p ] + q ];
+               rval[ 4 * p + q ] = val[ pp[ p ] + q ];
+            }
+         rval[ 4 * p + kRingPre ] = upper ? diag[ p ] : 1.0;
       }
       upload_vec( dpp, pp, s );
       upload_vec( ddep, dep, s );
       upload_vec( dval, val, s );
       if( ddiag )
          upload_vec( *ddiag, diag, s );
+      upload_vec( dridx, ridx, s );
+      upload_vec( drval, rval, s );
    };
-   positions( false, lo, P.llev, P.lruns, P.lpp, P.ldep, P.lval, nullptr );
-   positions( true, uo, P.ulev, P.uruns, P.upp, P.udep, P.uval, &P.udiag );
+   positions( false, lo, P.llev, P.lruns, P.lpp, P.ldep, P.lval, nullptr, P.lridx, P.lrval );
+   positions( true, uo, P.ulev, P.uruns, P.upp, P.udep, P.uval, &P.udiag, P.uridx, P.urval );
+   P.bpos.allocate( std::max< int64_t >( N, 1 ) );
    upload_vec( P.dllev, P.llev, s );
    upload_vec( P.dulev, P.ulev, s );
    upload_vec( P.lp, lp, s );
@@ -577,10 +627,16 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
       return;
    }
    // ILU0.hpp:99-113: triangularSolveLower then triangularSolveUpper<reversed rows>
+   auto any_ring = []( const std::vector< tnlmi_preconditioner::Run >& runs ) {
+      return std::any_of( runs.begin(), runs.end(), []( const tnlmi_preconditioner::Run& R ) { return R.ring; } );
+   };
+   const dim3 gpos( (unsigned) std::max< int64_t >( 1, ceil_div( n, 256 ) ) );
+   if( any_ring( P->lruns ) )
+      hipLaunchKernelGGL( gather_pos, gpos, dim3( 256 ), 0, s, n, P->lrows.ptr, b, P->bpos.ptr );
    for( const auto& R : P->lruns ) {
       if( R.ring ) {
          hipLaunchKernelGGL( tri_ring< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, P->dllev.ptr, R.l0,
-                             R.l1, P->lpp.ptr, P->ldep.ptr, P->lval.ptr, (const double*) nullptr, b, x );
+                             R.l1, (const RingIdx*) P->lridx.ptr, (const RingVal*) P->lrval.ptr, P->bpos.ptr, x );
          continue;
       }
       if( R.narrow ) {
@@ -592,10 +648,13 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
       hipLaunchKernelGGL( lower_level, dim3( (unsigned) ceil_div( cnt, 256 ) ), dim3( 256 ), 0, s,
                           P->lrows.ptr + P->llev[ R.l0 ], cnt, P->lp.ptr, P->lc.ptr, P->lv.ptr, b, x );
    }
+   // the upper solve's right-hand side is the lower solve's result
+   if( any_ring( P->uruns ) )
+      hipLaunchKernelGGL( gather_pos, gpos, dim3( 256 ), 0, s, n, P->urows.ptr, (const double*) x, P->bpos.ptr );
    for( const auto& R : P->uruns ) {
       if( R.ring ) {
          hipLaunchKernelGGL( tri_ring< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, P->dulev.ptr, R.l0,
-                             R.l1, P->upp.ptr, P->udep.ptr, P->uval.ptr, P->udiag.ptr, (const double*) x, x );
+                             R.l1, (const RingIdx*) P->uridx.ptr, (const RingVal*) P->urval.ptr, P->bpos.ptr, x );
          continue;
       }
       if( R.narrow ) {
