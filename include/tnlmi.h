@@ -86,6 +86,16 @@ int64_t tnlmi_segments_storage_size( const tnlmi_segments* segments );
 int64_t tnlmi_segments_element_count( const tnlmi_segments* segments );
 tnlmi_format tnlmi_segments_format( const tnlmi_segments* segments );
 
+/* The device-side layout of a segments object (tnlmi_segments_view.h: the
+ * format's index arrays as device pointers owned by the object, and the
+ * getGlobalIndex / getSegmentSize functions over them), for code that walks
+ * the rows of a matrix on the GPU -- the reference's SparseMatrix::getRow,
+ * forRows and forElements (Matrices/SparseMatrixBase.h:239-257, 552-606,
+ * 858-925) over Algorithms::Segments::forElements / forSegments.  The view is
+ * valid while the segments object lives. */
+struct tnlmi_segments_view;
+tnlmi_status tnlmi_segments_get_view( const tnlmi_segments* segments, struct tnlmi_segments_view* out );
+
 /* Index arrays of a segments object, for inspection (device pointers owned by
  * the object).  which: see tnlmi_segments_array.  Returns
  * TNLMI_INVALID_ARGUMENT if the format has no such array. */

@@ -948,6 +948,15 @@ tnlmi_format tnlmi_segments_format( const tnlmi_segments* s )
    return s ? s->format : TNLMI_CSR;
 }
 
+tnlmi_status tnlmi_segments_get_view( const tnlmi_segments* s, tnlmi_segments_view* out )
+{
+   return guarded( [ & ] {
+      if( ! s || ! out )
+         fail( TNLMI_INVALID_ARGUMENT, "segments_get_view: NULL argument" );
+      *out = tnlmi::make_view( *s );
+   } );
+}
+
 tnlmi_status tnlmi_segments_get_array( const tnlmi_segments* s, tnlmi_segments_array which, const int32_t** d_array,
                                        int64_t* length )
 {

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "tnlmi_segments_view.h"
 
 namespace tnlmi {
 struct AutoPlans;
@@ -90,127 +91,21 @@ inline int32_t csr_tile_size( int32_t max_size, int64_t total )
    return max_size <= kWholeRowMax && total <= kSmallTileElems ? kTileNnzSmall : kTileNnz;
 }
 
-// Plain-value view of a segments object for kernels.
-struct SegView
-{
-   int32_t format, col_major;
-   int64_t n;
-   const int32_t* sizes;
-   const int32_t* offsets;
-   // ellpack
-   int32_t ell_width, ell_aligned;
-   // sliced
-   int32_t slice_size;
-   const int32_t* slice_offsets;
-   const int32_t* slice_seg_sizes;
-   // biellpack / sorted
-   const int32_t* perm;
-   const int32_t* group_ptrs;
-   const int32_t* inv_perm;
-   int32_t warp, logw;
-   // chunked
-   const int32_t* ch_slices;
-   const int32_t* seg_to_chunk;
-   const int32_t* seg_to_slice;
-   int32_t chunks_in_slice;
-   // sorted
-   const int32_t* sorted_offsets;
-};
+// Plain-value view of a segments object for kernels: the public device-side
+// layout (include/tnlmi_segments_view.h), which also holds the one
+// implementation of getGlobalIndex / getSegmentSize for every format.
+using SegView = tnlmi_segments_view;
 
 SegView make_view( const tnlmi_segments& s );
 
-// getGlobalIndex( r, k ) of every format (reference file per branch).
 __host__ __device__ inline int64_t global_index( const SegView& v, int64_t r, int32_t k )
 {
-   switch( v.format ) {
-      case TNLMI_CSR:
-      case TNLMI_ADAPTIVE_CSR:
-         return v.offsets[ r ] + k;  // CSRBase.hpp:117-131
-      case TNLMI_ELLPACK:            // EllpackBase.hpp:108-117
-         return v.col_major ? r + (int64_t) v.ell_aligned * k : r * v.ell_width + k;
-      case TNLMI_SLICED_ELLPACK: {  // SlicedEllpackBase.hpp:119-146
-         const int64_t s = r / v.slice_size, in = r % v.slice_size;
-         return v.col_major ? v.slice_offsets[ s ] + in + (int64_t) v.slice_size * k
-                            : v.slice_offsets[ s ] + in * v.slice_seg_sizes[ s ] + k;
-      }
-      case TNLMI_BIELLPACK: {  // detail/BiEllpack.h getGlobalIndexDirect
-         const int64_t strip = r / v.warp;
-         const int64_t g0 = strip * ( v.logw + 1 );
-         const int32_t rsp = v.perm[ r ] - (int32_t) ( strip * v.warp );
-         int groups = 1;
-         {
-            int32_t bis = 1;
-            for( int i = 0; i < v.logw + 1; i++ ) {
-               if( rsp < bis ) {
-                  groups = v.logw + 1 - i;
-                  break;
-               }
-               bis *= 2;
-            }
-         }
-         int32_t local = k, global = v.group_ptrs[ g0 ], height = v.warp;
-         for( int g = 0; g < groups; g++ ) {
-            const int32_t gsize = v.group_ptrs[ g0 + g + 1 ] - v.group_ptrs[ g0 + g ];
-            if( gsize ) {
-               const int32_t gw = gsize / height;
-               if( local >= gw ) {
-                  local -= gw;
-                  global += gsize;
-               }
-               else
-                  return v.col_major ? global + rsp + local * height : global + rsp * gw + local;
-            }
-            height /= 2;
-         }
-         return -1;
-      }
-      case TNLMI_CHUNKED_ELLPACK: {  // detail/ChunkedEllpack.h getGlobalIndexDirect
-         const int32_t si = v.seg_to_slice[ r ];
-         const int32_t first = r != v.ch_slices[ 4 * si + 2 ] ? v.seg_to_chunk[ r - 1 ] : 0;
-         const int32_t off = v.ch_slices[ 4 * si + 3 ], cs = v.ch_slices[ 4 * si + 1 ];
-         return v.col_major ? off + ( k % cs ) * v.chunks_in_slice + first + k / cs : off + first * cs + k;
-      }
-      case TNLMI_SORTED_CSR:  // SortedSegmentsBase.hpp:85-96
-         return v.sorted_offsets[ v.perm[ r ] ] + k;
-   }
-   return -1;
+   return tnlmi_global_index( &v, r, k );
 }
 
-// getSegmentSize( r ) of every format: the slots a traversal of row r visits
-// (Ellpack: the width, EllpackBase.hpp; SlicedEllpack: its slice's width,
-// SlicedEllpackBase.hpp; BiEllpack: the widths of the row's groups and
-// ChunkedEllpack: its chunks x the slice's chunk size, getSegmentSizeDirect in
-// detail/BiEllpack.h and detail/ChunkedEllpack.h; the CSR formats: the size)
 __host__ __device__ inline int32_t segment_size( const SegView& v, int64_t r )
 {
-   switch( v.format ) {
-      case TNLMI_ELLPACK:
-         return v.ell_width;
-      case TNLMI_SLICED_ELLPACK:
-         return v.slice_seg_sizes[ r / v.slice_size ];
-      case TNLMI_BIELLPACK: {
-         const int64_t strip = r / v.warp;
-         const int64_t g0 = strip * ( v.logw + 1 );
-         const int32_t rsp = v.perm[ r ] - (int32_t) ( strip * v.warp );
-         int groups = 1;
-         for( int i = 0, bis = 1; i < v.logw + 1; i++, bis *= 2 )
-            if( rsp < bis ) {
-               groups = v.logw + 1 - i;
-               break;
-            }
-         int32_t size = 0;
-         for( int g = 0, height = v.warp; g < groups; g++, height /= 2 )
-            size += ( v.group_ptrs[ g0 + g + 1 ] - v.group_ptrs[ g0 + g ] ) / height;
-         return size;
-      }
-      case TNLMI_CHUNKED_ELLPACK: {
-         const int32_t si = v.seg_to_slice[ r ];
-         const int32_t first = r != v.ch_slices[ 4 * si + 2 ] ? v.seg_to_chunk[ r - 1 ] : 0;
-         return ( v.seg_to_chunk[ r ] - first ) * v.ch_slices[ 4 * si + 1 ];
-      }
-      default:
-         return v.sizes[ r ];
-   }
+   return tnlmi_segment_size( &v, r );
 }
 
 // The index arrays and counters of a segments object as TNL's segments save

@@ -80,6 +80,11 @@
 #include <TNL/Solvers/Linear/Utils/LinearResidueGetter.h>
 
 #include "tnlmi.h"
+#if defined( __HIPCC__ ) || defined( __HIP__ )
+   #include <TNL/Algorithms/parallelFor.h>
+
+   #include "tnlmi_segments_view.h"
+#endif
 
 namespace TNL {
 
@@ -89,6 +94,9 @@ namespace Devices {
 struct MI355X
 {
    static std::string getDeviceType() { return "Devices::MI355X"; }
+   //! Algorithms::parallelFor< MI355X > takes it (ParallelFor1D.h); nothing to configure
+   struct LaunchConfiguration
+   {};
 };
 }  // namespace Devices
 
@@ -607,6 +615,186 @@ inline tnlmi_launch launch( const Algorithms::Segments::LaunchConfiguration& c )
 }
 }  // namespace MI355XDetail
 
+#if defined( __HIPCC__ ) || defined( __HIP__ )
+// ------------------------------------------- device-side rows (hipcc only)
+// Code compiled by a HIP compiler walks a matrix's rows on the GPU the way the
+// reference's does: SparseMatrix::getView().getRow( i ) inside
+// Algorithms::parallelFor< Devices::MI355X >, forRows / forAllRows with a
+// RowView, and forElements / forAllElements with (row, local index, column,
+// value) (Matrices/SparseMatrixBase.h:239-257, 552-606, 858-925;
+// SparseMatrixRowView.h).  The slots are those of the matrix's segments
+// (tnlmi_segments_view.h: getGlobalIndex and getSegmentSize of every format),
+// column -1 marking a free slot, as in the reference.
+namespace MI355XDetail {
+
+//! A row of a matrix on the device (SparseMatrixRowView.h).  The column
+//! indexes are 32-bit on the device, so getColumnIndex returns a value rather
+//! than a reference to the stored index.
+template< typename Real, typename Index, bool Const >
+class DeviceRowView
+{
+public:
+   using RealType = Real;
+   using IndexType = Index;
+   using ValuePointer = std::conditional_t< Const, const Real*, Real* >;
+   using ColumnPointer = std::conditional_t< Const, const int32_t*, int32_t* >;
+
+   __host__ __device__ DeviceRowView( const tnlmi_segments_view& v, Index row, ValuePointer values,
+                                      ColumnPointer columns )
+   : v_( v ), row_( row ), values_( values ), columns_( columns )
+   {}
+   //! getSegmentSize of the row: its slots (the capacity the format gives it)
+   __host__ __device__ Index getSize() const { return (Index) tnlmi_segment_size( &v_, row_ ); }
+   __host__ __device__ Index getRowIndex() const { return row_; }
+   __host__ __device__ Index getColumnIndex( Index localIdx ) const { return columns_[ slot( localIdx ) ]; }
+   __host__ __device__ auto& getValue( Index localIdx ) const { return values_[ slot( localIdx ) ]; }
+   template< bool C = Const, std::enable_if_t< ! C, bool > = true >
+   __host__ __device__ void setValue( Index localIdx, const Real& value ) const
+   {
+      values_[ slot( localIdx ) ] = value;
+   }
+   template< bool C = Const, std::enable_if_t< ! C, bool > = true >
+   __host__ __device__ void setColumnIndex( Index localIdx, Index column ) const
+   {
+      columns_[ slot( localIdx ) ] = (int32_t) column;
+   }
+   template< bool C = Const, std::enable_if_t< ! C, bool > = true >
+   __host__ __device__ void setElement( Index localIdx, Index column, const Real& value ) const
+   {
+      const int64_t g = slot( localIdx );
+      columns_[ g ] = (int32_t) column;
+      values_[ g ] = value;
+   }
+   static constexpr bool isBinary() { return false; }
+
+private:
+   __host__ __device__ int64_t slot( Index localIdx ) const
+   {
+      return tnlmi_global_index( &v_, row_, (int32_t) localIdx );
+   }
+   tnlmi_segments_view v_;
+   Index row_;
+   ValuePointer values_;
+   ColumnPointer columns_;
+};
+
+//! The device view of a matrix (SparseMatrixView.h): plain values, passed to
+//! kernels by copy.  getRow of a non-const view gives a RowView, of a const
+//! one a ConstRowView, as the reference's.
+template< typename Real, typename Index >
+class DeviceMatrixView
+{
+public:
+   using RealType = Real;
+   using IndexType = Index;
+   using RowView = DeviceRowView< Real, Index, false >;
+   using ConstRowView = DeviceRowView< Real, Index, true >;
+
+   DeviceMatrixView() = default;
+   DeviceMatrixView( const tnlmi_segments_view& v, Index rows, Index columns, Real* values, int32_t* cols )
+   : v_( v ), rows_( rows ), columns_( columns ), values_( values ), cols_( cols )
+   {}
+   __host__ __device__ Index getRows() const { return rows_; }
+   __host__ __device__ Index getColumns() const { return columns_; }
+   __host__ __device__ RowView getRow( Index r ) { return RowView( v_, r, values_, cols_ ); }
+   __host__ __device__ ConstRowView getRow( Index r ) const { return ConstRowView( v_, r, values_, cols_ ); }
+   __host__ __device__ Real* getValuesData() const { return values_; }
+   __host__ __device__ int32_t* getColumnIndexesData() const { return cols_; }
+
+private:
+   tnlmi_segments_view v_{};
+   Index rows_ = 0, columns_ = 0;
+   Real* values_ = nullptr;
+   int32_t* cols_ = nullptr;
+};
+
+// one thread per row of [begin, end), grid-stride
+template< typename View, bool Const, typename Function >
+__global__ void for_rows_kernel( View view, int64_t begin, int64_t end, Function f )
+{
+   using Index = typename View::IndexType;
+   for( int64_t r = begin + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < end;
+        r += (int64_t) gridDim.x * blockDim.x ) {
+      if constexpr( Const ) {
+         const View& cv = view;
+         auto row = cv.getRow( (Index) r );
+         f( row );
+      }
+      else {
+         auto row = view.getRow( (Index) r );
+         f( row );
+      }
+   }
+}
+
+// one thread per row of [begin, end); the row's slots in order, each as
+// ( row, local index, column, value ) -- by reference unless Const; a column
+// index written by the function is stored back as 32 bits
+template< typename View, bool Const, typename Function >
+__global__ void for_elements_kernel( View view, int64_t begin, int64_t end, Function f )
+{
+   using Index = typename View::IndexType;
+   using Real = typename View::RealType;
+   for( int64_t r = begin + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < end;
+        r += (int64_t) gridDim.x * blockDim.x ) {
+      auto row = view.getRow( (Index) r );
+      const Index size = row.getSize();
+      for( Index k = 0; k < size; k++ ) {
+         if constexpr( Const ) {
+            const Index c = row.getColumnIndex( k );
+            const Real& v = row.getValue( k );
+            f( (Index) r, k, c, v );
+         }
+         else {
+            Index c = row.getColumnIndex( k );
+            const Index c0 = c;
+            f( (Index) r, k, c, row.getValue( k ) );
+            if( c != c0 )
+               row.setColumnIndex( k, c );
+         }
+      }
+   }
+}
+
+template< typename Index, typename Function, typename... Args >
+__global__ void parallel_for_kernel( Index begin, Index end, Function f, Args... args )
+{
+   for( Index i = begin + (Index) ( blockIdx.x * blockDim.x + threadIdx.x ); i < end;
+        i += (Index) ( gridDim.x * blockDim.x ) )
+      f( i, args... );
+}
+
+inline dim3 rows_grid( int64_t count )
+{
+   const int64_t blocks = ( count + 255 ) / 256;
+   return dim3( (unsigned) std::max< int64_t >( 1, std::min< int64_t >( blocks, 65536 ) ) );
+}
+
+}  // namespace MI355XDetail
+
+namespace Algorithms::detail {
+//! parallelFor< Devices::MI355X >( begin, end, f, args... ): f( i, args... )
+//! for every i of [begin, end) in a HIP kernel, synchronous on return as the
+//! reference's GPU launch by default (ParallelFor1D.h:69-120)
+template<>
+struct ParallelFor1D< Devices::MI355X >
+{
+   template< typename Index, typename Function, typename... FunctionArgs >
+   static void exec( Index begin, Index end, typename Devices::MI355X::LaunchConfiguration, Function f,
+                     FunctionArgs... args )
+   {
+      if( end <= begin )
+         return;
+      hipLaunchKernelGGL( ( MI355XDetail::parallel_for_kernel< Index, Function, FunctionArgs... > ),
+                          MI355XDetail::rows_grid( (int64_t) ( end - begin ) ), dim3( 256 ), 0, 0, begin, end, f,
+                          args... );
+      MI355XDetail::hip( hipGetLastError() );
+      MI355XDetail::hip( hipDeviceSynchronize() );
+   }
+};
+}  // namespace Algorithms::detail
+#endif
+
 // ------------------------------------------------------------------- matrix
 namespace Matrices {
 
@@ -1005,6 +1193,113 @@ public:
    {
       return { rows_, columns_, segments_, MI355XDetail::dtype< Real >(), values_, columnIndexes_, matrixType() };
    }
+
+#if defined( __HIPCC__ ) || defined( __HIP__ )
+   // ---- rows and elements on the device (code compiled by hipcc) ----
+   using ViewType = MI355XDetail::DeviceMatrixView< Real, Index >;
+   using RowView = typename ViewType::RowView;
+   using ConstRowView = typename ViewType::ConstRowView;
+   //! getView (SparseMatrix.h): the device view, for getRow inside kernels
+   ViewType getView() { return ViewType( segmentsView(), rows_, columns_, values_, columnIndexes_ ); }
+   const ViewType getConstView() const
+   {
+      return ViewType( segmentsView(), rows_, columns_, values_, columnIndexes_ );
+   }
+   //! forRows( begin, end, f ): f( RowView& ) for every row of [begin, end)
+   //! on the device (SparseMatrixBase.hpp:921-935)
+   template< typename Function >
+   void forRows( Index begin, Index end, Function&& function )
+   {
+      launchRows< false >( begin, end, function );
+      modified();
+   }
+   template< typename Function >
+   void forRows( Index begin, Index end, Function&& function ) const
+   {
+      launchRows< true >( begin, end, function );
+   }
+   template< typename Function >
+   void forAllRows( Function&& function )
+   {
+      forRows( (Index) 0, rows_, function );
+   }
+   template< typename Function >
+   void forAllRows( Function&& function ) const
+   {
+      forRows( (Index) 0, rows_, function );
+   }
+   //! forElements( begin, end, f ): f( rowIdx, localIdx, columnIdx, value )
+   //! for every slot of the rows [begin, end), column and value by reference
+   //! (SparseMatrixBase.hpp:746-766); free slots have column -1
+   template< typename Function >
+   void forElements( Index begin, Index end, Function&& function )
+   {
+      launchElements< false >( begin, end, function );
+      modified();
+   }
+   template< typename Function >
+   void forElements( Index begin, Index end, Function&& function ) const
+   {
+      launchElements< true >( begin, end, function );
+   }
+   template< typename Function >
+   void forAllElements( Function&& function )
+   {
+      forElements( (Index) 0, rows_, function );
+   }
+   template< typename Function >
+   void forAllElements( Function&& function ) const
+   {
+      forElements( (Index) 0, rows_, function );
+   }
+
+private:
+   tnlmi_segments_view segmentsView() const
+   {
+      tnlmi_segments_view v{};
+      if( segments_ )
+         MI355XDetail::check( tnlmi_segments_get_view( segments_, &v ) );
+      return v;
+   }
+   void checkRange( Index begin, Index end ) const
+   {
+      if( begin < 0 || end > rows_ || begin > end )
+         throw std::invalid_argument( "row range [" + std::to_string( begin ) + ", " + std::to_string( end )
+                                      + ") outside the matrix's " + std::to_string( rows_ ) + " rows" );
+   }
+   template< bool Const, typename Function >
+   void launchRows( Index begin, Index end, Function& f ) const
+   {
+      checkRange( begin, end );
+      if( begin == end || ! segments_ )
+         return;
+      hipLaunchKernelGGL( ( MI355XDetail::for_rows_kernel< ViewType, Const, std::decay_t< Function > > ),
+                          MI355XDetail::rows_grid( end - begin ), dim3( 256 ), 0, 0, getConstView(),
+                          (int64_t) begin, (int64_t) end, f );
+      MI355XDetail::hip( hipGetLastError() );
+      MI355XDetail::hip( hipDeviceSynchronize() );
+   }
+   template< bool Const, typename Function >
+   void launchElements( Index begin, Index end, Function& f ) const
+   {
+      checkRange( begin, end );
+      if( begin == end || ! segments_ )
+         return;
+      hipLaunchKernelGGL( ( MI355XDetail::for_elements_kernel< ViewType, Const, std::decay_t< Function > > ),
+                          MI355XDetail::rows_grid( end - begin ), dim3( 256 ), 0, 0, getConstView(),
+                          (int64_t) begin, (int64_t) end, f );
+      MI355XDetail::hip( hipGetLastError() );
+      MI355XDetail::hip( hipDeviceSynchronize() );
+   }
+   // the values or columns were written on the device: derived plans drop
+   void modified()
+   {
+      const tnlmi_matrix m = abi();
+      MI355XDetail::check( tnlmi_matrix_modified( &m ) );
+   }
+
+public:
+#endif
    static constexpr bool isSymmetric() { return MatrixType::isSymmetric(); }
    static constexpr int32_t matrixType() { return MatrixType::isSymmetric() ? TNLMI_SYMMETRIC : TNLMI_GENERAL; }
 
