@@ -827,7 +827,8 @@ __global__ void csr_to_ell( int64_t n, int64_t stride, int32_t w, const int32_t*
 // here on every solve, and runs the fused Ellpack kernels on it.  Every row
 // keeps its entries in CSR order, so the products and sums are those of the
 // Ellpack kernels on the same matrix stored as Ellpack.  TNLMI_CG_CSR_COPY=0
-// in the environment keeps the one-launch CSR kernel instead.
+// in the environment keeps the one-launch CSR kernel instead, and so does a
+// device without room for the copy.
 bool csr_ell_copy( Stage& S, hipStream_t s )
 {
    const char* env = std::getenv( "TNLMI_CG_CSR_COPY" );
@@ -843,8 +844,19 @@ bool csr_ell_copy( Stage& S, hipStream_t s )
    const size_t vs = A.value_type == TNLMI_F64 ? 8 : A.value_type == TNLMI_F32 ? 4 : 2;
    S.ecols = std::make_shared< DeviceArray< int32_t > >();
    S.evals = std::make_shared< DeviceArray< char > >();
-   S.ecols->allocate( stride * w );
-   S.evals->allocate( stride * w * (int64_t) vs );
+   try {
+      S.ecols->allocate( stride * w );
+      S.evals->allocate( stride * w * (int64_t) vs );
+   }
+   catch( const Error& e ) {
+      // no room for the copy: the one-launch CSR kernel on the matrix itself
+      if( e.status != TNLMI_BAD_ALLOC )
+         throw;
+      (void) hipGetLastError();
+      S.ecols.reset();
+      S.evals.reset();
+      return false;
+   }
    const dim3 grid( (unsigned) ceil_div( stride, 256 ) ), block( 256 );
    switch( A.value_type ) {
       case TNLMI_F64:
