@@ -82,6 +82,40 @@ __device__ __forceinline__ double sum_parts( const double* __restrict__ part, do
    return block_sum< kFT >( a, sm );
 }
 
+// N partial-sum arrays at once: their loads in flight together and one pair
+// of barriers, each total summed in the same order as sum_parts (so equal to
+// it bit for bit).  sm holds N * kFT / 64 doubles.
+template< int N >
+__device__ __forceinline__ void sum_parts_n( const double* const ( &part )[ N ], double* sm, double ( &out )[ N ] )
+{
+   double a[ N ];
+#pragma unroll
+   for( int j = 0; j < N; j++ )
+      a[ j ] = 0;
+   for( int i = threadIdx.x; i < kFB; i += kFT )
+#pragma unroll
+      for( int j = 0; j < N; j++ )
+         a[ j ] += part[ j ][ i ];
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+   for( int j = 0; j < N; j++ )
+      a[ j ] = group_sum< 64 >( a[ j ] );
+   __syncthreads();
+   if( lane == 0 )
+#pragma unroll
+      for( int j = 0; j < N; j++ )
+         sm[ j * ( kFT / 64 ) + w ] = a[ j ];
+   __syncthreads();
+#pragma unroll
+   for( int j = 0; j < N; j++ ) {
+      double r = 0;
+#pragma unroll
+      for( int i = 0; i < kFT / 64; i++ )
+         r += sm[ j * ( kFT / 64 ) + i ];
+      out[ j ] = wave_uniform( r );
+   }
+}
+
 __device__ __forceinline__ void put_part( double v, double* sm, double* part )
 {
    v = block_sum< kFT >( v, sm );
@@ -495,15 +529,26 @@ __global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __rest
                                                  const T* __restrict__ r, const D* __restrict__ d, T* __restrict__ p,
                                                  const CgDev* __restrict__ cur, CgDev* __restrict__ nxt, StopRule R )
 {
-   __shared__ double sm[ kFT / 64 ];
+   __shared__ double sm[ 3 * ( kFT / 64 ) ];
    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
    if( ! cur->active ) {
       if( lead )
          *nxt = *cur;
       return;
    }
-   const double s2 = sum_parts( part1, sm );
-   const double s1n = sum_parts( part2, sm );
+   double t[ 3 ] = { 0, 0, 0 };
+   if( R.kind == 1 ) {
+      double t3[ 3 ];
+      sum_parts_n< 3 >( { part1, part2, part3 }, sm, t3 );
+      t[ 0 ] = t3[ 0 ], t[ 1 ] = t3[ 1 ], t[ 2 ] = t3[ 2 ];
+   }
+   else {
+      double t2[ 2 ];
+      sum_parts_n< 2 >( { part1, part2 }, sm, t2 );
+      t[ 0 ] = t2[ 0 ], t[ 1 ] = t2[ 1 ];
+   }
+   const double s2 = t[ 0 ];
+   const double s1n = t[ 1 ];
    if( s2 == 0.0 ) {
       if( lead ) {
          *nxt = *cur;
@@ -515,7 +560,7 @@ __global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __rest
    }
    const double s1 = cur->s1;
    const double beta = s1 == 0.0 ? 0.0 : s1n / s1;
-   const double res = sqrt( R.kind == 1 ? sum_parts( part3, sm ) : s1n ) / cur->normb;
+   const double res = sqrt( R.kind == 1 ? t[ 2 ] : s1n ) / cur->normb;
    const bool go = check_next( R, res, cur->it );
    if( lead ) {
       *nxt = *cur;
@@ -548,16 +593,16 @@ __global__ void __launch_bounds__( kFT ) mx_dir( int64_t n, const double* __rest
                                                  float* __restrict__ p, const CgDev* __restrict__ cur,
                                                  CgDev* __restrict__ nxt, double delta2, double tol, int64_t maxit )
 {
-   __shared__ double sm[ kFT / 64 ];
+   __shared__ double sm[ 3 * ( kFT / 64 ) ];
    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
    if( ! cur->active ) {
       if( lead )
          *nxt = *cur;
       return;
    }
-   const double s2 = sum_parts( part1, sm );
-   const double s1n = sum_parts( part2, sm );
-   const double rr = sum_parts( part3, sm );
+   double t[ 3 ];
+   sum_parts_n< 3 >( { part1, part2, part3 }, sm, t );
+   const double s2 = t[ 0 ], s1n = t[ 1 ], rr = t[ 2 ];
    const double s1 = cur->s1;
    const int64_t it = cur->it + 1;
    const double beta = s1 == 0.0 ? 0.0 : s1n / s1;
