@@ -79,10 +79,10 @@
 #include <TNL/Solvers/Linear/TFQMR.h>
 #include <TNL/Solvers/Linear/Utils/LinearResidueGetter.h>
 
+#include <TNL/Algorithms/parallelFor.h>
+
 #include "tnlmi.h"
 #if defined( __HIPCC__ ) || defined( __HIP__ )
-   #include <TNL/Algorithms/parallelFor.h>
-
    #include "tnlmi_segments_view.h"
 #endif
 
@@ -790,6 +790,21 @@ struct ParallelFor1D< Devices::MI355X >
                           args... );
       MI355XDetail::hip( hipGetLastError() );
       MI355XDetail::hip( hipDeviceSynchronize() );
+   }
+};
+}  // namespace Algorithms::detail
+#else
+namespace Algorithms::detail {
+//! Without a HIP compiler there is no device code for the loop body: a
+//! parallelFor on Devices::MI355X does not compile (rather than run a device
+//! lambda on the host)
+template<>
+struct ParallelFor1D< Devices::MI355X >
+{
+   template< typename Index, typename Function, typename... FunctionArgs >
+   static void exec( Index, Index, typename Devices::MI355X::LaunchConfiguration, Function, FunctionArgs... )
+   {
+      static_assert( sizeof( Function ) == 0, "parallelFor< Devices::MI355X > needs code compiled by hipcc" );
    }
 };
 }  // namespace Algorithms::detail
