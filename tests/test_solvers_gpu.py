@@ -285,6 +285,26 @@ def test_ilu0_apply_narrow_and_wide_levels(orc):
             np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
 
 
+def test_ilu0_apply_tiny_matrices(orc):
+    """ILU(0) on matrices of 1, 2, 3 and 5 rows (one ring run of one or a few
+    levels of a single row; the stages past the run's last level load nothing
+    to solve) against the oracle's solve."""
+    rng = np.random.default_rng(41)
+    for n in (1, 2, 3, 5):
+        rows = [np.arange(max(r - 1, 0), min(r + 2, n), dtype=np.int32) for r in range(n)]
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum([len(c) for c in rows])
+        cols = np.concatenate(rows)
+        vals = np.where(cols == np.repeat(np.arange(n), np.diff(off)), 3.0 + rng.random(len(cols)),
+                        -1.0 + 0.2 * rng.random(len(cols)))
+        b = rng.standard_normal(n)
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+        L, U = orc.ilu0(off, cols, vals)
+        want = orc.ilu0_solve(L, U, b)
+        got = S.ILU0(A).solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+
+
 def test_blas1(orc):
     rng = np.random.default_rng(5)
     for n in [1, 1000, 3_000_001]:
