@@ -649,6 +649,9 @@ def wl_batched_cg(args):
                        "loop_sample": f"{sample} systems timed, scaled by {nsys}/{sample}"})
     out["roofline"] = roofline(out_res["fp64"]["GBs"])
     out["roofline"]["kernel"] = "batched_cg64 (FP64)"
+    out["roofline"]["note"] = ("achieved counts the bytes one CG iteration of each system is designed to move; a "
+                               "system (<= 16,384 rows) stays in its CU and its XCD's L2 during its solve, so this "
+                               "is not an HBM figure")
     out["cpu_baseline"] = None
     return out
 
