@@ -305,6 +305,34 @@ def test_ilu0_apply_tiny_matrices(orc):
         np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
 
 
+def test_ilu0_flow_and_ring_solves_agree(orc, monkeypatch):
+    """The sync-free triangular solve (rows waiting on per-position stamps)
+    and the level-by-level ring solve give the oracle's ILU(0) solve; with a
+    spin limit of one poll the sync-free solve gives up on waiting rows and
+    returns NaN instead of hanging."""
+    off, cols, vals, _ = tnl.convdiff(200, seed=5)
+    n = len(off) - 1
+    b = np.random.default_rng(43).standard_normal(n)
+    L, U = orc.ilu0(off, cols, vals)
+    want = orc.ilu0_solve(L, U, b)
+    got = {}
+    for flow in ("1", "0"):
+        monkeypatch.setenv("TNLMI_ILU_FLOW", flow)
+        A = tnl.SparseMatrix.from_csr(n, n, off, cols, vals)
+        P = S.ILU0(A)
+        got[flow] = P.solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+        np.testing.assert_allclose(got[flow], want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+    np.testing.assert_array_equal(got["1"], got["0"])
+    monkeypatch.setenv("TNLMI_ILU_FLOW", "1")
+    P = S.ILU0(tnl.SparseMatrix.from_csr(n, n, off, cols, vals))
+    monkeypatch.setenv("TNLMI_ILU_SPIN_CAP", "1")
+    bad = P.solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+    assert np.isnan(bad).any()
+    monkeypatch.delenv("TNLMI_ILU_SPIN_CAP")
+    again = P.solve(dev(b), torch.zeros(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+    np.testing.assert_array_equal(again, got["1"])
+
+
 def test_blas1(orc):
     rng = np.random.default_rng(5)
     for n in [1, 1000, 3_000_001]:

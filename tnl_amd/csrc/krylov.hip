@@ -40,6 +40,7 @@ struct tnlmi_preconditioner
       int64_t l0, l1;
       bool narrow;
       bool ring;  // every dependency of the run lies within kRing positions of its level's end
+      bool flow;  // every dependency lies within kFlowDist positions of its row (the sync-free solve)
    };
    std::vector< Run > lruns, uruns;
    // the factors in level-order positions (for the LDS-ring solve): per position p
@@ -52,6 +53,7 @@ struct tnlmi_preconditioner
    // the right-hand side of a solve gathered into position order before it
    DeviceArray< int32_t > lridx, uridx;
    DeviceArray< double > lrval, urval, bpos;
+   DeviceArray< double > xpos;  // the sync-free solve's results in position order (+ kLvThreads spare slots)
    tnlmi_matrix M{};                   // explicit: x = M b
 };
 
@@ -283,7 +285,7 @@ template< bool UPPER >
 __global__ void __launch_bounds__( kLvThreads )
    tri_ring( const int32_t* __restrict__ rows, const int64_t* __restrict__ bounds, int64_t l0, int64_t l1,
              const RingIdx* __restrict__ ridx, const RingVal* __restrict__ rval, const double* __restrict__ bpos,
-             double* x )
+             const double* __restrict__ x, double* __restrict__ xpos, int64_t N )
 {
    __shared__ double ring[ kRing ];
    __shared__ int32_t lb[ kRunLevels + 1 ];  // the run's level bounds (a stage's addresses wait for no scalar load)
@@ -318,8 +320,9 @@ __global__ void __launch_bounds__( kLvThreads )
 #pragma unroll
       for( int d = 0; d < kRingDepth; d++ ) {
          const RingRow& w = st[ d ];
+         double xi = 0.0;
          if( w.p >= 0 ) {
-            double xi = w.bi;
+            xi = w.bi;
 #pragma unroll
             for( int q = 0; q < kRingPre; q++ )
                if( w.r.q[ q ] >= 0 )
@@ -327,14 +330,154 @@ __global__ void __launch_bounds__( kLvThreads )
             if( UPPER )
                xi = xi / w.v.dii;
             ring[ w.p & ( kRing - 1 ) ] = xi;
-            x[ w.r.i ] = xi;
          }
+         // the value into xpos, in position order: a coalesced store that
+         // every lane issues (a lane without a row stores to a spare slot past
+         // N), where a store into x would scatter
+         xpos[ w.p >= 0 ? (int64_t) w.p : N + threadIdx.x ] = xi;
          load( st[ d ], l + d + kRingDepth );  // the stage's next level, kRingDepth ahead
-         // LDS alone orders the levels: the global stores of x are read
-         // back by no later level of this launch
+         // LDS alone orders the levels: the stores of xpos are read back by no
+         // later level of this launch
          __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
          __builtin_amdgcn_s_barrier();
          __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
+      }
+   }
+}
+
+// x[rows[p]] = xpos[p] for the positions [P0, P1) of a sync-free run
+__global__ void scatter_pos( int64_t P0, int64_t P1, const int32_t* __restrict__ rows,
+                             const double* __restrict__ xpos, double* __restrict__ x )
+{
+   const int64_t p = P0 + (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if( p < P1 )
+      x[ rows[ p ] ] = xpos[ p ];
+}
+
+// The sync-free form of a run, without a barrier per level: thread t solves
+// the run's positions P0 + t, P0 + t + kLvThreads, ... in turn (chunk j =
+// the kLvThreads positions P0 + j kLvThreads ...), and a row waits only for
+// the positions it depends on: LDS holds a ring of solved values with a stamp
+// per slot (the slot of position q holds q's value once its stamp reads q).
+// The slot of q is reused by q + kRing only after every position of chunk
+// chunk(q + kRing) - kFlowLag is solved (a per-chunk count of the waves done
+// with it); a position's readers lie at most kFlowDist positions after it
+// (checked when the factors are built), so by then they have read it.
+// Every wait is bounded: a row whose wait runs past spin_cap polls is
+// published as NaN and the whole launch then stops waiting (its dependents,
+// and the solve, carry NaN), so a fault in the schedule cannot hang the
+// workgroup.  A row's records are requested kFlowDepth chunks ahead, and every
+// global load and store of an iteration is unconditional (the waits between
+// them count them exactly).
+constexpr int kFlowLag = 4;                                // chunks
+constexpr int kFlowDist = kRing - kFlowLag * kLvThreads;  // 4096 positions
+constexpr int kFlowSlots = 16;                            // chunk counters in use at once (> kFlowLag + 1)
+constexpr int kFlowDepth = 4;
+constexpr int64_t kFlowSpinCap = int64_t( 1 ) << 20;
+
+template< bool UPPER >
+__global__ void __launch_bounds__( kLvThreads )
+   tri_flow( const int32_t* __restrict__ rows, int64_t P0, int64_t P1, const RingIdx* __restrict__ ridx,
+             const RingVal* __restrict__ rval, const double* __restrict__ bpos, const double* __restrict__ x,
+             double* __restrict__ xpos, int64_t N, int64_t spin_cap )
+{
+   __shared__ double ring[ kRing ];
+   __shared__ int32_t stamp[ kRing ];
+   __shared__ int32_t done[ kFlowSlots ];
+   __shared__ int32_t failed;
+   constexpr int32_t kMask = kRing - 1;
+   const int t = threadIdx.x, lane = t & 63;
+   // the kRing positions before the run (solved by earlier launches); a slot
+   // of a position before 0 gets the stamp -1, which no position matches
+   for( int k = t; k < kRing; k += kLvThreads ) {
+      const int64_t q = P0 - kRing + k;
+      const int32_t slot = (int32_t) ( q & kMask );
+      if( q >= 0 ) {
+         ring[ slot ] = x[ rows[ q ] ];
+         stamp[ slot ] = (int32_t) q;
+      }
+      else
+         stamp[ slot ] = -1;
+   }
+   if( t < kFlowSlots )
+      done[ t ] = 0;
+   if( t == 0 )
+      failed = 0;
+   __syncthreads();
+   const int64_t J = ( P1 - P0 + kLvThreads - 1 ) / kLvThreads;  // chunks
+   RingRow st[ kFlowDepth ];
+   auto load = [ & ]( RingRow& R, int64_t j ) {
+      const int64_t p = P0 + j * kLvThreads + t;
+      ring_load( R, p - t, P1, ridx, rval, bpos );  // position p at a clamped index; -1 past the run
+   };
+#pragma unroll
+   for( int d = 0; d < kFlowDepth; d++ )
+      load( st[ d ], d );
+   auto stamped = [ & ]( int32_t q ) {
+      return __hip_atomic_load( &stamp[ q & kMask ], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP ) == q;
+   };
+   for( int64_t j0 = 0; j0 < J; j0 += kFlowDepth ) {
+#pragma unroll
+      for( int d = 0; d < kFlowDepth; d++ ) {
+         const int64_t j = j0 + d;
+         const RingRow& w = st[ d ];
+         // the slots of chunk j were last used kRing positions ago: wait until
+         // chunk j - kFlowLag is solved (the same value for every lane)
+         if( j >= kFlowLag && j < J ) {
+            const int64_t c = j - kFlowLag;
+            const int32_t target = (int32_t) ( ( c / kFlowSlots + 1 ) * kLvThreads );
+            for( int64_t n = 0; n < spin_cap; n++ ) {
+               if( __hip_atomic_load( &done[ c % kFlowSlots ], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP )
+                      >= target
+                   || __hip_atomic_load( &failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP ) )
+                  break;
+               __builtin_amdgcn_s_sleep( 1 );
+            }
+         }
+         // this lane's row: solved as soon as its dependencies are stamped;
+         // the wave passes until none of its lanes waits
+         bool pending = w.p >= 0;
+         double xi = 0.0;
+         for( int64_t n = 0;; n++ ) {
+            if( pending ) {
+               bool ready = true;
+#pragma unroll
+               for( int q = 0; q < kRingPre; q++ )
+                  if( w.r.q[ q ] >= 0 )
+                     ready = ready && stamped( w.r.q[ q ] );
+               const bool give_up =
+                  ! ready
+                  && ( n >= spin_cap || __hip_atomic_load( &failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP ) );
+               if( ready || give_up ) {
+                  __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup", "local" );
+                  xi = w.bi;
+#pragma unroll
+                  for( int q = 0; q < kRingPre; q++ )
+                     if( w.r.q[ q ] >= 0 )
+                        xi -= w.v.v[ q ] * ring[ w.r.q[ q ] & kMask ];
+                  if( UPPER )
+                     xi = xi / w.v.dii;
+                  if( give_up ) {
+                     xi = __builtin_nan( "" );
+                     __hip_atomic_store( &failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+                  }
+                  ring[ w.p & kMask ] = xi;
+                  __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup", "local" );
+                  __hip_atomic_store( &stamp[ w.p & kMask ], w.p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+                  pending = false;
+               }
+            }
+            if( __ballot( pending ) == 0 )
+               break;
+            __builtin_amdgcn_s_sleep( 1 );
+         }
+         // the value into xpos, in position order (coalesced; a lane without a
+         // row stores to a spare slot past N: every lane stores)
+         xpos[ w.p >= 0 ? (int64_t) w.p : N + t ] = xi;
+         // this wave is done with chunk j: its reads of the ring are complete
+         if( j < J && lane == 0 )
+            __hip_atomic_fetch_add( &done[ j % kFlowSlots ], 64, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP );
+         load( st[ d ], j + kFlowDepth );  // the stage's next chunk, kFlowDepth ahead
       }
    }
 }
@@ -520,7 +663,7 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
          if( narrow )
             while( e < L && e - l < kRunLevels && bounds[ e + 1 ] - bounds[ e ] <= (int64_t) kLvThreads * kLvRows )
                e++;
-         runs.push_back( { l, e, narrow } );
+         runs.push_back( { l, e, narrow, false, false } );
          l = e;
       }
    };
@@ -554,6 +697,22 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
             val.push_back( vv[ g ] );
          }
          pp[ p + 1 ] = (int32_t) dep.size();
+      }
+      // the sync-free solve: rows of at most kRingPre entries, every
+      // dependency at most kFlowDist positions before its row
+      const char* fenv = std::getenv( "TNLMI_ILU_FLOW" );
+      const bool flow_on = ! ( fenv && std::string( fenv ) == "0" );
+      for( auto& R : runs ) {
+         R.flow = flow_on;
+         for( int64_t p = bounds[ R.l0 ]; p < bounds[ R.l1 ] && R.flow; p++ ) {
+            if( pp[ p + 1 ] - pp[ p ] > kRingPre )
+               R.flow = false;
+            for( int32_t g = pp[ p ]; g < pp[ p + 1 ]; g++ )
+               if( p - dep[ g ] > kFlowDist ) {
+                  R.flow = false;
+                  break;
+               }
+         }
       }
       for( auto& R : runs ) {
          R.ring = R.narrow;
@@ -596,6 +755,7 @@ void ilu0_build( tnlmi_preconditioner& P, const tnlmi_matrix& A, hipStream_t s )
    positions( false, lo, P.llev, P.lruns, P.lpp, P.ldep, P.lval, nullptr, P.lridx, P.lrval );
    positions( true, uo, P.ulev, P.uruns, P.upp, P.udep, P.uval, &P.udiag, P.uridx, P.urval );
    P.bpos.allocate( std::max< int64_t >( N, 1 ) );
+   P.xpos.allocate( std::max< int64_t >( N, 1 ) + kLvThreads );
    upload_vec( P.dllev, P.llev, s );
    upload_vec( P.dulev, P.ulev, s );
    upload_vec( P.lp, lp, s );
@@ -628,15 +788,35 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
    }
    // ILU0.hpp:99-113: triangularSolveLower then triangularSolveUpper<reversed rows>
    auto any_ring = []( const std::vector< tnlmi_preconditioner::Run >& runs ) {
-      return std::any_of( runs.begin(), runs.end(), []( const tnlmi_preconditioner::Run& R ) { return R.ring; } );
+      return std::any_of( runs.begin(), runs.end(),
+                          []( const tnlmi_preconditioner::Run& R ) { return R.ring || R.flow; } );
    };
+   int64_t spin_cap = kFlowSpinCap;
+   if( const char* e = std::getenv( "TNLMI_ILU_SPIN_CAP" ) )
+      spin_cap = std::max< int64_t >( 1, std::atoll( e ) );
    const dim3 gpos( (unsigned) std::max< int64_t >( 1, ceil_div( n, 256 ) ) );
+   // the ring and sync-free solves leave their results in position order in xpos
+   auto scatter_run = [ & ]( const int32_t* rows, int64_t p0, int64_t p1 ) {
+      if( p1 > p0 )
+         hipLaunchKernelGGL( scatter_pos, dim3( (unsigned) ceil_div( p1 - p0, 256 ) ), dim3( 256 ), 0, s, p0, p1,
+                             rows, (const double*) P->xpos.ptr, x );
+   };
    if( any_ring( P->lruns ) )
       hipLaunchKernelGGL( gather_pos, gpos, dim3( 256 ), 0, s, n, P->lrows.ptr, b, P->bpos.ptr );
    for( const auto& R : P->lruns ) {
+      if( R.flow ) {
+         const int64_t p0 = P->llev[ R.l0 ], p1 = P->llev[ R.l1 ];
+         hipLaunchKernelGGL( tri_flow< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, p0, p1,
+                             (const RingIdx*) P->lridx.ptr, (const RingVal*) P->lrval.ptr, P->bpos.ptr,
+                             (const double*) x, P->xpos.ptr, n, spin_cap );
+         scatter_run( P->lrows.ptr, p0, p1 );
+         continue;
+      }
       if( R.ring ) {
          hipLaunchKernelGGL( tri_ring< false >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->lrows.ptr, P->dllev.ptr, R.l0,
-                             R.l1, (const RingIdx*) P->lridx.ptr, (const RingVal*) P->lrval.ptr, P->bpos.ptr, x );
+                             R.l1, (const RingIdx*) P->lridx.ptr, (const RingVal*) P->lrval.ptr, P->bpos.ptr,
+                             (const double*) x, P->xpos.ptr, n );
+         scatter_run( P->lrows.ptr, P->llev[ R.l0 ], P->llev[ R.l1 ] );
          continue;
       }
       if( R.narrow ) {
@@ -652,9 +832,19 @@ void precond_apply( const tnlmi_preconditioner* P, int64_t n, const double* b, d
    if( any_ring( P->uruns ) )
       hipLaunchKernelGGL( gather_pos, gpos, dim3( 256 ), 0, s, n, P->urows.ptr, (const double*) x, P->bpos.ptr );
    for( const auto& R : P->uruns ) {
+      if( R.flow ) {
+         const int64_t p0 = P->ulev[ R.l0 ], p1 = P->ulev[ R.l1 ];
+         hipLaunchKernelGGL( tri_flow< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, p0, p1,
+                             (const RingIdx*) P->uridx.ptr, (const RingVal*) P->urval.ptr, P->bpos.ptr,
+                             (const double*) x, P->xpos.ptr, n, spin_cap );
+         scatter_run( P->urows.ptr, p0, p1 );
+         continue;
+      }
       if( R.ring ) {
          hipLaunchKernelGGL( tri_ring< true >, dim3( 1 ), dim3( kLvThreads ), 0, s, P->urows.ptr, P->dulev.ptr, R.l0,
-                             R.l1, (const RingIdx*) P->uridx.ptr, (const RingVal*) P->urval.ptr, P->bpos.ptr, x );
+                             R.l1, (const RingIdx*) P->uridx.ptr, (const RingVal*) P->urval.ptr, P->bpos.ptr,
+                             (const double*) x, P->xpos.ptr, n );
+         scatter_run( P->urows.ptr, P->ulev[ R.l0 ], P->ulev[ R.l1 ] );
          continue;
       }
       if( R.narrow ) {
