@@ -29,7 +29,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "segments.hpp"
@@ -95,11 +97,13 @@ __device__ __forceinline__ int64_t block_of( int64_t B, const int64_t* __restric
    return lo;
 }
 
-// the sorted arrays: value, local column, slot within the row block
+// the sorted arrays: value, local column; and the slot within the row block
+// of each nonzero of window k < K (chunk (b, k)), at i + dbase[b K + k]
 template< typename V >
 __global__ void w_fill( int64_t nnz, int32_t win, const int32_t* __restrict__ perm, const int32_t* __restrict__ cols,
-                        const V* __restrict__ vals, int64_t B, const int64_t* __restrict__ E, V* __restrict__ pv,
-                        uint16_t* __restrict__ pc, uint16_t* __restrict__ dest )
+                        const V* __restrict__ vals, int64_t B, const int64_t* __restrict__ E,
+                        const int32_t* __restrict__ skey, int32_t K, const int32_t* __restrict__ dbase,
+                        V* __restrict__ pv, uint16_t* __restrict__ pc, uint16_t* __restrict__ dest )
 {
    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    if( i >= nnz )
@@ -108,7 +112,11 @@ __global__ void w_fill( int64_t nnz, int32_t win, const int32_t* __restrict__ pe
    const int32_t c = cols[ e ];
    pv[ i ] = vals[ e ];
    pc[ i ] = (uint16_t) ( c >= 0 ? c % win : 0 );
-   dest[ i ] = (uint16_t) ( e - E[ block_of( B, E, e ) ] );
+   const int32_t k = skey[ i ];
+   if( k < K ) {  // window K: padding slots, in no chunk
+      const int64_t b = block_of( B, E, e );
+      dest[ i + dbase[ b * K + k ] ] = (uint16_t) ( e - E[ b ] );
+   }
 }
 
 // seg[b * K + k] = first position of window k holding a nonzero e >= E[b]
@@ -211,17 +219,18 @@ __device__ __forceinline__ C sum_row( const C* buf, int32_t q0, int32_t q1 )
    return sum;
 }
 
-// PER terms at i (products and their 16-bit slots), loaded unconditionally:
-// prod and dest carry kPad entries of padding, so a read past a segment's
-// end stays in bounds (and what it reads is not stored)
+// PER terms at i (products at i, their 16-bit slots at i + di), loaded
+// unconditionally: prod and dest carry kPad entries of padding, so a read past
+// a segment's end stays in bounds (and what it reads is not stored); di is a
+// multiple of 4, so both reads have the same alignment
 template< typename C, int PER >
-__device__ __forceinline__ void get_pair( const C* prod, const uint16_t* dest, int32_t i, C ( &v )[ PER ],
-                                          uint32_t ( &d )[ PER / 2 ] )
+__device__ __forceinline__ void get_pair( const C* prod, const uint16_t* dest, int32_t i, int32_t di,
+                                          C ( &v )[ PER ], uint32_t ( &d )[ PER / 2 ] )
 {
 #pragma unroll
    for( int u = 0; u < PER; u++ )
       v[ u ] = prod[ i + u ];
-   __builtin_memcpy( d, dest + i, sizeof( d ) );
+   __builtin_memcpy( d, dest + ( i + di ), sizeof( d ) );
 }
 
 template< typename C, int PER >
@@ -237,10 +246,11 @@ __device__ __forceinline__ void put_pair( C* buf, const C ( &v )[ PER ], const u
 template< typename C, int CAP, bool BETA, int SEGS, int PER, bool ALIGN >
 __global__ void __launch_bounds__( kP2 )
    w_phase2( int64_t B, int32_t K, const BlockHdr* __restrict__ hdr, const int32_t* __restrict__ seg,
-             const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ off,
+             const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ dbase,
+             const int32_t* __restrict__ off,
              const int32_t* __restrict__ lrows, const uint16_t* __restrict__ rorder, C* __restrict__ part,
              C* __restrict__ y, C alpha, C beta,
-             int64_t begin, int64_t end )
+             int64_t begin, int64_t end, int32_t across )
 {
    __shared__ C buf[ CAP ];
    constexpr int P2 = kP2;
@@ -262,14 +272,20 @@ __global__ void __launch_bounds__( kP2 )
    const int lane = t & 63, wid = wave_uniform( (int32_t) ( t >> 6 ) );
    const int gl = lane % kGrp, gb = lane - gl;
    // segment sx (window sx) of the block: its bounds, held by lane gb + i
-   // (i < R) of the group's lanes
-   const int32_t sx = wid * ( 64 / kGrp ) * R + ( lane / kGrp ) * R + std::min( gl, R - 1 );
+   // (i < R) of the group's lanes.  A wave takes G R consecutive segments;
+   // across: in round i its G groups read segments G i .. G i + G - 1 of
+   // them, whose slots lie side by side (the slot array is block-major);
+   // else group g reads segment g R + i
+   constexpr int G = 64 / kGrp;
+   const int32_t sx = wid * G * R
+                      + ( across ? std::min( gl, R - 1 ) * G + lane / kGrp : ( lane / kGrp ) * R + std::min( gl, R - 1 ) );
    const int32_t kt = sx < K ? sx : K - 1;
    // both bounds loaded unconditionally (a load under the condition would
    // wait for the first before it is issued)
    const int32_t a = seg[ b * K + kt ];
    const int32_t zk = seg[ ( b + 1 ) * K + kt ];
    const int32_t z = sx < K ? zk : a;
+   const int32_t da = dbase[ b * K + kt ];
    // the rows of this thread: ranks t and t + P2 of the block's rows ordered
    // by length, longest first (a block has at most 2 P2 rows), so the rows
    // of a wave, each summed serially by its lane, are about equally long;
@@ -290,9 +306,9 @@ __global__ void __launch_bounds__( kP2 )
    for( int i = 0; i < R; i++ ) {
       // lanes past the segment's end read its last term again (no extra
       // cache line per request); what they read is not stored
-      const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 );
-      get_pair< C, PER >( prod, dest, std::min( ( ai & kAl ) + PER * gl, std::max( zi - 1, ai ) & kAl ), v[ i ],
-                          d[ i ] );
+      const int32_t ai = __shfl( a, gb + i, 64 ), zi = __shfl( z, gb + i, 64 ), di = __shfl( da, gb + i, 64 );
+      get_pair< C, PER >( prod, dest, std::min( ( ai & kAl ) + PER * gl, std::max( zi - 1, ai ) & kAl ), di,
+                          v[ i ], d[ i ] );
    }
 #pragma unroll
    for( int j = 0; j < J; j++ )
@@ -315,10 +331,11 @@ __global__ void __launch_bounds__( kP2 )
       const int32_t a0 = ai & kAl;
       put_pair< C, PER >( buf, v[ i ], d[ i ], a0 + PER * gl, zi, ai );
       // on demand: the rest of a segment longer than kSpan
+      const int32_t di = __shfl( da, gb + i, 64 );
       for( int32_t i0 = a0 + kSpan + PER * gl; i0 < zi; i0 += kSpan ) {
          C w[ PER ];
          uint32_t dw[ PER / 2 ];
-         get_pair< C, PER >( prod, dest, i0, w, dw );
+         get_pair< C, PER >( prod, dest, i0, di, w, dw );
          put_pair< C, PER >( buf, w, dw, i0, zi );
       }
    }
@@ -326,12 +343,13 @@ __global__ void __launch_bounds__( kP2 )
    for( int32_t k0 = SEGS; k0 < K; k0 += SEGS ) {
       const int32_t k = std::min( k0 + sx, K - 1 );
       const int32_t ak = seg[ b * K + k ], zk = k0 + sx < K ? seg[ ( b + 1 ) * K + k ] : ak;
+      const int32_t dk = dbase[ b * K + k ];
       for( int i = 0; i < R; i++ ) {
-         const int32_t ai = __shfl( ak, gb + i, 64 ), zi = __shfl( zk, gb + i, 64 );
+         const int32_t ai = __shfl( ak, gb + i, 64 ), zi = __shfl( zk, gb + i, 64 ), di = __shfl( dk, gb + i, 64 );
          for( int32_t i0 = ( ai & kAl ) + PER * gl; i0 < zi; i0 += kSpan ) {
             C w[ PER ];
             uint32_t dw[ PER / 2 ];
-            get_pair< C, PER >( prod, dest, i0, w, dw );
+            get_pair< C, PER >( prod, dest, i0, di, w, dw );
             put_pair< C, PER >( buf, w, dw, i0, zi, ai );
          }
       }
@@ -502,16 +520,43 @@ void build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
                                                   bits, s ) );
    W.wstart.allocate( W.K + 2 );
    hipLaunchKernelGGL( w_starts, g( nnz + 1 ), b256, 0, s, nnz, W.K, skey.ptr, W.wstart.ptr );
-   W.pv.allocate( std::max< int64_t >( 1, nnz ) * (int64_t) sizeof( V ) );
-   W.pc.allocate( std::max< int64_t >( 1, nnz ) );
-   W.dest.allocate( nnz + kPad );
-   hipLaunchKernelGGL( ( w_fill< V > ), g( nnz ), b256, 0, s, nnz, W.win, perm.ptr, cols, vals, W.B, dE.ptr,
-                       (V*) W.pv.ptr, W.pc.ptr, W.dest.ptr );
    W.seg.allocate( ( W.B + 1 ) * W.K );
    hipLaunchKernelGGL( w_seg, g( ( W.B + 1 ) * W.K ), b256, 0, s, W.B, W.K, dE.ptr, W.wstart.ptr, perm.ptr,
                        W.seg.ptr );
+   // the slot array, block-major: chunk (b, k) (its window-k nonzeros, in
+   // prod at [seg[b K + k], seg[(b + 1) K + k])) gets the 4-slot groups
+   // [c, c + ceil((r + len) / 4)) after the block's previous chunk, r = its
+   // first index mod 4, and its slots start at 4 c + r, so a phase-2 read of
+   // a chunk's slots has the alignment of its read of the products, and a
+   // row block's slots are one contiguous run.  TNLMI_WINDOW_SLOTS=window
+   // keeps them in prod's order (dbase 0).
+   std::vector< int32_t > dbase( std::max< int64_t >( 1, W.B * W.K ), 0 );
+   int64_t slots = nnz;
+   const char* lay = std::getenv( "TNLMI_WINDOW_SLOTS" );
+   if( ! ( lay && std::string( lay ) == "window" ) && W.B > 0 ) {
+      std::vector< int32_t > hseg( ( W.B + 1 ) * W.K );
+      TNLMI_HIP( hipMemcpyAsync( hseg.data(), W.seg.ptr, sizeof( int32_t ) * hseg.size(), hipMemcpyDeviceToHost,
+                                 s ) );
+      TNLMI_HIP( hipStreamSynchronize( s ) );
+      int64_t c = kPad / 4;  // reads may start up to 3 slots before a chunk
+      for( int64_t b = 0; b < W.B; b++ )
+         for( int64_t k = 0; k < W.K; k++ ) {
+            const int64_t a = hseg[ b * W.K + k ], len = hseg[ ( b + 1 ) * W.K + k ] - a, r = a % 4;
+            dbase[ b * W.K + k ] = (int32_t) ( 4 * c + r - a );
+            c += ( r + len + 3 ) / 4;
+         }
+      slots = 4 * c;
+      if( slots + kPad > INT32_MAX )
+         fail( TNLMI_NOT_IMPLEMENTED, "windowed SpMV plan: more than 2^31 slot entries" );
+   }
+   upload( W.dbase, dbase );
+   W.dest.allocate( slots + kPad );
+   TNLMI_HIP( hipMemsetAsync( W.dest.ptr, 0, sizeof( uint16_t ) * ( slots + kPad ), s ) );
+   W.pv.allocate( std::max< int64_t >( 1, nnz ) * (int64_t) sizeof( V ) );
+   W.pc.allocate( std::max< int64_t >( 1, nnz ) );
+   hipLaunchKernelGGL( ( w_fill< V > ), g( nnz ), b256, 0, s, nnz, W.win, perm.ptr, cols, vals, W.B, dE.ptr,
+                       skey.ptr, W.K, W.dbase.ptr, (V*) W.pv.ptr, W.pc.ptr, W.dest.ptr );
    W.prod.allocate( ( nnz + kPad ) * (int64_t) sizeof( V ) );
-   TNLMI_HIP( hipMemsetAsync( W.dest.ptr + nnz, 0, sizeof( uint16_t ) * kPad, s ) );
    TNLMI_HIP( hipMemsetAsync( W.prod.ptr + nnz * (int64_t) sizeof( V ), 0, sizeof( V ) * kPad, s ) );
    TNLMI_HIP( hipGetLastError() );
    TNLMI_HIP( hipStreamSynchronize( s ) );
@@ -535,14 +580,14 @@ void phase2( const WindowPlan& W, C* y, double alpha, double beta, int64_t begin
       hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, true >
                                         : w_phase2< C, cap_of< C >(), true, SEGS, PER, true > ),
                           dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
-                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
-                          (C) beta, begin, end );
+                          (const C*) W.prod.ptr, W.dest.ptr, W.dbase.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
+                          (C) beta, begin, end, (int32_t) W.across );
    else
       hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, false >
                                         : w_phase2< C, cap_of< C >(), true, SEGS, PER, false > ),
                           dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
-                          (const C*) W.prod.ptr, W.dest.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
-                          (C) beta, begin, end );
+                          (const C*) W.prod.ptr, W.dest.ptr, W.dbase.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
+                          (C) beta, begin, end, (int32_t) W.across );
 }
 
 template< typename C >
@@ -604,6 +649,11 @@ void window_build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    W.align = true;
    if( const char* e = std::getenv( "TNLMI_WINDOW_ALIGN" ) )
       W.align = e[ 0 ] != '0';
+   // TNLMI_WINDOW_ROUNDS=grouped: phase-2 groups read their own consecutive
+   // segments round by round instead of a wave reading adjacent ones
+   W.across = true;
+   if( const char* e = std::getenv( "TNLMI_WINDOW_ROUNDS" ) )
+      W.across = std::string( e ) != "grouped";
    W.offs.allocate( A.rows + 1 );
    TNLMI_HIP( hipMemcpyAsync( W.offs.ptr, A.segments->offsets.ptr, sizeof( int32_t ) * ( A.rows + 1 ),
                               hipMemcpyDeviceToDevice, s ) );

@@ -14,6 +14,7 @@ struct WindowPlan
    int32_t per = 2;     // phase 2: consecutive terms per lane and load
    int32_t unroll = 4;  // phase 1: nonzeros per thread and batch
    bool align = true;    // phase 2 reads products and slots from aligned positions
+   bool across = true;   // phase 2: a wave's groups read adjacent windows' chunks in each round
    int32_t K = 0;    // column windows
    int64_t B = 0;     // row blocks
    int64_t H = 0;     // rows longer than cap (cut into pieces)
@@ -28,7 +29,9 @@ struct WindowPlan
    DeviceArray< char > part;       // B: partial sums of the piece blocks (compute type)
    DeviceArray< char > pv;         // nonzeros sorted by window (rows ascending inside), value type
    DeviceArray< uint16_t > pc;     // their column inside the window
-   DeviceArray< uint16_t > dest;   // their CSR slot inside their row block
+   DeviceArray< uint16_t > dest;   // CSR slot inside its row block of each product, block-major: a row
+                                   // block's chunks (one per window) side by side
+   DeviceArray< int32_t > dbase;   // B x K: slot of chunk (b, k)'s first product minus its index in prod
    DeviceArray< char > prod;       // phase-1 products (compute type)
 };
 
