@@ -379,6 +379,28 @@ def test_windowed_plan_matches_tnl_bit_for_bit(orc):
         assert np.array_equal(got[short], ref.spmv("CSR", off, cols, vals, x)[short])
 
 
+@pytest.mark.parametrize("slots,rounds", [("block", "across"), ("window", "grouped"), ("block", "grouped")])
+def test_windowed_plan_more_windows_than_a_sweep(orc, monkeypatch, slots, rounds):
+    # 611 column windows: phase 2 reads windows past its 512-window sweep on
+    # demand; most chunks are empty or hold one or two products.  Every slot
+    # layout and read order gives TNL's host loop exactly (rows <= 256)
+    monkeypatch.setenv("TNLMI_WINDOW_SLOTS", slots)
+    monkeypatch.setenv("TNLMI_WINDOW_ROUNDS", rounds)
+    rng = np.random.default_rng(8)
+    n, m = 20000, 5_000_000
+    off, cols, vals = random_csr(n, m, 40, rng, long_rows=[(3, 2000), (19999, 300)])
+    A = tnl.SparseMatrix.from_csr(n, m, off, cols, vals)
+    P = tnl.SpmvPlan(A, 0)
+    x = rng.standard_normal(m)
+    yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.vectorProduct(gpu(x), yd)
+    torch.cuda.synchronize()
+    got, want = yd.cpu().numpy(), orc.spmv(off, cols, vals, x)
+    short = np.diff(off) <= 256
+    assert np.array_equal(got[short], want[short])
+    assert relerr(got[~short], want[~short]) <= 1e-13
+
+
 def test_windowed_plan_fp32_and_errors(orc):
     rng = np.random.default_rng(6)
     n, m = 5000, 20000
