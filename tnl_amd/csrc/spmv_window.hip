@@ -177,16 +177,17 @@ __global__ void __launch_bounds__( kP1 ) w_phase1( int64_t ncols, const int64_t*
 
 // phase 2: row block b -> LDS in CSR order -> row sums -> y.  One workgroup
 // per row block.  The products of a block's segment (window k) are loaded by
-// a group of kGrp lanes together, PER consecutive terms each: one load
-// instruction covers kGrp * PER terms of a segment, so the lanes of a group
+// a group of GRP lanes together, PER consecutive terms each: one load
+// instruction covers GRP * PER terms of a segment, so the lanes of a group
 // touch consecutive cache lines (a lane per segment would touch a line of its
 // own in every load, and the L1 tag lookups, one per line, would bound the
-// kernel).  All rounds of a thread (kP2 / kGrp groups, R = SEGS / (kP2 /
-// kGrp) segments each) are issued before the first is stored.  SEGS, the
-// segments (windows) per sweep of a block, is 512, or 256 for plans of at
-// most 256 windows.  (1024-thread workgroups at 64 VGPRs, two per CU,
+// kernel).  All rounds of a thread (kP2 / GRP groups, R = SEGS / (kP2 /
+// GRP) segments each) are issued before the first is stored.  SEGS, the
+// segments (windows) per sweep of a block, is 512 (default shape: 8-lane
+// groups of 4 terms per lane), or 256 for plans of at most 256 windows
+// (16-lane groups of 2).  (1024-thread workgroups at 64 VGPRs, two per CU,
 // measured no faster than 512.)
-constexpr int kGrp = 16;
+constexpr int kGrp = 16;  // the default group width (GRP)
 
 // per-block header: 8 int32 -- rows [r0, r1), nonzeros [e0, e1), the long
 // row a piece belongs to (-1: a run of whole rows), long-row list [l0, l1)
@@ -243,7 +244,7 @@ __device__ __forceinline__ void put_pair( C* buf, const C ( &v )[ PER ], const u
          buf[ ( d[ u / 2 ] >> ( 16 * ( u % 2 ) ) ) & 0xffff ] = v[ u ];
 }
 
-template< typename C, int CAP, bool BETA, int SEGS, int PER, bool ALIGN >
+template< typename C, int CAP, bool BETA, int SEGS, int PER, bool ALIGN, int GRP = kGrp >
 __global__ void __launch_bounds__( kP2 )
    w_phase2( int64_t B, int32_t K, const BlockHdr* __restrict__ hdr, const int32_t* __restrict__ seg,
              const C* __restrict__ prod, const uint16_t* __restrict__ dest, const int32_t* __restrict__ dbase,
@@ -255,10 +256,10 @@ __global__ void __launch_bounds__( kP2 )
    __shared__ C buf[ CAP ];
    constexpr int P2 = kP2;
    __shared__ C red[ P2 / 64 ];
-   // SEGS segments per sweep: P2 / kGrp groups of R segments each
-   constexpr int R = SEGS * kGrp / P2;
-   constexpr int kSpan = kGrp * PER;  // terms of a segment per load round
-   static_assert( R >= 1 && R <= kGrp, "a group's lanes hold its segments' bounds" );
+   // SEGS segments per sweep: P2 / GRP groups of R segments each
+   constexpr int R = SEGS * GRP / P2;
+   constexpr int kSpan = GRP * PER;  // terms of a segment per load round
+   static_assert( R >= 1 && R <= GRP, "a group's lanes hold its segments' bounds" );
    // a read of PER terms starts at most at a segment's last term (or, for an
    // empty segment, its start, at most nnz): it ends inside the kPad padding
    static_assert( PER % 2 == 0 && PER <= kPad, "slot words hold two slots; reads stay in the padding" );
@@ -270,15 +271,15 @@ __global__ void __launch_bounds__( kP2 )
       return;  // no row of this block is asked for
    const int t = threadIdx.x;
    const int lane = t & 63, wid = wave_uniform( (int32_t) ( t >> 6 ) );
-   const int gl = lane % kGrp, gb = lane - gl;
+   const int gl = lane % GRP, gb = lane - gl;
    // segment sx (window sx) of the block: its bounds, held by lane gb + i
    // (i < R) of the group's lanes.  A wave takes G R consecutive segments;
    // across: in round i its G groups read segments G i .. G i + G - 1 of
    // them, whose slots lie side by side (the slot array is block-major);
    // else group g reads segment g R + i
-   constexpr int G = 64 / kGrp;
+   constexpr int G = 64 / GRP;
    const int32_t sx = wid * G * R
-                      + ( across ? std::min( gl, R - 1 ) * G + lane / kGrp : ( lane / kGrp ) * R + std::min( gl, R - 1 ) );
+                      + ( across ? std::min( gl, R - 1 ) * G + lane / GRP : ( lane / GRP ) * R + std::min( gl, R - 1 ) );
    const int32_t kt = sx < K ? sx : K - 1;
    // both bounds loaded unconditionally (a load under the condition would
    // wait for the first before it is issued)
@@ -573,18 +574,18 @@ void phase1( const WindowPlan& W, const C* x, hipStream_t s )
                           W.wstart.ptr, (const C*) W.pv.ptr, W.pc.ptr, x, (C*) W.prod.ptr );
 }
 
-template< typename C, int SEGS, int PER >
+template< typename C, int SEGS, int PER, int GRP = kGrp >
 void phase2( const WindowPlan& W, C* y, double alpha, double beta, int64_t begin, int64_t end, hipStream_t s )
 {
    if( W.align )
-      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, true >
-                                        : w_phase2< C, cap_of< C >(), true, SEGS, PER, true > ),
+      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, true, GRP >
+                                        : w_phase2< C, cap_of< C >(), true, SEGS, PER, true, GRP > ),
                           dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
                           (const C*) W.prod.ptr, W.dest.ptr, W.dbase.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
                           (C) beta, begin, end, (int32_t) W.across );
    else
-      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, false >
-                                        : w_phase2< C, cap_of< C >(), true, SEGS, PER, false > ),
+      hipLaunchKernelGGL( ( beta == 0.0 ? w_phase2< C, cap_of< C >(), false, SEGS, PER, false, GRP >
+                                        : w_phase2< C, cap_of< C >(), true, SEGS, PER, false, GRP > ),
                           dim3( (unsigned) W.B ), dim3( kP2 ), 0, s, W.B, W.K, (const BlockHdr*) W.hdr.ptr, W.seg.ptr,
                           (const C*) W.prod.ptr, W.dest.ptr, W.dbase.ptr, W.offs.ptr, W.lrows.ptr, W.rorder.ptr, (C*) W.part.ptr, y, (C) alpha,
                           (C) beta, begin, end, (int32_t) W.across );
@@ -609,6 +610,7 @@ void run( const WindowPlan& W, const C* x, C* y, double alpha, double beta, int6
          case 5122: phase2< C, 512, 2 >( W, y, alpha, beta, begin, end, s ); break;
          case 2562: phase2< C, 256, 2 >( W, y, alpha, beta, begin, end, s ); break;
          case 2564: phase2< C, 256, 4 >( W, y, alpha, beta, begin, end, s ); break;
+         case 5124: phase2< C, 512, 4, 8 >( W, y, alpha, beta, begin, end, s ); break;  // 8-lane groups
          default: fail( TNLMI_RUNTIME_ERROR, "windowed SpMV plan: unsupported phase-2 shape" );
       }
    }
@@ -636,10 +638,13 @@ void window_build( WindowPlan& W, const tnlmi_matrix& A, hipStream_t s )
    }
    W.K = (int32_t) std::max< int64_t >( 1, ceil_div( A.columns, W.win ) );
    W.segs = W.K <= 256 ? 256 : 512;
-   W.per = 2;
+   // phase 2 with 512 windows per sweep: 8-lane groups of 4 terms per lane
+   // (449 against 455 us per power-law SpMV with 16-lane groups of 2 terms,
+   // 460 with 4-lane groups of 8); with 256: 16-lane groups of 2 terms
+   W.per = W.segs == 512 ? 4 : 2;
    if( const char* e = std::getenv( "TNLMI_WINDOW_PER" ) )
-      if( std::atoi( e ) == 4 && W.segs == 256 )
-         W.per = 4;
+      if( std::atoi( e ) == 2 || std::atoi( e ) == 4 )
+         W.per = std::atoi( e );
    W.unroll = 4;
    if( const char* e = std::getenv( "TNLMI_WINDOW_UNROLL" ) )
       if( std::atoi( e ) == 8 )

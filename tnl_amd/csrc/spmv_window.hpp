@@ -11,7 +11,7 @@ struct WindowPlan
    tnlmi_dtype value_type = TNLMI_F64;
    int32_t win = 8192;  // columns per window (x entries staged in LDS by phase 1)
    int32_t segs = 512;  // phase 2: windows per sweep of a row block (256 or 512)
-   int32_t per = 2;     // phase 2: consecutive terms per lane and load
+   int32_t per = 4;     // phase 2: consecutive terms per lane and load (4: 8-lane groups; 2: 16-lane)
    int32_t unroll = 4;  // phase 1: nonzeros per thread and batch
    bool align = true;    // phase 2 reads products and slots from aligned positions
    bool across = true;   // phase 2: a wave's groups read adjacent windows' chunks in each round
