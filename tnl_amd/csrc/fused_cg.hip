@@ -8,7 +8,8 @@
 //              partial sums of (r, z) [and (r, r)]
 //   cg_dir     beta, the residue, TNL's stopping rule (IterativeSolver.hpp:98-121)
 //              and the iteration counter;  p = r / d + beta p
-// Every block of a kernel reduces the previous kernels' 1024 partial sums
+// Every block of a kernel reduces the previous kernels' partial sums (one
+// per block: 1024 in the FP64 CG, 2048 in the mixed solves' FP32 iterations)
 // in the same fixed order, so the scalars are identical in every block and
 // from run to run.  The iteration state lives on the device, double-buffered
 // by iteration parity (iteration k reads state k & 1, block 0 of its last
@@ -43,10 +44,23 @@
 namespace tnlmi {
 namespace {
 
-constexpr int kFB = 1024;  // blocks of every fused vector kernel = number of partial sums
+// Blocks of a fused vector kernel = its number of partial sums: 1024 for the
+// FP64 CG, 2048 for the FP32 iterations of the mixed solves (their FP32
+// vectors move half the bytes per row, and twice the waves keep enough loads
+// in flight: reliable BF16 CG on Poisson 256^3 0.281 against 0.320 s; the FP64
+// CG measured 2 % slower with 2048, its blocks then summing twice the partials
+// first).  kFB: the length of every partial-sum array.
+constexpr int kFB = 2048;
+template< typename T >
+constexpr int blocks_of()
+{
+   return sizeof( T ) == 8 ? 1024 : 2048;
+}
+constexpr int kFB32 = blocks_of< float >();  // the mixed solve's kernels
 constexpr int kFT = 256;
 constexpr int kBatch = 8;  // iterations enqueued between two host reads of the state
-static_assert( kFB % kXcds == 0, "the SpMV kernels split the grid evenly over the XCDs" );
+static_assert( blocks_of< double >() % kXcds == 0 && blocks_of< float >() % kXcds == 0 && blocks_of< float >() <= kFB,
+               "grids split evenly over the XCDs, partial sums fit their arrays" );
 
 struct CgDev
 {
@@ -73,11 +87,11 @@ __device__ __forceinline__ bool check_next( const StopRule& R, double res, int64
    return ! ( isnan( res ) || it > R.maxit || ( res > R.div && it >= R.minit ) || ( res < R.tol && it >= R.minit ) );
 }
 
-// Sum of kFB partial sums in a fixed order; the same value in every block.
-__device__ __forceinline__ double sum_parts( const double* __restrict__ part, double* sm )
+// Sum of np partial sums in a fixed order; the same value in every block.
+__device__ __forceinline__ double sum_parts( const double* __restrict__ part, double* sm, int np )
 {
    double a = 0;
-   for( int i = threadIdx.x; i < kFB; i += kFT )
+   for( int i = threadIdx.x; i < np; i += kFT )
       a += part[ i ];
    return block_sum< kFT >( a, sm );
 }
@@ -86,13 +100,14 @@ __device__ __forceinline__ double sum_parts( const double* __restrict__ part, do
 // of barriers, each total summed in the same order as sum_parts (so equal to
 // it bit for bit).  sm holds N * kFT / 64 doubles.
 template< int N >
-__device__ __forceinline__ void sum_parts_n( const double* const ( &part )[ N ], double* sm, double ( &out )[ N ] )
+__device__ __forceinline__ void sum_parts_n( const double* const ( &part )[ N ], double* sm, double ( &out )[ N ],
+                                             int np )
 {
    double a[ N ];
 #pragma unroll
    for( int j = 0; j < N; j++ )
       a[ j ] = 0;
-   for( int i = threadIdx.x; i < kFB; i += kFT )
+   for( int i = threadIdx.x; i < np; i += kFT )
 #pragma unroll
       for( int j = 0; j < N; j++ )
          a[ j ] += part[ j ][ i ];
@@ -128,7 +143,6 @@ __device__ __forceinline__ void put_part( double v, double* sm, double* part )
 // Loops stride over the grid kU rows / elements at a time, so that every
 // thread has kU independent loads in flight.
 constexpr int kU = 4;
-constexpr int64_t kStride = (int64_t) kFB * kFT;
 
 // Ellpack / SlicedEllpack, column-major: each thread R adjacent rows, whose
 // slots are adjacent in every slot column (vector loads of R column indexes
@@ -447,15 +461,15 @@ __global__ void __launch_bounds__( kFT ) code_encode( SegView v, const int32_t* 
    }
 }
 
-// The SpMV kernels' nin block partial sums folded, in order, into the kFB
-// partial sums the following kernels read.
+// The SpMV kernels' nin block partial sums folded, in order, into the
+// gridDim.x partial sums the following kernels read.
 __global__ void __launch_bounds__( kFT ) fold_parts( const double* __restrict__ in, int64_t nin,
                                                      double* __restrict__ out, const CgDev* __restrict__ st )
 {
    __shared__ double sm[ kFT / 64 ];
    if( ! st->active )
       return;
-   const int64_t chunk = ( nin + kFB - 1 ) / kFB;
+   const int64_t chunk = ( nin + gridDim.x - 1 ) / gridDim.x;
    const int64_t b = blockIdx.x * chunk, e = min( nin, b + chunk );
    double a = 0;
    for( int64_t i = b + threadIdx.x; i < e; i += kFT )
@@ -472,10 +486,10 @@ __global__ void __launch_bounds__( kFT ) dot_part( int64_t n, const T* __restric
    if( ! st->active )
       return;
    double acc = 0;
-   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * (int64_t) gridDim.x * kFT )
 #pragma unroll
       for( int u = 0; u < kU; u++ ) {
-         const int64_t i = i0 + u * kStride;
+         const int64_t i = i0 + u * (int64_t) gridDim.x * kFT;
          if( i < n )
             acc += (double) a[ i ] * (double) b[ i ];
       }
@@ -494,15 +508,15 @@ __global__ void __launch_bounds__( kFT ) cg_upd( int64_t n, const double* __rest
    __shared__ double sm[ kFT / 64 ];
    if( ! st->active )
       return;
-   const double s2 = sum_parts( part1, sm );
+   const double s2 = sum_parts( part1, sm, (int) gridDim.x );  // the producers' grid is this grid
    if( s2 == 0.0 )
       return;  // cg_dir ends the solve
    const T alpha = (T) ( st->s1 / s2 );
    double a2 = 0, a3 = 0;
-   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * (int64_t) gridDim.x * kFT )
 #pragma unroll
       for( int u = 0; u < kU; u++ ) {
-         const int64_t i = i0 + u * kStride;
+         const int64_t i = i0 + u * (int64_t) gridDim.x * kFT;
          if( i < n ) {
             const T pi = p[ i ];
             x[ i ] += (X) ( alpha * pi );
@@ -539,12 +553,12 @@ __global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __rest
    double t[ 3 ] = { 0, 0, 0 };
    if( R.kind == 1 ) {
       double t3[ 3 ];
-      sum_parts_n< 3 >( { part1, part2, part3 }, sm, t3 );
+      sum_parts_n< 3 >( { part1, part2, part3 }, sm, t3, (int) gridDim.x );
       t[ 0 ] = t3[ 0 ], t[ 1 ] = t3[ 1 ], t[ 2 ] = t3[ 2 ];
    }
    else {
       double t2[ 2 ];
-      sum_parts_n< 2 >( { part1, part2 }, sm, t2 );
+      sum_parts_n< 2 >( { part1, part2 }, sm, t2, (int) gridDim.x );
       t[ 0 ] = t2[ 0 ], t[ 1 ] = t2[ 1 ];
    }
    const double s2 = t[ 0 ];
@@ -573,10 +587,10 @@ __global__ void __launch_bounds__( kFT ) cg_dir( int64_t n, const double* __rest
    if( ! go )
       return;
    const T tb = (T) beta;
-   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * (int64_t) gridDim.x * kFT )
 #pragma unroll
       for( int u = 0; u < kU; u++ ) {
-         const int64_t i = i0 + u * kStride;
+         const int64_t i = i0 + u * (int64_t) gridDim.x * kFT;
          if( i < n ) {
             const T ri = r[ i ];
             p[ i ] = ( d ? ri / (T) d[ i ] : ri ) + tb * p[ i ];
@@ -601,7 +615,7 @@ __global__ void __launch_bounds__( kFT ) mx_dir( int64_t n, const double* __rest
       return;
    }
    double t[ 3 ];
-   sum_parts_n< 3 >( { part1, part2, part3 }, sm, t );
+   sum_parts_n< 3 >( { part1, part2, part3 }, sm, t, (int) gridDim.x );
    const double s2 = t[ 0 ], s1n = t[ 1 ], rr = t[ 2 ];
    const double s1 = cur->s1;
    const int64_t it = cur->it + 1;
@@ -626,10 +640,10 @@ __global__ void __launch_bounds__( kFT ) mx_dir( int64_t n, const double* __rest
    if( want )
       return;
    const float tb = (float) beta;
-   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * kStride )
+   for( int64_t i0 = (int64_t) blockIdx.x * kFT + threadIdx.x; i0 < n; i0 += kU * (int64_t) gridDim.x * kFT )
 #pragma unroll
       for( int u = 0; u < kU; u++ ) {
-         const int64_t i = i0 + u * kStride;
+         const int64_t i = i0 + u * (int64_t) gridDim.x * kFT;
          if( i < n )
             p[ i ] = r[ i ] / d[ i ] + tb * p[ i ];
       }
@@ -654,7 +668,7 @@ __global__ void __launch_bounds__( kFT ) resid_part( int64_t n, const double* __
 {
    __shared__ double sm[ kFT / 64 ];
    double acc = 0;
-   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT ) {
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) gridDim.x * kFT ) {
       const double ri = b[ i ] - r[ i ];
       r[ i ] = ri;
       acc += ri * ri;
@@ -662,10 +676,10 @@ __global__ void __launch_bounds__( kFT ) resid_part( int64_t n, const double* __
    put_part( acc, sm, part );
 }
 
-__global__ void __launch_bounds__( kFT ) sum_into( const double* __restrict__ part, double* out )
+__global__ void __launch_bounds__( kFT ) sum_into( const double* __restrict__ part, int np, double* out )
 {
    __shared__ double sm[ kFT / 64 ];
-   const double v = sum_parts( part, sm );
+   const double v = sum_parts( part, sm, np );
    if( threadIdx.x == 0 )
       out[ 0 ] = v;
 }
@@ -680,7 +694,7 @@ __global__ void __launch_bounds__( kFT ) mx_restart( int64_t n, const double* __
 {
    __shared__ double sm[ kFT / 64 ];
    double a2 = 0, a3 = 0, a4 = 0;
-   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT ) {
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) gridDim.x * kFT ) {
       const float ri = (float) r64[ i ];
       const float zi = ri / d[ i ];
       r[ i ] = ri;
@@ -700,12 +714,12 @@ __global__ void __launch_bounds__( kFT ) mx_restart( int64_t n, const double* __
 // beta = (r, z) / (r, z)_previous -- the residual replacement of Clark et al.
 __global__ void __launch_bounds__( kFT ) mx_state( CgDev* st, const double* __restrict__ part2,
                                                    const double* __restrict__ part3, const double* __restrict__ part4,
-                                                   const double* normb, int64_t it, int first )
+                                                   const double* normb, int64_t it, int first, int np )
 {
    __shared__ double sm[ kFT / 64 ];
-   const double s1 = sum_parts( part2, sm );
-   const double rr = sum_parts( part3, sm );
-   const double rp = sum_parts( part4, sm );
+   const double s1 = sum_parts( part2, sm, np );
+   const double rr = sum_parts( part3, sm, np );
+   const double rp = sum_parts( part4, sm, np );
    if( threadIdx.x != 0 )
       return;
    if( first ) {
@@ -730,7 +744,7 @@ __global__ void __launch_bounds__( kFT ) mx_redir( int64_t n, const float* __res
                                                    float* __restrict__ p, const CgDev* __restrict__ st, int first )
 {
    const float beta = (float) st->beta, proj = (float) st->proj;
-   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT ) {
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) gridDim.x * kFT ) {
       const float zi = r[ i ] / d[ i ];
       p[ i ] = first ? zi : zi + beta * ( p[ i ] - proj * zi );
    }
@@ -739,14 +753,14 @@ __global__ void __launch_bounds__( kFT ) mx_redir( int64_t n, const float* __res
 // x += xc (FP64 accumulation of the FP32 correction)
 __global__ void __launch_bounds__( kFT ) mx_accumulate( int64_t n, const float* __restrict__ xc, double* __restrict__ x )
 {
-   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT )
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) gridDim.x * kFT )
       x[ i ] += (double) xc[ i ];
 }
 
 template< typename D >
 __global__ void to_f32( int64_t n, const D* __restrict__ a, float* __restrict__ o )
 {
-   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) kFB * kFT )
+   for( int64_t i = (int64_t) blockIdx.x * kFT + threadIdx.x; i < n; i += (int64_t) gridDim.x * kFT )
       o[ i ] = (float) a[ i ];
 }
 
@@ -1003,7 +1017,7 @@ void stage_launch_v( const Stage& S, const T* p, T* Ap, double* part, const CgDe
          hipLaunchKernelGGL( ( spmv_dot_ell< 1, V, T > ), dim3( (unsigned) S.grid ), dim3( kFT ), 0, s, S.v,
                              S.cols, vals, p, Ap, bp, st );
    }
-   hipLaunchKernelGGL( fold_parts, dim3( kFB ), dim3( kFT ), 0, s, bp, S.grid, part, st );
+   hipLaunchKernelGGL( fold_parts, dim3( blocks_of< T >() ), dim3( kFT ), 0, s, bp, S.grid, part, st );
 }
 
 // Ap = A p and the partial sums of (Ap, p)
@@ -1018,7 +1032,7 @@ void stage_launch( const Stage& S, const T* p, T* Ap, double* part, const CgDev*
          const tnlmi_launch cfg{ TNLMI_MAP_AUTO, 1 };
          spmv_dispatch( *S.A, xt, p, Ap, xt, 1.0, 0.0, 0, S.v.n, cfg, s );
       }
-      hipLaunchKernelGGL( dot_part< T >, dim3( kFB ), dim3( kFT ), 0, s, S.v.n, Ap, p, part, st );
+      hipLaunchKernelGGL( dot_part< T >, dim3( blocks_of< T >() ), dim3( kFT ), 0, s, S.v.n, Ap, p, part, st );
       return;
    }
    switch( S.A->value_type ) {
@@ -1119,9 +1133,11 @@ void fused_cg_solve( const Operator& A, const tnlmi_preconditioner* M, const dou
          CgDev* cur = dev.ptr + par;
          stage_launch< double >( S, p.ptr, Ap.ptr, parts.a.ptr, cur, s );
          double* part3 = residue_kind == 1 ? parts.c.ptr : nullptr;
-         hipLaunchKernelGGL( ( cg_upd< double, double, double > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
+         hipLaunchKernelGGL( ( cg_upd< double, double, double > ), dim3( blocks_of< double >() ), dim3( kFT ), 0, s, n,
+                             parts.a.ptr,
                              r.ptr, p.ptr, Ap.ptr, x, d, parts.b.ptr, part3, cur );
-         hipLaunchKernelGGL( ( cg_dir< double, double > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
+         hipLaunchKernelGGL( ( cg_dir< double, double > ), dim3( blocks_of< double >() ), dim3( kFT ), 0, s, n,
+                             parts.a.ptr,
                              parts.b.ptr, part3, r.ptr, d, p.ptr, cur, dev.ptr + ( par ^ 1 ), R );
       }
       TNLMI_HIP( hipGetLastError() );
@@ -1178,7 +1194,7 @@ tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_
          const tnlmi_status rc = tnlmi_preconditioner_create( TNLMI_PRECOND_DIAGONAL, A, stream, &M );
          if( rc != TNLMI_OK )
             fail( rc, tnlmi_last_error() );
-         hipLaunchKernelGGL( to_f32< double >, dim3( kFB ), dim3( kFT ), 0, s, n, precond_diagonal( M ), d32.ptr );
+         hipLaunchKernelGGL( to_f32< double >, dim3( kFB32 ), dim3( kFT ), 0, s, n, precond_diagonal( M ), d32.ptr );
          TNLMI_HIP( hipStreamSynchronize( s ) );
          tnlmi_preconditioner_destroy( M );
       }
@@ -1205,8 +1221,8 @@ tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_
       while( true ) {
          // reliable update: r = b - A x in FP64 and the true residual
          op.apply( d_x, r64.ptr, s );
-         hipLaunchKernelGGL( resid_part, dim3( kFB ), dim3( kFT ), 0, s, n, d_b, r64.ptr, parts.c.ptr );
-         hipLaunchKernelGGL( sum_into, dim3( 1 ), dim3( kFT ), 0, s, parts.c.ptr, sc.ptr + 1 );
+         hipLaunchKernelGGL( resid_part, dim3( kFB32 ), dim3( kFT ), 0, s, n, d_b, r64.ptr, parts.c.ptr );
+         hipLaunchKernelGGL( sum_into, dim3( 1 ), dim3( kFT ), 0, s, parts.c.ptr, kFB32, sc.ptr + 1 );
          double rr = 0;
          TNLMI_HIP( hipMemcpyAsync( &rr, sc.ptr + 1, sizeof( double ), hipMemcpyDeviceToHost, s ) );
          TNLMI_HIP( hipStreamSynchronize( s ) );
@@ -1216,11 +1232,11 @@ tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_
             break;
          // replace the FP32 residual by the FP64 one and form the next direction
          CgDev* cur = dev.ptr + par;
-         hipLaunchKernelGGL( mx_restart, dim3( kFB ), dim3( kFT ), 0, s, n, r64.ptr, r.ptr, d32.ptr, p.ptr, xc.ptr,
+         hipLaunchKernelGGL( mx_restart, dim3( kFB32 ), dim3( kFT ), 0, s, n, r64.ptr, r.ptr, d32.ptr, p.ptr, xc.ptr,
                              parts.a.ptr, parts.b.ptr, parts.c.ptr );
          hipLaunchKernelGGL( mx_state, dim3( 1 ), dim3( kFT ), 0, s, cur, parts.a.ptr, parts.b.ptr, parts.c.ptr,
-                             sc.ptr + 0, it, first ? 1 : 0 );
-         hipLaunchKernelGGL( mx_redir, dim3( kFB ), dim3( kFT ), 0, s, n, r.ptr, d32.ptr, p.ptr, cur,
+                             sc.ptr + 0, it, first ? 1 : 0, kFB32 );
+         hipLaunchKernelGGL( mx_redir, dim3( kFB32 ), dim3( kFT ), 0, s, n, r.ptr, d32.ptr, p.ptr, cur,
                              first ? 1 : 0 );
          first = false;
          const CgDev* h = nullptr;
@@ -1228,16 +1244,16 @@ tnlmi_status tnlmi_mixed_cg_solve( const tnlmi_matrix* A, const tnlmi_matrix* A_
             for( int k = 0; k < kBatch; k++, par ^= 1 ) {
                cur = dev.ptr + par;
                stage_launch< float >( S, p.ptr, Ap.ptr, parts.a.ptr, cur, s );
-               hipLaunchKernelGGL( ( cg_upd< float, float, float > ), dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr,
+               hipLaunchKernelGGL( ( cg_upd< float, float, float > ), dim3( kFB32 ), dim3( kFT ), 0, s, n, parts.a.ptr,
                                    r.ptr, p.ptr, Ap.ptr, xc.ptr, d32.ptr, parts.b.ptr, parts.c.ptr, cur );
-               hipLaunchKernelGGL( mx_dir, dim3( kFB ), dim3( kFT ), 0, s, n, parts.a.ptr, parts.b.ptr, parts.c.ptr,
+               hipLaunchKernelGGL( mx_dir, dim3( kFB32 ), dim3( kFT ), 0, s, n, parts.a.ptr, parts.b.ptr, parts.c.ptr,
                                    r.ptr, d32.ptr, p.ptr, cur, dev.ptr + ( par ^ 1 ), delta * delta, R.tol, R.maxit );
             }
             TNLMI_HIP( hipGetLastError() );
             h = &hs.read( dev.ptr + par, s );
          } while( h->active );
          it = h->it;
-         hipLaunchKernelGGL( mx_accumulate, dim3( kFB ), dim3( kFT ), 0, s, n, xc.ptr, d_x );
+         hipLaunchKernelGGL( mx_accumulate, dim3( kFB32 ), dim3( kFT ), 0, s, n, xc.ptr, d_x );
       }
       st->iterations = it;
       st->residue = res;
