@@ -104,12 +104,18 @@ __global__ void __launch_bounds__( 256 ) multi_dot_pass1( int64_t n, int cols, c
    for( int k = 0; k < kDotGroup; k++ )
       acc[ k ] = 0;
    const int64_t stride = (int64_t) gridDim.x * 256;
+   // every column load is issued (a group's columns past cols read its last
+   // column again and add nothing): loads under a condition would each wait
+   // for the ones before them
    for( int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += stride ) {
       const double wi = w[ i ];
+      double v[ kDotGroup ];
 #pragma unroll
       for( int k = 0; k < kDotGroup; k++ )
-         if( k < nc )
-            acc[ k ] += Vc[ i + k * ld ] * wi;
+         v[ k ] = Vc[ i + (int64_t) min( k, nc - 1 ) * ld ];
+#pragma unroll
+      for( int k = 0; k < kDotGroup; k++ )
+         acc[ k ] = fma( k < nc ? v[ k ] : 0.0, wi, acc[ k ] );  // (the contracted multiply-add)
    }
    // every column's wave total at once, then one barrier and a fixed-order
    // sum of the four waves' totals
